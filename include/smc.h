@@ -1,0 +1,202 @@
+/*
+ * smc.h -- C-ABI of libsmc, the MI355X (gfx950) implementation of
+ * superMC's per-event initial-condition hot path.
+ *
+ * The reference exposes this path only as C++ classes, called from one
+ * thread inside superMC.e; there is no FFI.  Each entry point below replaces
+ * a reference member function (cited beside it); the C++ facade in
+ * supermc_amd/host/ re-exposes MCnucl / MakeDensity / KLNModel /
+ * ParameterReader with the reference's signatures on top of these calls.
+ *
+ * Conventions
+ *  - Plain pointers and sizes only.  Host arrays are caller-owned and only
+ *    borrowed for the duration of a call.  Device buffers belong to the ctx.
+ *  - Grids are [event][ix][iy] with iy contiguous (the reference's
+ *    double[Maxx][Maxy], MCnucl.cpp:125-136); cell centre x = -maxx + ix*dx.
+ *  - Nucleon positions are quantised to multiples of SMC_POS_QUANTUM_FM
+ *    (2^-24 fm) on the way in; all index arithmetic on them is exact.
+ *  - Every call returns an smc_status; the reference's fatal exits map to
+ *    the codes below (it printed and called exit()).
+ *  - A ctx owns one HIP stream and its device buffers; it is not
+ *    thread-safe.  Use one ctx per host thread.
+ */
+#ifndef SMC_H
+#define SMC_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMC_VERSION 1
+#define SMC_POS_QUANTUM_FM (1.0 / 16777216.0) /* 2^-24 fm */
+#define SMC_MAX_A 256                         /* nucleons per nucleus */
+
+typedef int smc_status;
+#define SMC_OK 0
+#define SMC_ERR_ARG 1          /* bad argument / parameter value */
+#define SMC_ERR_HIP 2          /* HIP runtime failure */
+#define SMC_ERR_CAPACITY 3     /* batch or pair capacity exceeded */
+#define SMC_ERR_NO_EVENT 4     /* no accepted event within max_attempts (MakeDensity.cpp:2154-2161) */
+#define SMC_ERR_KLN_TABLE 5    /* thickness outside the dN/dy table (MCnucl.cpp:424-427, :665-669) */
+#define SMC_ERR_DNDY_ZERO 6    /* dndy < 1e-15 (MCnucl.cpp:812-816) */
+#define SMC_ERR_UNSUPPORTED 7  /* option outside the implemented path */
+#define SMC_ERR_STATE 8        /* call made before the state it needs exists */
+#define SMC_ERR_MISSING_KEY 9  /* parameter missing (ParameterReader.cpp:149-156 exit(-1)) */
+
+/* Parameters.  Names and meanings are those of the reference's
+ * parameters.dat keys (ParameterReader semantics); the "extension" block
+ * adds keys the reference hard-codes, whose defaults reproduce it. */
+typedef struct smc_params {
+  /* model (parameters.dat:6-21) */
+  int which_mc_model;          /* 1 MC-KLN, 5 MC-Glauber, 7 sqrt(TA TB) */
+  int sub_model;               /* 1 (Glauber classical), 7 (KLN) */
+  double lambda;               /* KLN saturation exponent */
+  int tmax, tmax_subdivision;  /* KLN table size (MCnucl.cpp:915-919) */
+  double alpha;                /* two-component mixing */
+  /* nuclei (parameters.dat:29-35) */
+  int Aproj, Atarg, proj_deformed, targ_deformed;
+  /* nucleons (parameters.dat:38-60) */
+  int shape_of_nucleons, collision_criterion, shape_of_entropy;
+  double quark_width, gauss_nucl_width;
+  /* collision (parameters.dat:64-68) */
+  double ecm, bmin, bmax;
+  int Npmin, Npmax;
+  /* fluctuations (parameters.dat:136-139) */
+  int cc_fluctuation_model;    /* 0 none, 6 Gamma per source */
+  double cc_fluctuation_Gamma_theta;
+  /* grid (parameters.dat:128-133) */
+  double maxx, maxy, dx, dy;
+  /* output (parameters.dat:97-110) */
+  double finalFactor;
+  int ecc_from_order, ecc_to_order;
+  /* extension keys: <= 0 (or NaN for betas) selects the reference value */
+  double sigma_nn_mb;          /* else Regge96 at ecm (MCnucl.cpp:58-64) */
+  double proj_ws_R, proj_ws_a, targ_ws_R, targ_ws_a;   /* Nucleus.cpp:65-111 */
+  double proj_beta2, proj_beta4, targ_beta2, targ_beta4; /* Nucleus.cpp:122-148 */
+  double nucleon_min_dist;     /* hard core, reference 0.9 fm (Nucleus.cpp:190); <0 = reference */
+  /* KLN quadrature extension (replaces the BASES table integral) */
+  int kln_n_pt, kln_n_kt, kln_n_phi;
+  double kln_pt_min, kln_pt_max;
+  int kln_direct;              /* 1: per-cell quadrature; 0: table + sixPoint2dInterp */
+  /* capacities */
+  int max_events;              /* events per batch held on the device */
+  int max_pairs;               /* per-event collision-pair capacity (<=0: Aproj*Atarg) */
+  int max_attempts;            /* per-event acceptance attempts */
+  int smear_path;              /* 0 VALU gather (exact footprint), 1 MFMA fp32, 2 MFMA bf16x3 */
+} smc_params;
+
+/* Derived quantities (for the facade and tests) */
+typedef struct smc_derived {
+  double siginNN;      /* mb */
+  double width;        /* nucleon Gaussian width [fm] (GaussianNucleonsCal.cpp:34-54) */
+  double dsq;          /* 0.1 sigma / pi [fm^2] (MCnucl.cpp:120) */
+  int nx, ny;          /* Maxx, Maxy */
+  double xmin, ymin;
+  int64_t hit_n_max;   /* largest dx^2+dy^2 (in quanta^2) that collides */
+  double kln_dT;       /* table step (MCnucl.cpp:915-919) */
+  int kln_tmax;
+  double ws_R[2], ws_a[2], beta2[2], beta4[2], dmin;
+} smc_derived;
+
+/* Per-event result record written by smc_moments / smc_run.
+ * Index layout (doubles). */
+#define SMC_EV_STRIDE 64
+#define SMC_EV_EPS_RE 0    /* [10] -Re sum r^2 e^{in theta} rho / (sum r^2 rho + 1e-15) */
+#define SMC_EV_EPS_IM 10   /* [10] */
+#define SMC_EV_EPSP_RE 20  /* [10] r^n weight (r^3 for n=1) */
+#define SMC_EV_EPSP_IM 30  /* [10] */
+#define SMC_EV_RN 40       /* [10] <r^n>, n = 0..9 */
+#define SMC_EV_XC 50
+#define SMC_EV_YC 51
+#define SMC_EV_TOTAL 52    /* sum of finalFactor * s over the grid */
+#define SMC_EV_DSDY 53     /* total * dx * dy (MakeDensity.cpp:2442) */
+#define SMC_EV_NPART 54
+#define SMC_EV_NCOLL 55
+#define SMC_EV_B 56
+#define SMC_EV_STATUS 57
+
+/* Fills p with the values of the reference's parameters.dat. */
+smc_status smc_params_default(smc_params* p);
+/* sizeof(smc_params) and sizeof(smc_derived), for bindings to check layout. */
+size_t smc_params_sizeof(void);
+size_t smc_derived_sizeof(void);
+
+typedef struct smc_ctx smc_ctx;
+
+/* MCnucl::MCnucl (MCnucl.cpp:22-155) + MakeDensity::MakeDensity set-up
+ * (MakeDensity.cpp:21-84): validates parameters, derives sigma/width/dsq,
+ * builds the exact footprint tables, allocates the batch on the device. */
+smc_status smc_create(smc_ctx** ctx, const smc_params* p);
+void smc_destroy(smc_ctx* ctx);
+const char* smc_last_error(const smc_ctx* ctx);
+smc_status smc_get_derived(const smc_ctx* ctx, smc_derived* out);
+
+/* Event generation with acceptance, MakeDensity.cpp:2143-2162 +
+ * MCnucl::generateNuclei (MCnucl.cpp:208-214) + Nucleus::populate
+ * (Nucleus.cpp:187-317) + MCnucl::getBinaryCollision (MCnucl.cpp:217-308).
+ * Counter-based: event i of the batch is event (event0 + i) of stream
+ * `seed`, independent of batch boundaries. */
+smc_status smc_sample(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev);
+
+/* Fed positions (oracle-parity path): nev events of Aproj / Atarg nucleons,
+ * xy interleaved in fm, in any order; b per event is recorded only.
+ * Runs the collision step on them.  fluct_seed drives the Gamma factors. */
+smc_status smc_set_positions(smc_ctx* ctx, int nev, const double* xy_proj,
+                             const double* xy_targ, const double* b, uint64_t fluct_seed);
+
+/* Npart per nucleus and Ncoll per event (MCnucl::getNpart1/2, getNcoll). */
+smc_status smc_collide(smc_ctx* ctx, int* npart_proj, int* npart_targ, int* ncoll);
+
+/* Nucleons of event ev in collision order (sorted by x, Nucleus.cpp:314):
+ * xy [A][2] fm, wound order (participant rank, -1 if spectator) and fluct
+ * factor.  Either nucleus pointer may be NULL. */
+smc_status smc_get_nucleons(smc_ctx* ctx, int ev, double* xy_proj, int* wound_proj,
+                            float* fluct_proj, double* xy_targ, int* wound_targ,
+                            float* fluct_targ);
+
+/* Collision pairs of event ev in reference order (MCnucl.cpp:326-352):
+ * indices into the sorted nucleon lists and the pair fluct factor. */
+smc_status smc_get_pairs(smc_ctx* ctx, int ev, int cap, int* pair_proj, int* pair_targ,
+                         float* fluct, int* npairs);
+
+/* MCnucl::calculateThickness (MCnucl.cpp:398-478).  TA/TB host [nev][nx][ny]
+ * (NULL: keep on device only). */
+smc_status smc_thickness(smc_ctx* ctx, float* TA, float* TB);
+
+/* MCnucl::setDensity(iy=0, ipt=-1) (MCnucl.cpp:626-820) for the configured
+ * model; s host [nev][nx][ny] (NULL: device only); dndy per event
+ * (sum of s, no finalFactor; NULL allowed). */
+smc_status smc_density(smc_ctx* ctx, float* s, double* dndy);
+
+/* MakeDensity::setSd + dumpEccentricities (MakeDensity.cpp:2244-2522):
+ * per-event records, SMC_EV_STRIDE doubles each (NULL: device only). */
+smc_status smc_moments(smc_ctx* ctx, double* ev_out);
+
+/* The whole hot path for nev events: sample -> collide -> density ->
+ * moments, inputs and outputs resident on the device.  ev_out as above
+ * (NULL: keep on device, fetch with smc_fetch_events). */
+smc_status smc_run(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
+smc_status smc_fetch_events(smc_ctx* ctx, int nev, double* ev_out);
+
+/* MC-KLN: KLNModel::getdNdy (KLNModel.cpp:97-125) at y=0 for n (TA,TB)
+ * points, by the Gauss-Legendre product rule on the device. */
+smc_status smc_kln_dndy(smc_ctx* ctx, int n, const double* ta, const double* tb, double* out);
+/* MCnucl::makeTable (MCnucl.cpp:911-960) into the ctx (and to host table
+ * [tmax][tmax] if non-NULL). */
+smc_status smc_kln_make_table(smc_ctx* ctx, double* table);
+
+/* Timing of the ctx's kernels with HIP events on its stream. */
+#define SMC_K_EVENT 0
+#define SMC_K_SMEAR 1
+#define SMC_K_MOMENTS 2
+#define SMC_K_KLN 3
+#define SMC_K_COUNT 8
+smc_status smc_set_timing(smc_ctx* ctx, int on);
+smc_status smc_get_timing(smc_ctx* ctx, int kernel, double* total_ms, int* launches);
+smc_status smc_synchronize(smc_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

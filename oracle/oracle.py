@@ -1,0 +1,287 @@
+"""ctypes loader for the FP64 CPU oracle (smc_oracle.c) and, when built,
+the reference harness oracle/_ref/libsmcref.so.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "libsmc_oracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libsmcref.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+
+class OrcGrid(C.Structure):
+    _fields_ = [("xmin", C.c_double), ("ymin", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+                ("nx", C.c_int), ("ny", C.c_int)]
+
+
+class OrcKln(C.Structure):
+    _fields_ = [("ecm", C.c_double), ("lambda_", C.c_double), ("ptmin", C.c_double), ("ptmax", C.c_double),
+                ("g2hfac", C.c_double), ("lambda_qcd", C.c_double), ("alpha_max", C.c_double)]
+
+
+class OrcRng(C.Structure):
+    _fields_ = [("s", C.c_uint64)]
+
+
+class OrcEventCfg(C.Structure):
+    _fields_ = [("A", C.c_int), ("B", C.c_int), ("R_a", C.c_double), ("a_a", C.c_double), ("R_b", C.c_double),
+                ("a_b", C.c_double), ("dmin", C.c_double), ("siginNN", C.c_double), ("w", C.c_double),
+                ("alpha", C.c_double), ("gamma_theta", C.c_double), ("fluct", C.c_int), ("bmin", C.c_double),
+                ("bmax", C.c_double), ("npmin", C.c_int), ("npmax", C.c_int), ("maxx", C.c_double),
+                ("dx", C.c_double), ("from_order", C.c_int), ("to_order", C.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "oracle"], check=True)
+
+
+_orc = None
+_ref = None
+
+
+def lib():
+    global _orc
+    if _orc is None:
+        if not os.path.exists(ORACLE_SO):
+            build()
+        o = C.CDLL(ORACLE_SO)
+        sig = {
+            "orc_grid_init": (None, [C.POINTER(OrcGrid), C.c_double, C.c_double, C.c_double, C.c_double]),
+            "orc_sigma_inel": (C.c_double, [C.c_double]),
+            "orc_nucleon_width": (C.c_double, [C.c_int, C.c_double, C.c_double]),
+            "orc_dsq": (C.c_double, [C.c_double]),
+            "orc_particle_box": (None, [C.c_double, C.c_double, C.c_double, _dp]),
+            "orc_sort_nucleus": (None, [C.c_int, _dp, _dp, C.c_double, _ip]),
+            "orc_collide": (C.c_int, [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_double, C.c_double,
+                                      _ip, _ip, _ip, _ip, _ip, _ip, _ip, _ip, C.c_int]),
+            "orc_collide_bruteforce": (C.c_int, [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_double, _ip, _ip]),
+            "orc_thickness": (None, [C.POINTER(OrcGrid), C.c_double, C.c_int, _dp, _dp, _dp]),
+            "orc_add_density": (None, [C.POINTER(OrcGrid), C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
+            "orc_density_glauber": (C.c_double, [C.POINTER(OrcGrid), C.c_double, C.c_double,
+                                                 C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
+                                                 C.c_int, _dp, _dp, _dp, _dp]),
+            "orc_density_trento": (C.c_double, [C.POINTER(OrcGrid), C.c_double, C.c_int, _dp, _dp, _dp,
+                                                C.c_int, _dp, _dp, _dp, _dp]),
+            "orc_moments": (C.c_int, [C.POINTER(OrcGrid), _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
+                                      _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_kln_defaults": (None, [C.POINTER(OrcKln), C.c_double, C.c_double]),
+            "orc_kln_func": (C.c_double, [C.POINTER(OrcKln), C.c_double, C.c_double, _dp]),
+            "orc_kln_dndy_gl": (C.c_double, [C.POINTER(OrcKln), C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
+            "orc_gauss_legendre01": (None, [C.c_int, _dp, _dp]),
+            "orc_six_point": (C.c_double, [C.c_double] * 8),
+            "orc_density_kln_table": (C.c_double, [C.POINTER(OrcGrid), _dp, _dp, _dp, C.c_int, C.c_double, _dp]),
+            "orc_rng_seed": (None, [C.POINTER(OrcRng), C.c_uint64]),
+            "orc_rng_uniform": (C.c_double, [C.POINTER(OrcRng)]),
+            "orc_sample_ws": (None, [C.POINTER(OrcRng), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                     C.c_double, C.c_int, _dp, _dp, _dp]),
+            "orc_populate": (None, [C.POINTER(OrcRng), C.c_int, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    C.c_int, C.c_double, C.c_double, _dp, _dp]),
+            "orc_hulthen_cdf": (C.c_double, [C.c_double]),
+            "orc_gamma": (C.c_double, [C.POINTER(OrcRng), C.c_double, C.c_double]),
+            "orc_rotate": (None, [C.c_double, C.c_double, _dp, _dp, _dp]),
+            "orc_ylm": (C.c_double, [C.c_int, C.c_double]),
+            "orc_run_events": (C.c_int, [C.POINTER(OrcEventCfg), C.c_uint64, C.c_int, _dp]),
+        }
+        for n, (r, a) in sig.items():
+            f = getattr(o, n)
+            f.restype = r
+            f.argtypes = a
+        _orc = o
+    return _orc
+
+
+def ref():
+    """The reference-harness library, or None when it is not built."""
+    global _ref
+    if _ref is None:
+        if not os.path.exists(REF_SO):
+            return None
+        r = C.CDLL(REF_SO)
+        sig = {
+            "ref_sigma_inel": (C.c_double, [C.c_double]),
+            "ref_kln_dndy": (C.c_double, [C.c_double, C.c_double, C.c_double, C.c_double]),
+            "ref_kln_func": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, _dp, _dp]),
+            "ref_six_point": (C.c_double, [C.c_double] * 8),
+            "ref_particle_box": (None, [C.c_double, C.c_double, C.c_int, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, _dp]),
+            "ref_overunion": (None, [_dp, _dp, _dp]),
+            "ref_rotate": (None, [C.c_double, C.c_double, _dp]),
+            "ref_hulthen_cdf": (C.c_double, [C.c_double]),
+            "ref_hulthen_invcdf": (C.c_double, [C.c_double]),
+            "ref_param_value": (C.c_double, [C.c_char_p, C.c_int, C.POINTER(C.c_char_p), C.c_char_p, _ip]),
+        }
+        for n, (rt, a) in sig.items():
+            f = getattr(r, n)
+            f.restype = rt
+            f.argtypes = a
+        _ref = r
+    return _ref
+
+
+def _p(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _pi(a):
+    return a.ctypes.data_as(_ip)
+
+
+def grid(maxx=13.0, dx=0.1, maxy=None, dy=None):
+    g = OrcGrid()
+    lib().orc_grid_init(C.byref(g), maxx, maxx if maxy is None else maxy, dx, dx if dy is None else dy)
+    return g
+
+
+def particle_box(x, y, w):
+    out = np.zeros(4)
+    lib().orc_particle_box(x, y, w, _p(out))
+    return out
+
+
+def sort_nucleus(xy, w):
+    xy = np.ascontiguousarray(xy, np.float64)
+    x, y = np.ascontiguousarray(xy[:, 0]), np.ascontiguousarray(xy[:, 1])
+    perm = np.zeros(len(x), np.int32)
+    lib().orc_sort_nucleus(len(x), _p(x), _p(y), w, _pi(perm))
+    return perm
+
+
+def collide(xy_a, xy_b, w, dsq):
+    """Reference sweep; inputs must already be in sorted order."""
+    xa, ya = np.ascontiguousarray(xy_a[:, 0]), np.ascontiguousarray(xy_a[:, 1])
+    xb, yb = np.ascontiguousarray(xy_b[:, 0]), np.ascontiguousarray(xy_b[:, 1])
+    na, nb = len(xa), len(xb)
+    wa, wb = np.zeros(na, np.int32), np.zeros(nb, np.int32)
+    oa, ob = np.zeros(na, np.int32), np.zeros(nb, np.int32)
+    npa, npb = C.c_int(), C.c_int()
+    mp = max(1, na * nb)
+    pa, pb = np.zeros(mp, np.int32), np.zeros(mp, np.int32)
+    nc = lib().orc_collide(na, _p(xa), _p(ya), nb, _p(xb), _p(yb), w, dsq, _pi(wa), _pi(wb), _pi(oa), _pi(ob),
+                           C.byref(npa), C.byref(npb), _pi(pa), _pi(pb), mp)
+    return dict(ncoll=nc, wound_a=wa, wound_b=wb, order_a=oa[:npa.value], order_b=ob[:npb.value],
+                npart_a=npa.value, npart_b=npb.value, pair_a=pa[:nc], pair_b=pb[:nc])
+
+
+def collide_bruteforce(xy_a, xy_b, dsq):
+    xa, ya = np.ascontiguousarray(xy_a[:, 0]), np.ascontiguousarray(xy_a[:, 1])
+    xb, yb = np.ascontiguousarray(xy_b[:, 0]), np.ascontiguousarray(xy_b[:, 1])
+    wa, wb = np.zeros(len(xa), np.int32), np.zeros(len(xb), np.int32)
+    nc = lib().orc_collide_bruteforce(len(xa), _p(xa), _p(ya), len(xb), _p(xb), _p(yb), dsq, _pi(wa), _pi(wb))
+    return nc, wa, wb
+
+
+def thickness(g, w, xy):
+    xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+    x, y = np.ascontiguousarray(xy[:, 0]), np.ascontiguousarray(xy[:, 1])
+    T = np.zeros((g.nx, g.ny))
+    lib().orc_thickness(C.byref(g), w, len(x), _p(x), _p(y), _p(T))
+    return T
+
+
+def density_glauber(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c):
+    def cols(xy):
+        xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+        return len(xy), np.ascontiguousarray(xy[:, 0]), np.ascontiguousarray(xy[:, 1])
+    na, xa, ya = cols(xy_a)
+    nb, xb, yb = cols(xy_b)
+    nc, xc, yc = cols(xy_c)
+    fa = np.ascontiguousarray(f_a, np.float64)
+    fb = np.ascontiguousarray(f_b, np.float64)
+    fc = np.ascontiguousarray(f_c, np.float64)
+    s = np.zeros((g.nx, g.ny))
+    dndy = lib().orc_density_glauber(C.byref(g), w, alpha, na, _p(xa), _p(ya), _p(fa), nb, _p(xb), _p(yb), _p(fb),
+                                     nc, _p(xc), _p(yc), _p(fc), _p(s))
+    return s, dndy
+
+
+def density_trento(g, w, xy_a, f_a, xy_b, f_b):
+    def cols(xy):
+        xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
+        return len(xy), np.ascontiguousarray(xy[:, 0]), np.ascontiguousarray(xy[:, 1])
+    na, xa, ya = cols(xy_a)
+    nb, xb, yb = cols(xy_b)
+    fa = np.ascontiguousarray(f_a, np.float64)
+    fb = np.ascontiguousarray(f_b, np.float64)
+    s = np.zeros((g.nx, g.ny))
+    dndy = lib().orc_density_trento(C.byref(g), w, na, _p(xa), _p(ya), _p(fa), nb, _p(xb), _p(yb), _p(fb), _p(s))
+    return s, dndy
+
+
+def moments(g, density, boxes, ncoll, from_order, to_order):
+    density = np.ascontiguousarray(density, np.float64)
+    boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 4)
+    out = {k: np.zeros(10) for k in ("mom_re", "mom_im", "momp_re", "momp_im", "rn")}
+    cm = np.zeros(3)
+    rc = lib().orc_moments(C.byref(g), _p(density), len(boxes), _p(boxes), ncoll, from_order, to_order,
+                           _p(out["mom_re"]), _p(out["mom_im"]), _p(out["momp_re"]), _p(out["momp_im"]),
+                           _p(out["rn"]), _p(cm))
+    out["xc"], out["yc"], out["total"] = cm
+    out["rc"] = rc
+    return out
+
+
+def kln(ecm=2760.0, lam=0.138):
+    k = OrcKln()
+    lib().orc_kln_defaults(C.byref(k), ecm, lam)
+    return k
+
+
+def kln_func(k, ta, tb, u):
+    u = np.ascontiguousarray(u, np.float64)
+    return lib().orc_kln_func(C.byref(k), ta, tb, _p(u))
+
+
+def kln_dndy_gl(k, ta, tb, n_pt, n_kt, n_phi):
+    return lib().orc_kln_dndy_gl(C.byref(k), ta, tb, n_pt, n_kt, n_phi)
+
+
+def gauss_legendre01(n):
+    x, w = np.zeros(n), np.zeros(n)
+    lib().orc_gauss_legendre01(n, _p(x), _p(w))
+    return x, w
+
+
+def six_point(*a):
+    return lib().orc_six_point(*[float(v) for v in a])
+
+
+class Rng:
+    def __init__(self, seed):
+        self.r = OrcRng()
+        lib().orc_rng_seed(C.byref(self.r), seed)
+
+    def uniform(self):
+        return lib().orc_rng_uniform(C.byref(self.r))
+
+    def gamma(self, shape, scale):
+        return lib().orc_gamma(C.byref(self.r), shape, scale)
+
+    def populate(self, A, R, a, dmin, xcenter=0.0, beta2=0.0, beta4=0.0, deformed=0):
+        x, y = np.zeros(A), np.zeros(A)
+        lib().orc_populate(C.byref(self.r), A, R, a, beta2, beta4, deformed, dmin, xcenter, _p(x), _p(y))
+        return np.stack([x, y], axis=1)
+
+
+def hulthen_cdf(r):
+    return lib().orc_hulthen_cdf(r)
+
+
+def rotate(cth, phi, xyz):
+    x, y, z = (C.c_double(v) for v in xyz)
+    lib().orc_rotate(cth, phi, C.byref(x), C.byref(y), C.byref(z))
+    return np.array([x.value, y.value, z.value])
+
+
+def run_events(cfg, seed, nev, want_eps=False):
+    eps = np.zeros(2 * nev) if want_eps else None
+    n = lib().orc_run_events(C.byref(cfg), seed, nev, _p(eps) if eps is not None else None)
+    return n, (eps.reshape(nev, 2) if eps is not None else None)

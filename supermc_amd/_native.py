@@ -1,0 +1,278 @@
+"""ctypes binding of libsmc (include/smc.h).
+
+The library is built in-tree (supermc_amd/build.py) and loaded from
+supermc_amd/lib/libsmc.so.  There is no fallback: if the library is missing,
+loading raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libsmc.so")
+
+SMC_OK = 0
+SMC_ERR_ARG = 1
+SMC_ERR_HIP = 2
+SMC_ERR_CAPACITY = 3
+SMC_ERR_NO_EVENT = 4
+SMC_ERR_KLN_TABLE = 5
+SMC_ERR_DNDY_ZERO = 6
+SMC_ERR_UNSUPPORTED = 7
+SMC_ERR_STATE = 8
+SMC_ERR_MISSING_KEY = 9
+
+POS_QUANTUM_FM = 1.0 / 16777216.0
+MAX_A = 256
+
+EV_STRIDE = 64
+EV_EPS_RE, EV_EPS_IM, EV_EPSP_RE, EV_EPSP_IM, EV_RN = 0, 10, 20, 30, 40
+EV_XC, EV_YC, EV_TOTAL, EV_DSDY, EV_NPART, EV_NCOLL, EV_B, EV_STATUS = 50, 51, 52, 53, 54, 55, 56, 57
+
+K_EVENT, K_SMEAR, K_MOMENTS, K_KLN = 0, 1, 2, 3
+
+
+class SmcParams(C.Structure):
+    _fields_ = [
+        ("which_mc_model", C.c_int), ("sub_model", C.c_int), ("lambda_", C.c_double),
+        ("tmax", C.c_int), ("tmax_subdivision", C.c_int), ("alpha", C.c_double),
+        ("Aproj", C.c_int), ("Atarg", C.c_int), ("proj_deformed", C.c_int), ("targ_deformed", C.c_int),
+        ("shape_of_nucleons", C.c_int), ("collision_criterion", C.c_int), ("shape_of_entropy", C.c_int),
+        ("quark_width", C.c_double), ("gauss_nucl_width", C.c_double),
+        ("ecm", C.c_double), ("bmin", C.c_double), ("bmax", C.c_double),
+        ("Npmin", C.c_int), ("Npmax", C.c_int),
+        ("cc_fluctuation_model", C.c_int), ("cc_fluctuation_Gamma_theta", C.c_double),
+        ("maxx", C.c_double), ("maxy", C.c_double), ("dx", C.c_double), ("dy", C.c_double),
+        ("finalFactor", C.c_double), ("ecc_from_order", C.c_int), ("ecc_to_order", C.c_int),
+        ("sigma_nn_mb", C.c_double),
+        ("proj_ws_R", C.c_double), ("proj_ws_a", C.c_double), ("targ_ws_R", C.c_double), ("targ_ws_a", C.c_double),
+        ("proj_beta2", C.c_double), ("proj_beta4", C.c_double), ("targ_beta2", C.c_double), ("targ_beta4", C.c_double),
+        ("nucleon_min_dist", C.c_double),
+        ("kln_n_pt", C.c_int), ("kln_n_kt", C.c_int), ("kln_n_phi", C.c_int),
+        ("kln_pt_min", C.c_double), ("kln_pt_max", C.c_double), ("kln_direct", C.c_int),
+        ("max_events", C.c_int), ("max_pairs", C.c_int), ("max_attempts", C.c_int), ("smear_path", C.c_int),
+    ]
+
+
+class SmcDerived(C.Structure):
+    _fields_ = [
+        ("siginNN", C.c_double), ("width", C.c_double), ("dsq", C.c_double),
+        ("nx", C.c_int), ("ny", C.c_int), ("xmin", C.c_double), ("ymin", C.c_double),
+        ("hit_n_max", C.c_int64), ("kln_dT", C.c_double), ("kln_tmax", C.c_int),
+        ("ws_R", C.c_double * 2), ("ws_a", C.c_double * 2), ("beta2", C.c_double * 2),
+        ("beta4", C.c_double * 2), ("dmin", C.c_double),
+    ]
+
+
+_P = C.POINTER
+_dp = _P(C.c_double)
+_ip = _P(C.c_int)
+_fp = _P(C.c_float)
+
+# name -> (restype, argtypes); every symbol include/smc.h declares
+SIGNATURES = {
+    "smc_params_default": (C.c_int, [_P(SmcParams)]),
+    "smc_params_sizeof": (C.c_size_t, []),
+    "smc_derived_sizeof": (C.c_size_t, []),
+    "smc_create": (C.c_int, [_P(C.c_void_p), _P(SmcParams)]),
+    "smc_destroy": (None, [C.c_void_p]),
+    "smc_last_error": (C.c_char_p, [C.c_void_p]),
+    "smc_get_derived": (C.c_int, [C.c_void_p, _P(SmcDerived)]),
+    "smc_sample": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int]),
+    "smc_set_positions": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_uint64]),
+    "smc_collide": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
+    "smc_get_nucleons": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _fp, _dp, _ip, _fp]),
+    "smc_get_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _fp, _ip]),
+    "smc_thickness": (C.c_int, [C.c_void_p, _fp, _fp]),
+    "smc_density": (C.c_int, [C.c_void_p, _fp, _dp]),
+    "smc_moments": (C.c_int, [C.c_void_p, _dp]),
+    "smc_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
+    "smc_fetch_events": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "smc_kln_dndy": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
+    "smc_kln_make_table": (C.c_int, [C.c_void_p, _dp]),
+    "smc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
+    "smc_get_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
+    "smc_synchronize": (C.c_int, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def load(path=LIB_PATH):
+    """Loads libsmc.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libsmc.so not built ({path}); run `python -m supermc_amd.build`")
+    lib = C.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    if lib.smc_params_sizeof() != C.sizeof(SmcParams):
+        raise RuntimeError("smc_params layout mismatch between smc.h and the binding")
+    if lib.smc_derived_sizeof() != C.sizeof(SmcDerived):
+        raise RuntimeError("smc_derived layout mismatch between smc.h and the binding")
+    _lib = lib
+    return lib
+
+
+class SmcError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[smc status {code}] {msg}")
+        self.code = code
+
+
+def default_params(**kw):
+    lib = load()
+    p = SmcParams()
+    lib.smc_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "lambda":
+            k = "lambda_"
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def _ptr(a, ctype):
+    return a.ctypes.data_as(_P(ctype)) if a is not None else None
+
+
+class Context:
+    """One libsmc context: a HIP stream plus device buffers for a batch."""
+
+    def __init__(self, params):
+        self.lib = load()
+        self.params = params
+        h = C.c_void_p()
+        rc = self.lib.smc_create(C.byref(h), C.byref(params))
+        if rc != SMC_OK:
+            raise SmcError(rc, "smc_create failed (see stderr)")
+        self.h = h
+        d = SmcDerived()
+        self.lib.smc_get_derived(self.h, C.byref(d))
+        self.derived = d
+        self.nx, self.ny = d.nx, d.ny
+        self.nev = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.smc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc, allow=()):
+        if rc != SMC_OK and rc not in allow:
+            raise SmcError(rc, self.lib.smc_last_error(self.h).decode())
+        return rc
+
+    # -- events
+    def sample(self, seed, event0, nev):
+        self._check(self.lib.smc_sample(self.h, seed, event0, nev))
+        self.nev = nev
+
+    def set_positions(self, xy_proj, xy_targ, b=None, fluct_seed=0):
+        xy_proj = np.ascontiguousarray(xy_proj, dtype=np.float64)
+        xy_targ = np.ascontiguousarray(xy_targ, dtype=np.float64)
+        nev = xy_proj.shape[0]
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        self._check(self.lib.smc_set_positions(self.h, nev, _ptr(xy_proj, C.c_double), _ptr(xy_targ, C.c_double),
+                                               _ptr(bb, C.c_double), fluct_seed))
+        self.nev = nev
+
+    def collide(self, allow=()):
+        n = self.nev
+        a, b, c = (np.zeros(n, np.int32) for _ in range(3))
+        rc = self._check(self.lib.smc_collide(self.h, _ptr(a, C.c_int), _ptr(b, C.c_int), _ptr(c, C.c_int)), allow)
+        return a, b, c, rc
+
+    def nucleons(self, ev):
+        A, B = self.params.Aproj, self.params.Atarg
+        out = dict(xy_proj=np.zeros((A, 2)), wound_proj=np.zeros(A, np.int32), fluct_proj=np.zeros(A, np.float32),
+                   xy_targ=np.zeros((B, 2)), wound_targ=np.zeros(B, np.int32), fluct_targ=np.zeros(B, np.float32))
+        self._check(self.lib.smc_get_nucleons(
+            self.h, ev, _ptr(out["xy_proj"], C.c_double), _ptr(out["wound_proj"], C.c_int),
+            _ptr(out["fluct_proj"], C.c_float), _ptr(out["xy_targ"], C.c_double),
+            _ptr(out["wound_targ"], C.c_int), _ptr(out["fluct_targ"], C.c_float)))
+        return out
+
+    def pairs(self, ev, cap=None):
+        if cap is None:
+            cap = self.params.Aproj * self.params.Atarg
+        pa, pb = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        pf = np.zeros(cap, np.float32)
+        n = C.c_int()
+        self._check(self.lib.smc_get_pairs(self.h, ev, cap, _ptr(pa, C.c_int), _ptr(pb, C.c_int),
+                                           _ptr(pf, C.c_float), C.byref(n)))
+        k = n.value
+        return pa[:k], pb[:k], pf[:k]
+
+    def thickness(self):
+        shape = (self.nev, self.nx, self.ny)
+        ta, tb = np.zeros(shape, np.float32), np.zeros(shape, np.float32)
+        self._check(self.lib.smc_thickness(self.h, _ptr(ta, C.c_float), _ptr(tb, C.c_float)))
+        return ta, tb
+
+    def density(self, want_grid=True):
+        s = np.zeros((self.nev, self.nx, self.ny), np.float32) if want_grid else None
+        dndy = np.zeros(self.nev)
+        rc = self._check(self.lib.smc_density(self.h, _ptr(s, C.c_float), _ptr(dndy, C.c_double)),
+                         allow=(SMC_ERR_DNDY_ZERO,))
+        return s, dndy, rc
+
+    def moments(self):
+        ev = np.zeros((self.nev, EV_STRIDE))
+        self._check(self.lib.smc_moments(self.h, _ptr(ev, C.c_double)))
+        return ev
+
+    def run(self, seed, event0, nev, fetch=True):
+        ev = np.zeros((nev, EV_STRIDE)) if fetch else None
+        self._check(self.lib.smc_run(self.h, seed, event0, nev, _ptr(ev, C.c_double)))
+        self.nev = nev
+        return ev
+
+    def fetch_events(self, nev):
+        ev = np.zeros((nev, EV_STRIDE))
+        self._check(self.lib.smc_fetch_events(self.h, nev, _ptr(ev, C.c_double)))
+        return ev
+
+    def kln_dndy(self, ta, tb):
+        ta = np.ascontiguousarray(ta, np.float64)
+        tb = np.ascontiguousarray(tb, np.float64)
+        out = np.zeros(ta.shape[0])
+        self._check(self.lib.smc_kln_dndy(self.h, ta.shape[0], _ptr(ta, C.c_double), _ptr(tb, C.c_double),
+                                          _ptr(out, C.c_double)))
+        return out
+
+    def kln_make_table(self):
+        n = self.derived.kln_tmax
+        t = np.zeros((n, n))
+        self._check(self.lib.smc_kln_make_table(self.h, _ptr(t, C.c_double)))
+        return t
+
+    def set_timing(self, on=True):
+        self._check(self.lib.smc_set_timing(self.h, 1 if on else 0))
+
+    def timing(self, kernel):
+        ms = C.c_double()
+        n = C.c_int()
+        self._check(self.lib.smc_get_timing(self.h, kernel, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def synchronize(self):
+        self._check(self.lib.smc_synchronize(self.h))

@@ -1,0 +1,776 @@
+// smc_api.cpp -- host side of libsmc: parameter semantics, derived
+// quantities, exact footprint tables, device buffers and the C-ABI entry
+// points declared in include/smc.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <functional>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "smc_internal.h"
+
+using namespace smc;
+
+namespace {
+
+struct SmcError : std::runtime_error {
+  int code;
+  SmcError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t err_ = (x);                                                               \
+    if (err_ != hipSuccess)                                                              \
+      throw SmcError(SMC_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(err_));     \
+  } while (0)
+
+template <class T>
+T* dalloc(size_t n) {
+  void* p = nullptr;
+  if (n == 0) n = 1;
+  HIPCHK(hipMalloc(&p, n * sizeof(T)));
+  return (T*)p;
+}
+
+// Regge96.cpp:27-50 (pp total, baryon-baryon elastic); MCnucl.cpp:58-64
+double regge_sigma_inel(double ecm) {
+  const double s = ecm * ecm;
+  const double sig = 22.0 * std::pow(s, 0.079) + 56.1 * std::pow(s, -0.46);
+  const double bel = 2.0 * 2.3 + 2.0 * 2.3 + 4.0 * std::pow(s, 0.0808) - 4.2;
+  const double sigel = 0.0511 * sig * sig / bel;
+  return sig - sigel;
+}
+
+// Woods-Saxon radius / diffuseness per mass number (Nucleus.cpp:65-111)
+void ws_table(int A, double& R, double& a) {
+  const double Ad = (double)A;
+  R = 1.12 * std::pow(Ad, 0.333333) - 0.86 / std::pow(Ad, 0.333333);
+  a = 0.54;
+  if (A == 197) { R = 6.42; a = 0.45; }
+  else if (A == 63) { R = 4.28; a = 0.5; }
+  else if (A == 238) { R = 6.86; a = 0.44; }
+  else if (A == 208) { R = 6.67; a = 0.44; }
+  else if (A == 129) { R = 5.36; a = 0.590; }
+}
+
+// Deformation per mass number (Nucleus.cpp:122-148)
+void deform_table(int A, double& b2, double& b4) {
+  b2 = 0.0;
+  b4 = 0.0;
+  if (A == 197) { b2 = -0.13; b4 = -0.03; }
+  else if (A == 63) { b2 = 0.162; b4 = 0.006; }
+  else if (A == 129) { b2 = 0.162; b4 = -0.003; }
+  else if (A == 238) { b2 = 0.28; b4 = 0.093; }
+}
+
+struct HostTable {
+  std::vector<int> thr;
+  int vlo = 0;
+  float a = 0.f, b = 0.f;
+};
+
+// Thresholds of a monotone non-decreasing step function on [pmin, pmax].
+HostTable build_steps(const std::function<long long(long long)>& f, long long pmin, long long pmax) {
+  HostTable T;
+  const long long vlo = f(pmin), vhi = f(pmax);
+  if (vhi < vlo) throw SmcError(SMC_ERR_ARG, "footprint table is not monotone");
+  T.vlo = (int)vlo;
+  const long long n = vhi - vlo;
+  T.thr.resize((size_t)n);
+  for (long long k = 0; k < n; ++k) {
+    const long long target = vlo + k + 1;
+    long long lo = (k > 0) ? T.thr[(size_t)k - 1] : pmin, hi = pmax;  // f(hi) >= target
+    while (lo < hi) {
+      const long long mid = lo + (hi - lo) / 2;
+      if (f(mid) >= target) hi = mid; else lo = mid + 1;
+    }
+    T.thr[(size_t)k] = (int)lo;
+  }
+  if (n >= 2) {
+    const double a = (double)(n - 1) / ((double)T.thr[(size_t)n - 1] - (double)T.thr[0]);
+    T.a = (float)a;
+    T.b = (float)(1.0 - a * (double)T.thr[0]);
+  } else {
+    T.a = 0.f;
+    T.b = 0.f;
+  }
+  return T;
+}
+
+}  // namespace
+
+struct smc_ctx {
+  smc_params p{};
+  smc_derived d{};
+  DevCfg cfg{};
+  EvBufs buf{};
+  hipStream_t stream = nullptr;
+  std::string err;
+  int cap = 0;        // max events
+  int nev = 0;        // events currently held
+  int have_events = 0, have_density = 0, have_thickness = 0, have_moments = 0;
+  int* dtab = nullptr;
+  float* grid = nullptr;
+  float* ta = nullptr;
+  float* tb = nullptr;
+  double* evout = nullptr;
+  double* kln_table = nullptr;
+  // timing
+  int timing = 0;
+  struct Pending { int k; hipEvent_t a, b; };
+  std::vector<Pending> pending;
+  double tms[SMC_K_COUNT] = {0};
+  int tcount[SMC_K_COUNT] = {0};
+};
+
+namespace {
+
+int fail(smc_ctx* c, const SmcError& e) {
+  if (c) c->err = e.what();
+  return e.code;
+}
+
+struct TimerScope {
+  smc_ctx* c;
+  int k;
+  hipEvent_t a = nullptr, b = nullptr;
+  TimerScope(smc_ctx* c_, int k_) : c(c_), k(k_) {
+    if (c->timing) {
+      HIPCHK(hipEventCreate(&a));
+      HIPCHK(hipEventCreate(&b));
+      HIPCHK(hipEventRecord(a, c->stream));
+    }
+  }
+  void done() {
+    if (c->timing) {
+      HIPCHK(hipEventRecord(b, c->stream));
+      c->pending.push_back({k, a, b});
+      a = b = nullptr;
+    }
+  }
+};
+
+void check_launch(int rc, const char* what) {
+  if (rc != 0) throw SmcError(SMC_ERR_HIP, std::string(what) + ": " + hipGetErrorString((hipError_t)rc));
+}
+
+void validate_and_derive(smc_ctx* c) {
+  smc_params& p = c->p;
+  smc_derived& d = c->d;
+  if (!(p.which_mc_model == 5 || p.which_mc_model == 7 || p.which_mc_model == 1))
+    throw SmcError(SMC_ERR_UNSUPPORTED, "which_mc_model must be 1, 5 or 7");
+  if (p.which_mc_model == 5 && p.sub_model != 1)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "MC-Glauber sub_model 2 is not on this path");
+  if (p.which_mc_model == 1 && p.sub_model != 7)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN requires sub_model 7 (KLN uGD)");
+  if (!(p.shape_of_nucleons == 2 || p.shape_of_nucleons == 4))
+    throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_nucleons must be 2 or 4 (Gaussian nucleons)");
+  if (p.collision_criterion != 1)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "collision_criterion must be 1 (disk)");
+  if (p.shape_of_entropy != 2)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_entropy must be 2 (Gaussian)");
+  if (!(p.cc_fluctuation_model == 0 || p.cc_fluctuation_model == 6))
+    throw SmcError(SMC_ERR_UNSUPPORTED, "cc_fluctuation_model must be 0 or 6");
+  const int As[2] = {p.Aproj, p.Atarg};
+  for (int i = 0; i < 2; ++i) {
+    if (As[i] < 1 || As[i] > kAMax) throw SmcError(SMC_ERR_ARG, "A must be in [1, 256]");
+    if (As[i] == 3 || As[i] == 4 || As[i] == 12 || As[i] == 16)
+      throw SmcError(SMC_ERR_UNSUPPORTED, "light nuclei read configuration tables (not on this path)");
+  }
+  if (!(p.dx > 0 && p.dy > 0 && p.maxx > 0 && p.maxy > 0)) throw SmcError(SMC_ERR_ARG, "bad grid");
+  if (p.ecc_from_order < 1 || p.ecc_to_order > 9 || p.ecc_from_order > p.ecc_to_order)
+    throw SmcError(SMC_ERR_ARG, "ecc orders must satisfy 1 <= from <= to <= 9");
+  if (p.bmax < p.bmin || p.bmin < 0) throw SmcError(SMC_ERR_ARG, "bad impact parameter range");
+
+  // sigma_inel and width (MCnucl.cpp:58-64, 93; GaussianNucleonsCal.cpp:34-54)
+  d.siginNN = (p.sigma_nn_mb > 0) ? p.sigma_nn_mb : regge_sigma_inel(p.ecm);
+  if (p.shape_of_nucleons == 2) d.width = std::sqrt(0.1 * d.siginNN / M_PI) / std::sqrt(8);
+  else d.width = p.gauss_nucl_width;
+  d.dsq = 0.1 * d.siginNN / M_PI / 1;
+  const double w = d.width;
+  if (!(w > 0.05 && w <= 1.0)) throw SmcError(SMC_ERR_ARG, "nucleon width must be in (0.05, 1] fm");
+  if (std::sqrt(d.dsq) > 8 * w)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "collision disk wider than the nucleon box (sweep pruning differs)");
+  // grid (MCnucl.cpp:33-40)
+  d.xmin = -p.maxx;
+  d.ymin = -p.maxy;
+  d.nx = (int)((p.maxx - d.xmin) / p.dx + 0.1) + 1;
+  d.ny = (int)((p.maxy - d.ymin) / p.dy + 0.1) + 1;
+  if (d.nx > 2048 || d.ny > 2048) throw SmcError(SMC_ERR_ARG, "grid too large");
+  if (p.maxx + 4 * std::max(w, p.quark_width) > 56 || p.maxy > 56)
+    throw SmcError(SMC_ERR_ARG, "grid extent beyond the 56 fm position range");
+
+  // nuclear geometry (Nucleus.cpp:65-148), extension keys override
+  const double Rext[2] = {p.proj_ws_R, p.targ_ws_R}, aext[2] = {p.proj_ws_a, p.targ_ws_a};
+  const double b2ext[2] = {p.proj_beta2, p.targ_beta2}, b4ext[2] = {p.proj_beta4, p.targ_beta4};
+  const int defo[2] = {p.proj_deformed, p.targ_deformed};
+  for (int i = 0; i < 2; ++i) {
+    double R, a, b2, b4;
+    ws_table(As[i], R, a);
+    deform_table(As[i], b2, b4);
+    if (Rext[i] > 0) R = Rext[i];
+    if (aext[i] > 0) a = aext[i];
+    if (!std::isnan(b2ext[i])) b2 = b2ext[i];
+    if (!std::isnan(b4ext[i])) b4 = b4ext[i];
+    if (!defo[i]) b2 = b4 = 0.0;
+    d.ws_R[i] = R;
+    d.ws_a[i] = a;
+    d.beta2[i] = b2;
+    d.beta4[i] = b4;
+    NucSys& ns = c->cfg.nuc[i];
+    ns.A = As[i];
+    ns.kind = As[i] == 1 ? kProton : (As[i] == 2 ? kDeuteron : kWoodsSaxon);
+    ns.deformed = defo[i] ? 1 : 0;
+    ns.R = (float)R;
+    ns.a = (float)a;
+    ns.beta2 = (float)b2;
+    ns.beta4 = (float)b4;
+    ns.rmax = (float)(R + 2.5);
+    ns.rwmax = (float)(1.0 / (1.0 + std::exp(-R / a)));
+  }
+  d.dmin = (p.nucleon_min_dist >= 0) ? p.nucleon_min_dist : 0.9;
+
+  // exact collision predicate: largest n = dxq^2 + dyq^2 with
+  // fl(fl(sqrt(n q^2))^2) <= dsq (MCnucl.cpp:359-366)
+  {
+    auto hit = [&](long long n) {
+      const double d2 = (double)n * (kQ * kQ);
+      const double b = std::sqrt(d2);
+      return b * b <= d.dsq;
+    };
+    long long lo = 0, hi = (long long)1 << 52;
+    while (lo < hi) {
+      const long long mid = lo + (hi - lo + 1) / 2;
+      if (hit(mid)) lo = mid; else hi = mid - 1;
+    }
+    d.hit_n_max = lo;
+  }
+  // KLN table step (MCnucl.cpp:915-919)
+  d.kln_dT = 10.0 / d.siginNN / 1 / std::max(1, p.tmax_subdivision);
+  d.kln_tmax = std::max(1, p.tmax_subdivision) * (p.tmax - 1) + 1;
+
+  DevCfg& g = c->cfg;
+  g.dmin2 = (float)(d.dmin * d.dmin);
+  g.bmin2 = p.bmin * p.bmin;
+  g.bmax2 = p.bmax * p.bmax;
+  g.npmin = p.Npmin;
+  g.npmax = p.Npmax;
+  g.max_attempts = p.max_attempts > 0 ? p.max_attempts : 10000;
+  g.max_pairs = p.max_pairs > 0 ? p.max_pairs : std::min(p.Aproj * p.Atarg, 8192);
+  g.hit_n = d.hit_n_max;
+  g.hit_lim = (int)std::floor(std::sqrt((double)d.hit_n_max)) + 1;
+  g.fluct = p.cc_fluctuation_model == 6;
+  {
+    // MCnucl.cpp:1271-1301 (shape_of_entropy != 3)
+    const double eps = 1e-8, th = p.cc_fluctuation_Gamma_theta, gk = 1. / th;
+    g.gk_part = (float)((1 - p.alpha + eps) / 2. * gk);
+    g.gs_part = (float)(2. / (1 - p.alpha + eps) * th);
+    g.gk_bin = (float)((p.alpha + eps) * gk);
+    g.gs_bin = (float)(1. / (p.alpha + eps) * th);
+    if (g.fluct && !(th > 0)) throw SmcError(SMC_ERR_ARG, "cc_fluctuation_Gamma_theta must be > 0");
+  }
+  g.nx = d.nx;
+  g.ny = d.ny;
+  g.xmin = d.xmin;
+  g.ymin = d.ymin;
+  g.dx = p.dx;
+  g.dy = p.dy;
+  g.w = (float)w;
+  g.inv2w2 = (float)(1.0 / (2 * w * w));
+  g.norm = (float)(1 / (2 * M_PI * w * w));
+  g.cut2 = (float)(25. * w * w);
+  const double hb = 4 * std::max(w, p.quark_width);
+  g.hb_h = (int)std::llround(hb / kH);
+  g.alpha = (float)p.alpha;
+  g.pref_part = (float)((1.0 - p.alpha) / 2.);
+  g.use_binary = p.alpha > 1e-8;
+  g.final_factor = p.finalFactor;
+  g.from_order = p.ecc_from_order;
+  g.to_order = p.ecc_to_order;
+}
+
+// Footprint tables reproducing the reference's FP64 index expressions.
+void build_tables(smc_ctx* c) {
+  const smc_params& p = c->p;
+  const double w = c->d.width;
+  const double d_max = 5. * w;  // MCnucl.cpp:455-456, 648-649
+  const double Xmin = c->d.xmin, Ymin = c->d.ymin, dx = p.dx, dy = p.dy;
+  const int nx = c->d.nx, ny = c->d.ny;
+  const double qw = p.quark_width;
+  const long long PH = (long long)(56.0 / kH), PQ = (long long)(120.0 / kQ), PO = (long long)(60.0 / kQ);
+  auto clampi = [](long long v, long long lo, long long hi) { return std::max(lo, std::min(hi, v)); };
+  // participant box edge: base box (Particle.cpp:19-20) united with the
+  // quark boxes at the centre (Particle.cpp:94-99, Quark.cpp:5-10)
+  auto boxL = [&](double x) { return std::min(x - 8 * w / 2, x - 8 * qw / 2); };
+  auto boxR = [&](double x) { return std::max(x + 8 * w / 2, x + 8 * qw / 2); };
+  std::vector<HostTable> T(kNumTables);
+  T[kWinLoX] = build_steps([&](long long ph) { double x = ph * kH; return clampi((int)((x - d_max - Xmin) / dx), 0, nx); }, -PH, PH);
+  T[kWinHiX] = build_steps([&](long long ph) { double x = ph * kH; return clampi((int)((x + d_max - Xmin) / dx), 0, nx); }, -PH, PH);
+  T[kWinLoY] = build_steps([&](long long ph) { double y = ph * kH; return clampi((int)((y - d_max - Ymin) / dy), 0, ny); }, -PH, PH);
+  T[kWinHiY] = build_steps([&](long long ph) { double y = ph * kH; return clampi((int)((y + d_max - Ymin) / dy), 0, ny); }, -PH, PH);
+  // MCnucl.cpp:831-844
+  T[kBoxLoX] = build_steps([&](long long ph) { double x = ph * kH; return clampi((int)((boxL(x) - Xmin) / dx), 0, nx); }, -PH, PH);
+  T[kBoxHiX] = build_steps([&](long long ph) { double x = ph * kH; return clampi((int)((boxR(x) - Xmin) / dx), 0, nx); }, -PH, PH);
+  T[kBoxLoY] = build_steps([&](long long ph) { double y = ph * kH; return clampi((int)((boxL(y) - Ymin) / dy), 0, ny); }, -PH, PH);
+  T[kBoxHiY] = build_steps([&](long long ph) { double y = ph * kH; return clampi((int)((boxR(y) - Ymin) / dy), 0, ny); }, -PH, PH);
+  // MakeDensity.cpp:2355-2357 and :2399-2400
+  T[kMaskX] = build_steps([&](long long dq) { return (long long)(int)((dq * kQ) / dx); }, 0, PQ);
+  T[kMaskY] = build_steps([&](long long dq) { return (long long)(int)((dq * kQ) / dy); }, 0, PQ);
+  T[kOffX] = build_steps([&](long long rq) { return (long long)(int)((rq * kQ - Xmin) / dx); }, -PO, 0);
+  T[kOffY] = build_steps([&](long long rq) { return (long long)(int)((rq * kQ - Ymin) / dy); }, -PO, 0);
+  size_t total = 0;
+  for (auto& t : T) total += t.thr.size();
+  c->dtab = dalloc<int>(total);
+  size_t off = 0;
+  for (int i = 0; i < kNumTables; ++i) {
+    StepTable& st = c->cfg.tab[i];
+    st.thr = c->dtab + off;
+    st.n = (int)T[i].thr.size();
+    st.vlo = T[i].vlo;
+    st.a = T[i].a;
+    st.b = T[i].b;
+    if (st.n) HIPCHK(hipMemcpy(c->dtab + off, T[i].thr.data(), st.n * sizeof(int), hipMemcpyHostToDevice));
+    off += T[i].thr.size();
+  }
+}
+
+void alloc_buffers(smc_ctx* c) {
+  const size_t E = (size_t)c->cap;
+  const size_t MP = (size_t)c->cfg.max_pairs;
+  EvBufs& b = c->buf;
+  b.pos = dalloc<int>(E * 2 * kAMax * 2);
+  b.fluct = dalloc<float>(E * 2 * kAMax);
+  b.wound = dalloc<int>(E * 2 * kAMax);
+  b.pairs = dalloc<unsigned>(E * MP);
+  b.pfluct = dalloc<float>(E * MP);
+  b.counts = dalloc<int>(E * 8);
+  b.bval = dalloc<double>(E);
+  b.raw = dalloc<int>(E * 2 * kAMax * 2);
+  b.src_cap = 2 * kAMax + (int)MP;
+  b.src = dalloc<int4>(E * (size_t)b.src_cap);
+  c->evout = dalloc<double>(E * SMC_EV_STRIDE);
+  HIPCHK(hipMemset(b.counts, 0, E * 8 * sizeof(int)));
+}
+
+float* grid_buffer(smc_ctx* c, float** slot) {
+  if (!*slot) *slot = dalloc<float>((size_t)c->cap * c->d.nx * c->d.ny);
+  return *slot;
+}
+
+void sync(smc_ctx* c) { HIPCHK(hipStreamSynchronize(c->stream)); }
+
+SmearSpec density_spec(const smc_ctx* c) {
+  const DevCfg& g = c->cfg;
+  SmearSpec s{};
+  if (c->p.which_mc_model == 5) {
+    // MCnucl.cpp:689-760: (1-alpha)/2 sum_part f T_n + alpha sum_coll f G
+    s.enabled[kSrcPartA] = s.enabled[kSrcPartB] = 1;
+    s.win[kSrcPartA] = s.win[kSrcPartB] = kWinBox;
+    s.scale[kSrcPartA] = s.scale[kSrcPartB] = g.pref_part * g.norm;
+    s.use_fluct[kSrcPartA] = s.use_fluct[kSrcPartB] = g.fluct;
+    s.enabled[kSrcPair] = g.use_binary;
+    s.win[kSrcPair] = kWin5;
+    s.scale[kSrcPair] = g.alpha * g.norm;
+    s.use_fluct[kSrcPair] = g.fluct;
+    s.combine = 0;
+  } else {
+    // MCnucl.cpp:762-811: sqrt(rho_A rho_B), rho from addDensity
+    s.enabled[kSrcPartA] = s.enabled[kSrcPartB] = 1;
+    s.win[kSrcPartA] = s.win[kSrcPartB] = kWinBox;
+    s.scale[kSrcPartA] = s.scale[kSrcPartB] = g.norm;
+    s.use_fluct[kSrcPartA] = s.use_fluct[kSrcPartB] = g.fluct;
+    s.acc[kSrcPartB] = 1;
+    s.combine = 1;
+  }
+  return s;
+}
+
+SmearSpec thickness_spec(const smc_ctx* c) {
+  // MCnucl.cpp:432-478: participants, +-5w window, getSmoothTn (no fluct)
+  SmearSpec s{};
+  s.enabled[kSrcPartA] = s.enabled[kSrcPartB] = 1;
+  s.win[kSrcPartA] = s.win[kSrcPartB] = kWin5;
+  s.scale[kSrcPartA] = s.scale[kSrcPartB] = c->cfg.norm;
+  s.acc[kSrcPartB] = 1;
+  s.combine = 2;
+  return s;
+}
+
+void run_density(smc_ctx* c) {
+  if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
+  if (c->p.which_mc_model == 1) throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN density not available yet");
+  float* s = grid_buffer(c, &c->grid);
+  TimerScope ts(c, SMC_K_SMEAR);
+  check_launch(launch_smear(c->cfg, c->buf, density_spec(c), c->nev, s, nullptr, c->stream), "k_smear");
+  ts.done();
+  c->have_density = 1;
+}
+
+void run_moments(smc_ctx* c, int mode) {
+  TimerScope ts(c, SMC_K_MOMENTS);
+  DevCfg g = c->cfg;
+  if (mode == 1) g.final_factor = 1.0;
+  check_launch(launch_moments(g, c->buf, c->grid, c->nev, c->evout, mode, c->stream), "k_moments");
+  ts.done();
+}
+
+void run_event(smc_ctx* c, int nev, uint64_t seed, uint64_t event0, int fed) {
+  TimerScope ts(c, SMC_K_EVENT);
+  check_launch(launch_event(c->cfg, c->buf, nev, seed, event0, fed, c->stream), "k_event");
+  ts.done();
+  c->nev = nev;
+  c->have_events = 1;
+  c->have_density = c->have_thickness = c->have_moments = 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+smc_status smc_params_default(smc_params* p) {
+  if (!p) return SMC_ERR_ARG;
+  std::memset(p, 0, sizeof(*p));
+  // parameters.dat of the reference
+  p->which_mc_model = 7;
+  p->sub_model = 1;
+  p->lambda = 0.138;
+  p->tmax = 71;
+  p->tmax_subdivision = 3;
+  p->alpha = 0.118;
+  p->Aproj = 208;
+  p->Atarg = 208;
+  p->proj_deformed = 0;
+  p->targ_deformed = 0;
+  p->shape_of_nucleons = 2;
+  p->collision_criterion = 2;
+  p->shape_of_entropy = 2;
+  p->quark_width = 0.3;
+  p->gauss_nucl_width = 0.812;
+  p->ecm = 5020.;
+  p->bmin = 0.;
+  p->bmax = 20.;
+  p->Npmin = 2;
+  p->Npmax = 500;
+  p->cc_fluctuation_model = 6;
+  p->cc_fluctuation_Gamma_theta = 0.75;
+  p->maxx = 15.;
+  p->maxy = 15.;
+  p->dx = 0.1;
+  p->dy = 0.1;
+  p->finalFactor = 40.0;
+  p->ecc_from_order = 1;
+  p->ecc_to_order = 9;
+  // extensions: reference behaviour
+  p->sigma_nn_mb = -1;
+  p->proj_ws_R = p->proj_ws_a = p->targ_ws_R = p->targ_ws_a = -1;
+  p->proj_beta2 = p->proj_beta4 = p->targ_beta2 = p->targ_beta4 = NAN;
+  p->nucleon_min_dist = -1;
+  p->kln_n_pt = 32;
+  p->kln_n_kt = 32;
+  p->kln_n_phi = 48;
+  p->kln_pt_min = 0.1;
+  p->kln_pt_max = 12.0;
+  p->kln_direct = 0;
+  p->max_events = 1024;
+  p->max_pairs = 0;
+  p->max_attempts = 10000;
+  p->smear_path = 0;
+  return SMC_OK;
+}
+
+size_t smc_params_sizeof(void) { return sizeof(smc_params); }
+size_t smc_derived_sizeof(void) { return sizeof(smc_derived); }
+
+smc_status smc_create(smc_ctx** out, const smc_params* p) {
+  if (!out || !p) return SMC_ERR_ARG;
+  *out = nullptr;
+  smc_ctx* c = new smc_ctx();
+  try {
+    c->p = *p;
+    c->cap = p->max_events > 0 ? p->max_events : 1024;
+    validate_and_derive(c);
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    build_tables(c);
+    alloc_buffers(c);
+    *out = c;
+    return SMC_OK;
+  } catch (const SmcError& e) {
+    static thread_local std::string last;
+    last = e.what();
+    std::fprintf(stderr, "smc_create: %s\n", e.what());
+    smc_destroy(c);
+    return e.code;
+  }
+}
+
+void smc_destroy(smc_ctx* c) {
+  if (!c) return;
+  for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
+  void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
+                  c->buf.bval, c->buf.raw, c->buf.src, c->dtab, c->grid, c->ta, c->tb, c->evout,
+                  c->kln_table};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* smc_last_error(const smc_ctx* c) { return c ? c->err.c_str() : "null ctx"; }
+
+smc_status smc_get_derived(const smc_ctx* c, smc_derived* out) {
+  if (!c || !out) return SMC_ERR_ARG;
+  *out = c->d;
+  return SMC_OK;
+}
+
+smc_status smc_sample(smc_ctx* c, uint64_t seed, uint64_t event0, int nev) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    run_event(c, nev, seed, event0, 0);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_set_positions(smc_ctx* c, int nev, const double* xy_proj, const double* xy_targ,
+                             const double* b, uint64_t fluct_seed) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    if (!xy_proj || !xy_targ) throw SmcError(SMC_ERR_ARG, "null positions");
+    const int A = c->p.Aproj, B = c->p.Atarg;
+    std::vector<int> raw((size_t)nev * 2 * kAMax * 2, 0);
+    std::vector<double> bv((size_t)nev, 0.0);
+    for (int e = 0; e < nev; ++e) {
+      for (int nu = 0; nu < 2; ++nu) {
+        const int n = nu ? B : A;
+        const double* src = (nu ? xy_targ : xy_proj) + (size_t)e * n * 2;
+        for (int i = 0; i < 2 * n; ++i) {
+          const double v = src[i];
+          if (!(std::fabs(v) < 56.0)) throw SmcError(SMC_ERR_ARG, "nucleon position beyond +-56 fm");
+          raw[((size_t)e * 2 + nu) * kAMax * 2 + i] = (int)std::llround(v / kQ);
+        }
+      }
+      bv[e] = b ? b[e] : 0.0;
+    }
+    HIPCHK(hipMemcpyAsync(c->buf.raw, raw.data(), raw.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->buf.bval, bv.data(), bv.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    run_event(c, nev, fluct_seed, 0, 1);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_collide(smc_ctx* c, int* npart_proj, int* npart_targ, int* ncoll) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events");
+    std::vector<int> cnt((size_t)c->nev * 8);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->buf.counts, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    int st = SMC_OK;
+    for (int e = 0; e < c->nev; ++e) {
+      if (npart_proj) npart_proj[e] = cnt[(size_t)e * 8 + 0];
+      if (npart_targ) npart_targ[e] = cnt[(size_t)e * 8 + 1];
+      if (ncoll) ncoll[e] = cnt[(size_t)e * 8 + 2];
+      const int s = cnt[(size_t)e * 8 + 3];
+      if (s & kStNoEvent) st = SMC_ERR_NO_EVENT;
+      else if ((s & kStPairOverflow) && st == SMC_OK) st = SMC_ERR_CAPACITY;
+    }
+    if (st == SMC_ERR_NO_EVENT) c->err = "no accepted event within max_attempts";
+    if (st == SMC_ERR_CAPACITY) c->err = "collision pairs exceed max_pairs";
+    return st;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_get_nucleons(smc_ctx* c, int ev, double* xy_proj, int* wound_proj, float* fluct_proj,
+                            double* xy_targ, int* wound_targ, float* fluct_targ) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events || ev < 0 || ev >= c->nev) throw SmcError(SMC_ERR_ARG, "event index out of range");
+    std::vector<int> pos(2 * kAMax * 2), wd(2 * kAMax);
+    std::vector<float> fl(2 * kAMax);
+    const size_t base = (size_t)ev * 2 * kAMax;
+    HIPCHK(hipMemcpyAsync(pos.data(), c->buf.pos + base * 2, pos.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(wd.data(), c->buf.wound + base, wd.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(fl.data(), c->buf.fluct + base, fl.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    for (int nu = 0; nu < 2; ++nu) {
+      const int n = nu ? c->p.Atarg : c->p.Aproj;
+      double* xy = nu ? xy_targ : xy_proj;
+      int* w = nu ? wound_targ : wound_proj;
+      float* f = nu ? fluct_targ : fluct_proj;
+      for (int i = 0; i < n; ++i) {
+        const size_t k = (size_t)nu * kAMax + i;
+        if (xy) { xy[2 * i] = pos[2 * k] * kQ; xy[2 * i + 1] = pos[2 * k + 1] * kQ; }
+        if (w) w[i] = wd[k];
+        if (f) f[i] = fl[k];
+      }
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_get_pairs(smc_ctx* c, int ev, int cap, int* pair_proj, int* pair_targ, float* fluct,
+                         int* npairs) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events || ev < 0 || ev >= c->nev) throw SmcError(SMC_ERR_ARG, "event index out of range");
+    int cnt[8];
+    HIPCHK(hipMemcpyAsync(cnt, c->buf.counts + (size_t)ev * 8, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    const int n = std::min(cnt[2], c->cfg.max_pairs);
+    if (npairs) *npairs = cnt[2];
+    const int m = std::min(n, cap);
+    if (m > 0) {
+      std::vector<unsigned> pr(m);
+      std::vector<float> pf(m);
+      const size_t base = (size_t)ev * c->cfg.max_pairs;
+      HIPCHK(hipMemcpyAsync(pr.data(), c->buf.pairs + base, m * sizeof(unsigned), hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(hipMemcpyAsync(pf.data(), c->buf.pfluct + base, m * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+      for (int k = 0; k < m; ++k) {
+        if (pair_proj) pair_proj[k] = (int)(pr[k] & 0xFFFFu);
+        if (pair_targ) pair_targ[k] = (int)(pr[k] >> 16);
+        if (fluct) fluct[k] = pf[k];
+      }
+    }
+    if (cnt[2] > cap) { c->err = "pair buffer too small"; return SMC_ERR_CAPACITY; }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_thickness(smc_ctx* c, float* TA, float* TB) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events");
+    float* a = grid_buffer(c, &c->ta);
+    float* b = grid_buffer(c, &c->tb);
+    TimerScope ts(c, SMC_K_SMEAR);
+    check_launch(launch_smear(c->cfg, c->buf, thickness_spec(c), c->nev, a, b, c->stream), "k_smear");
+    ts.done();
+    c->have_thickness = 1;
+    const size_t n = (size_t)c->nev * c->d.nx * c->d.ny;
+    if (TA) HIPCHK(hipMemcpyAsync(TA, a, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (TB) HIPCHK(hipMemcpyAsync(TB, b, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_density(smc_ctx* c, float* s, double* dndy) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    run_density(c);
+    const size_t n = (size_t)c->nev * c->d.nx * c->d.ny;
+    if (s) HIPCHK(hipMemcpyAsync(s, c->grid, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    int st = SMC_OK;
+    if (dndy) {
+      run_moments(c, 1);
+      std::vector<double> ev((size_t)c->nev * SMC_EV_STRIDE);
+      HIPCHK(hipMemcpyAsync(ev.data(), c->evout, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+      for (int e = 0; e < c->nev; ++e) {
+        dndy[e] = ev[(size_t)e * SMC_EV_STRIDE + SMC_EV_TOTAL];
+        if (dndy[e] < 1e-15) st = SMC_ERR_DNDY_ZERO;  // MCnucl.cpp:812-816
+      }
+      if (st) c->err = "dndy = 0 for an event";
+    }
+    sync(c);
+    return st;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_moments(smc_ctx* c, double* ev_out) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_density) run_density(c);
+    run_moments(c, 0);
+    c->have_moments = 1;
+    if (ev_out)
+      HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)c->nev * SMC_EV_STRIDE * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_run(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* ev_out) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    run_event(c, nev, seed, event0, 0);
+    run_density(c);
+    run_moments(c, 0);
+    c->have_moments = 1;
+    if (ev_out) {
+      HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_fetch_events(smc_ctx* c, int nev, double* ev_out) {
+  if (!c || !ev_out) return SMC_ERR_ARG;
+  try {
+    if (!c->have_moments || nev > c->nev) throw SmcError(SMC_ERR_STATE, "no moments for these events");
+    HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double), hipMemcpyDeviceToHost,
+                          c->stream));
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_kln_dndy(smc_ctx* c, int, const double*, const double*, double*) {
+  if (!c) return SMC_ERR_ARG;
+  c->err = "MC-KLN quadrature not built yet";
+  return SMC_ERR_UNSUPPORTED;
+}
+
+smc_status smc_kln_make_table(smc_ctx* c, double*) {
+  if (!c) return SMC_ERR_ARG;
+  c->err = "MC-KLN table not built yet";
+  return SMC_ERR_UNSUPPORTED;
+}
+
+smc_status smc_set_timing(smc_ctx* c, int on) {
+  if (!c) return SMC_ERR_ARG;
+  c->timing = on;
+  return SMC_OK;
+}
+
+smc_status smc_get_timing(smc_ctx* c, int k, double* total_ms, int* launches) {
+  if (!c || k < 0 || k >= SMC_K_COUNT) return SMC_ERR_ARG;
+  try {
+    sync(c);
+    for (auto& pe : c->pending) {
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, pe.a, pe.b));
+      c->tms[pe.k] += ms;
+      c->tcount[pe.k] += 1;
+      (void)hipEventDestroy(pe.a);
+      (void)hipEventDestroy(pe.b);
+    }
+    c->pending.clear();
+    if (total_ms) *total_ms = c->tms[k];
+    if (launches) *launches = c->tcount[k];
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_synchronize(smc_ctx* c) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// smc_internal.h -- structures shared by the host side (smc_api.cpp) and the
+// gfx950 kernels (smc_kernels.hip).  Not part of the C-ABI.
+#pragma once
+#include <stdint.h>
+
+#include "../../include/smc.h"
+
+namespace smc {
+
+constexpr int kAMax = SMC_MAX_A;       // nucleons per nucleus held in LDS
+constexpr double kQ = SMC_POS_QUANTUM_FM;  // position quantum (2^-24 fm)
+constexpr double kH = kQ * 0.5;            // half quantum: pair midpoints
+constexpr double kPosLimitFm = 60.0;       // |x|,|y| bound for int32 math
+
+// Nucleus kinds (Nucleus.cpp:199-316)
+enum NucKind : int { kWoodsSaxon = 0, kProton = 1, kDeuteron = 2 };
+
+struct NucSys {
+  int A;
+  int kind;
+  int deformed;
+  float R, a, beta2, beta4;
+  float rmax;   // R + 2.5 (Nucleus.cpp:118)
+  float rwmax;  // 1/(1+exp(-R/a)) (Nucleus.cpp:119)
+};
+
+// Monotone step function f: int32 -> [vlo, vlo+n] given by thresholds:
+// f(p) = vlo + #{k : thr[k] <= p}.  Guess: vlo-relative index ~ a*p + b.
+struct StepTable {
+  const int* thr;  // device pointer, n entries, non-decreasing
+  int n;
+  int vlo;
+  float a, b;
+};
+
+// Footprint tables (exact reproduction of the reference's (int) casts).
+enum TableId : int {
+  kWinLoX = 0,  // (int)((x - 5w - Xmin)/dx)   clamped [0,nx]   p in half-quanta
+  kWinHiX,      // (int)((x + 5w - Xmin)/dx)
+  kWinLoY,
+  kWinHiY,
+  kBoxLoX,      // (int)((fl(x - hb) - Xmin)/dx)  hb = 4*max(w,qw)
+  kBoxHiX,
+  kBoxLoY,
+  kBoxHiY,
+  kMaskX,       // (int)(d/dx)   d = box edge - region edge >= 0, quanta
+  kMaskY,       // (int)(d/dy)
+  kOffX,        // (int)((rXL - Xmin)/dx)   rXL in quanta (may be < Xmin)
+  kOffY,
+  kNumTables
+};
+
+struct DevCfg {
+  NucSys nuc[2];
+  float dmin2;           // hard core squared [fm^2]
+  double bmin2, bmax2;   // impact parameter range squared
+  int npmin, npmax;
+  int max_attempts;
+  int max_pairs;
+  long long hit_n;       // collide iff dxq^2+dyq^2 <= hit_n (quanta^2)
+  int hit_lim;           // |dxq| > hit_lim cannot collide
+  int fluct;             // Gamma factors on
+  float gk_part, gs_part, gk_bin, gs_bin;  // MCnucl.cpp:1271-1301
+  // grid
+  int nx, ny;
+  double xmin, ymin, dx, dy;
+  float w, inv2w2, norm, cut2;  // width, 1/(2w^2), 1/(2 pi w^2), 25 w^2
+  int hb_h;              // box half-width 4*max(w,qw) in half-quanta (rounded)
+  float alpha, pref_part;
+  int use_binary;        // alpha > 1e-8 (MCnucl.cpp:724)
+  double final_factor;
+  int from_order, to_order;
+  StepTable tab[kNumTables];
+};
+
+// Per-event device buffers (structure of arrays).
+struct EvBufs {
+  int* pos;        // [ev][2][kAMax][2] sorted positions (quanta)
+  float* fluct;    // [ev][2][kAMax]
+  int* wound;      // [ev][2][kAMax] participant rank (wound order) or -1
+  unsigned* pairs; // [ev][max_pairs] proj index | targ index << 16
+  float* pfluct;   // [ev][max_pairs]
+  int* counts;     // [ev][8]: npartA, npartB, ncoll, status, attempts, nsrc
+  double* bval;    // [ev]
+  int* raw;        // [ev][2][kAMax][2] fed positions (quanta), input order
+  int4* src;       // [ev][src_cap] {x_h, y_h, fluct bits, kind}
+  int src_cap;
+};
+
+// Event status bits
+constexpr int kStOk = 0;
+constexpr int kStNoEvent = 1;
+constexpr int kStPairOverflow = 2;
+constexpr int kStPosRange = 4;
+
+// Smear kernel source kinds and accumulation modes
+enum SrcKind : int { kSrcPartA = 0, kSrcPartB = 1, kSrcPair = 2 };
+enum WinType : int { kWinBox = 0, kWin5 = 1 };
+
+struct SmearSpec {
+  int enabled[3];   // per kind
+  int win[3];       // WinType per kind
+  float scale[3];   // weight multiplier per kind
+  int use_fluct[3];
+  int acc[3];       // accumulator (0/1) per kind
+  int combine;      // 0: out0 = acc0; 1: out0 = sqrt(acc0*acc1); 2: out0=acc0,out1=acc1
+};
+
+// Launch wrappers (smc_kernels.hip)
+int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
+                 unsigned long long event0, int fed, void* stream);
+int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,
+                 float* out0, float* out1, void* stream);
+int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev,
+                   double* ev_out, int mode, void* stream);
+int launch_kln_points(const DevCfg& cfg, int n, const double* ta, const double* tb,
+                      double* out, const float* nodes, int npt, int nkt, int nphi,
+                      double ecm, double lambda, double ptmin, double ptmax, void* stream);
+
+}  // namespace smc

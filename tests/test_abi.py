@@ -1,0 +1,69 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol that
+include/smc.h declares, its struct layouts match the binding, its defaults
+are the reference's parameters.dat, and parameter validation maps onto the
+documented status codes.  No device compute is issued here."""
+import ctypes as C
+import json
+import math
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_vectors.json")))
+
+
+def _declared_symbols():
+    txt = open(os.path.join(ROOT, "include", "smc.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(smc_[a-z_]+)\s*\(", txt)))
+
+
+def test_library_exports_every_declared_symbol():
+    from supermc_amd import _native
+    lib = _native.load()
+    syms = _declared_symbols()
+    assert len(syms) >= 20
+    for s in syms:
+        assert hasattr(lib, s), s
+    assert set(syms) == set(_native.SIGNATURES), "binding and header disagree"
+
+
+def test_struct_layouts():
+    from supermc_amd import _native
+    lib = _native.load()
+    assert lib.smc_params_sizeof() == C.sizeof(_native.SmcParams)
+    assert lib.smc_derived_sizeof() == C.sizeof(_native.SmcDerived)
+
+
+def test_defaults_are_reference_parameters_dat():
+    from supermc_amd import default_params
+    p = default_params()
+    for k, v in GOLD["parameters_dat"].items():
+        if k.startswith("_"):
+            continue
+        attr = "lambda_" if k == "lambda" else k
+        assert getattr(p, attr) == pytest.approx(v), k
+    assert math.isnan(p.proj_beta2) and p.sigma_nn_mb < 0
+
+
+@pytest.mark.parametrize("override,code", [
+    (dict(which_mc_model=3), 7),
+    (dict(which_mc_model=5, sub_model=2, collision_criterion=1), 7),
+    (dict(which_mc_model=5, sub_model=1, collision_criterion=2), 7),
+    (dict(which_mc_model=5, sub_model=1, collision_criterion=1, Aproj=16), 7),
+    (dict(which_mc_model=5, sub_model=1, collision_criterion=1, Aproj=300), 1),
+    (dict(which_mc_model=5, sub_model=1, collision_criterion=1, ecc_from_order=4, ecc_to_order=2), 1),
+    (dict(which_mc_model=5, sub_model=1, collision_criterion=1, cc_fluctuation_model=1), 7),
+    (dict(which_mc_model=5, sub_model=1, collision_criterion=1, shape_of_nucleons=4, gauss_nucl_width=0.08,
+          ecm=5020.0), 7),
+])
+def test_validation_status_codes(override, code):
+    from supermc_amd import _native, default_params
+    lib = _native.load()
+    p = default_params(**override)
+    h = C.c_void_p()
+    rc = lib.smc_create(C.byref(h), C.byref(p))
+    assert rc == code
+    assert not h.value

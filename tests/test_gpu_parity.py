@@ -1,0 +1,182 @@
+"""Parity of the HIP path against the FP64 oracle on the same nucleon
+positions (and the same Gamma factors).
+
+Bars (written here, checked below):
+  Npart, Ncoll, participant sets, wound order, pair lists: bit-exact
+  T_A, T_B, entropy grid: relative L-inf (max|d| / max|ref|) <= 1e-5
+  eps_n (r^2 and r^n weights, n = 2..5): absolute <= 1e-6
+  <r^n>, dS/dy: relative <= 1e-6;  centre of mass: absolute <= 1e-6 fm
+"""
+import numpy as np
+import pytest
+
+import parity_helpers as H
+
+pytestmark = pytest.mark.gpu
+
+GRID_TOL = 1e-5
+EPS_TOL = 1e-6
+
+
+def _ctx(p):
+    from supermc_amd import Context
+    return Context(p)
+
+
+def _check_collision(ctx, ev, o):
+    nuc = o["nuc"]
+    col = o["col"]
+    pa, pb, _ = o["gpu_pairs"]
+    # the HIP path already returns collision order
+    assert (o["perm_a"] == np.arange(len(o["perm_a"]))).all()
+    assert (o["perm_b"] == np.arange(len(o["perm_b"]))).all()
+    assert (nuc["wound_proj"] >= 0).sum() == col["npart_a"]
+    assert (nuc["wound_targ"] >= 0).sum() == col["npart_b"]
+    assert ((nuc["wound_proj"] >= 0) == (col["wound_a"] == 1)).all()
+    assert ((nuc["wound_targ"] >= 0) == (col["wound_b"] == 1)).all()
+    assert len(pa) == col["ncoll"]
+    assert (pa == col["pair_a"]).all() and (pb == col["pair_b"]).all()
+    # wound order (Nucleus.cpp:325-331)
+    assert (o["ia"] == col["order_a"]).all()
+    assert (o["ib"] == col["order_b"]).all()
+
+
+def _check_event_record(rec, o, p):
+    m = o["mom"]
+    for n in range(p.ecc_from_order, p.ecc_to_order + 1):
+        for key, off in (("mom_re", 0), ("mom_im", 10), ("momp_re", 20), ("momp_im", 30)):
+            assert abs(rec[off + n] - m[key][n]) <= EPS_TOL, (key, n, rec[off + n], m[key][n])
+    for n in range(10):
+        assert rec[40 + n] == pytest.approx(m["rn"][n], rel=1e-6)
+    assert abs(rec[50] - m["xc"]) <= 1e-6 and abs(rec[51] - m["yc"]) <= 1e-6
+    assert rec[52] == pytest.approx(m["total"], rel=1e-6)
+
+
+@pytest.mark.parametrize("system", ["AuAu", "PbPb"])
+def test_fed_positions_full_chain(system):
+    p = H.glauber_params(system, max_events=12)
+    bvals = np.array([0.0, 1.0, 2.5, 4.0, 5.5, 7.0, 8.5, 10.0, 11.5, 12.5, 13.5, 14.5])
+    xa, xb, bs = H.cpu_configs(p, len(bvals), seed=1234 + p.Aproj, bvals=bvals)
+    with _ctx(p) as ctx:
+        ctx.set_positions(xa, xb, bs, fluct_seed=99)
+        npa, npb, nc, rc = ctx.collide(allow=(4,))
+        s, dndy, _ = ctx.density()
+        rec = ctx.moments()
+        ta, tb = ctx.thickness()
+        for e in range(len(bvals)):
+            o = H.oracle_event(ctx, e)
+            # positions come back exactly (quantised input)
+            assert np.array_equal(np.sort(o["nuc"]["xy_proj"][:, 0]), np.sort(xa[e][:, 0]))
+            assert npa[e] == o["col"]["npart_a"] and npb[e] == o["col"]["npart_b"]
+            assert nc[e] == o["col"]["ncoll"]
+            if nc[e] == 0:
+                continue
+            _check_collision(ctx, e, o)
+            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL
+            assert H.rel_linf(ta[e], o["TA"]) <= GRID_TOL
+            assert H.rel_linf(tb[e], o["TB"]) <= GRID_TOL
+            assert dndy[e] == pytest.approx(o["dndy"], rel=1e-6)
+            _check_event_record(rec[e], o, p)
+
+
+def test_sampled_events_match_oracle():
+    p = H.glauber_params("PbPb", max_events=48)
+    with _ctx(p) as ctx:
+        ev = ctx.run(seed=2026, event0=0, nev=48)
+        assert (ev[:, 57] == 0).all()
+        for e in range(0, 48, 3):
+            o = H.oracle_event(ctx, e)
+            _check_collision(ctx, e, o)
+            rec = ev[e]
+            assert rec[54] == o["col"]["npart_a"] + o["col"]["npart_b"]
+            assert rec[55] == o["col"]["ncoll"]
+            assert 2 <= rec[54] <= 500 and rec[55] > 0
+            _check_event_record(rec, o, p)
+        s, _, _ = ctx.density()
+        for e in range(0, 48, 7):
+            o = H.oracle_event(ctx, e)
+            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL
+
+
+def test_counter_based_events_are_batch_independent():
+    p = H.glauber_params("PbPb", max_events=32)
+    with _ctx(p) as ctx:
+        a = ctx.run(seed=7, event0=100, nev=32)
+        b = ctx.run(seed=7, event0=110, nev=8)
+        assert np.array_equal(a[10:18], b)
+        c = ctx.run(seed=8, event0=100, nev=32)
+        assert not np.array_equal(a[:, 54], c[:, 54])
+
+
+def test_trento_model_density():
+    p = H.glauber_params("PbPb", which_mc_model=7, max_events=6)
+    xa, xb, bs = H.cpu_configs(p, 6, seed=77, bvals=np.array([0.0, 3.0, 6.0, 9.0, 11.0, 13.0]))
+    with _ctx(p) as ctx:
+        ctx.set_positions(xa, xb, bs, fluct_seed=5)
+        ctx.collide(allow=(4,))
+        s, dndy, _ = ctx.density()
+        rec = ctx.moments()
+        for e in range(6):
+            o = H.oracle_event(ctx, e)
+            if o["col"]["ncoll"] == 0:
+                continue
+            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL
+            _check_event_record(rec[e], o, p)
+
+
+def test_edge_cases_single_nucleons_and_no_collision():
+    # p+p head-on (A=1 nucleons sit at +-b/2) and a miss
+    p = H.glauber_params("AuAu", Aproj=1, Atarg=1, max_events=3)
+    xa = np.array([[[0.25, 0.0]], [[3.0, 0.0]], [[0.0, 0.6]]])
+    xb = np.array([[[-0.25, 0.0]], [[-3.0, 0.0]], [[0.0, -0.55]]])
+    with _ctx(p) as ctx:
+        ctx.set_positions(xa, xb, np.array([0.5, 6.0, 0.0]))
+        npa, npb, nc, _ = ctx.collide()
+        d = ctx.derived
+        for e in range(3):
+            dist2 = ((xa[e, 0] - xb[e, 0]) ** 2).sum()
+            hit = np.sqrt(dist2) ** 2 <= d.dsq
+            assert nc[e] == int(hit) and npa[e] == int(hit) and npb[e] == int(hit)
+
+
+def test_gamma_factors_follow_gamma_law():
+    from scipy import stats
+    p = H.glauber_params("PbPb", max_events=256)
+    with _ctx(p) as ctx:
+        ctx.sample(seed=31, event0=0, nev=256)
+        fp, fc = [], []
+        for e in range(256):
+            nuc = ctx.nucleons(e)
+            fp += list(nuc["fluct_proj"][nuc["wound_proj"] >= 0])
+            fc += list(ctx.pairs(e)[2])
+    eps, th, a = 1e-8, 1.25, 0.14
+    kp, sp = (1 - a + eps) / 2 / th, 2 / (1 - a + eps) * th
+    kb, sb = (a + eps) / th, th / (a + eps)
+    assert stats.kstest(fp, stats.gamma(kp, scale=sp).cdf).pvalue > 1e-3
+    assert stats.kstest(fc, stats.gamma(kb, scale=sb).cdf).pvalue > 1e-3
+
+
+def test_sampled_nuclei_follow_reference_sampler():
+    """Two-sample KS of GPU-sampled nucleon radii (about the nucleus centre)
+    and Npart against the oracle's restatement of Nucleus::populate."""
+    from scipy import stats
+    import oracle
+    p = H.glauber_params("PbPb", bmin=0.0, bmax=0.0, Npmin=0, Npmax=1000, max_events=200)
+    with _ctx(p) as ctx:
+        ctx.sample(seed=4242, event0=0, nev=200)
+        npa, npb, nc, _ = ctx.collide()
+        rg = []
+        for e in range(200):
+            nuc = ctx.nucleons(e)
+            rg += list(np.hypot(nuc["xy_proj"][:, 0], nuc["xy_proj"][:, 1]))
+    rng = oracle.Rng(99)
+    rc, npc = [], []
+    dsq = oracle.lib().orc_dsq(64.0)
+    for e in range(200):
+        a = rng.populate(208, 6.62, 0.546, 0.4, 0.0)
+        b = rng.populate(208, 6.62, 0.546, 0.4, 0.0)
+        rc += list(np.hypot(a[:, 0], a[:, 1]))
+        n, wa, wb = oracle.collide_bruteforce(a, b, dsq)
+        npc.append(wa.sum() + wb.sum())
+    assert stats.ks_2samp(rg, rc).pvalue > 1e-3
+    assert abs(np.mean(npa + npb) - np.mean(npc)) < 4 * np.std(npc) / np.sqrt(200) * 1.5
