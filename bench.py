@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark: batched MC-Glauber Pb+Pb on a 261^2 grid (events/s).
+
+Workload (one "step"): one batch of --events minimum-bias Pb+Pb events through
+the whole hot path on one MI355X -- impact parameter and Philox nucleon
+sampling (Woods-Saxon R=6.62 fm, a=0.546 fm, hard core 0.4 fm, A=208),
+acceptance (Ncoll > 0, 2 <= Npart <= 500), A x B disk test at sigma_NN=64 mb,
+Gamma fluctuations (k = 1/theta = 0.8), FP32 Gaussian smearing (w=0.5 fm,
+alpha=0.14) onto 261 x 261 cells (dx=0.1 fm, +-13 fm) and FP64 eps_2..eps_5 /
+<r^n> moments.  Inputs are generated on the device (counter-based RNG), so the
+timed region holds no host transfers.
+
+Prints one JSON line (see the keys below).  The CPU baseline times the FP64
+oracle's event loop (a restatement of the reference's MakeDensity event
+loop, no file output) on a bounded sample, one process per core.
+"""
+import argparse
+import json
+import multiprocessing as mp
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK = 157.3e12
+HBM_PEAK = 8.0e12
+
+
+def bench_params(nev):
+    from supermc_amd import default_params
+    return default_params(
+        which_mc_model=5, sub_model=1, shape_of_nucleons=4, gauss_nucl_width=0.5, shape_of_entropy=2,
+        collision_criterion=1, Aproj=208, Atarg=208, proj_ws_R=6.62, proj_ws_a=0.546, targ_ws_R=6.62,
+        targ_ws_a=0.546, nucleon_min_dist=0.4, sigma_nn_mb=64.0, alpha=0.14, cc_fluctuation_model=6,
+        cc_fluctuation_Gamma_theta=1.25, bmin=0.0, bmax=20.0, Npmin=2, Npmax=500, maxx=13.0, maxy=13.0, dx=0.1,
+        dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev, max_pairs=4096)
+
+
+def _cpu_worker(args):
+    seed, nev = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    cfg = oracle.OrcEventCfg(A=208, B=208, R_a=6.62, a_a=0.546, R_b=6.62, a_b=0.546, dmin=0.4, siginNN=64.0,
+                             w=0.5, alpha=0.14, gamma_theta=1.25, fluct=1, bmin=0.0, bmax=20.0, npmin=2,
+                             npmax=500, maxx=13.0, dx=0.1, from_order=2, to_order=5)
+    t = time.perf_counter()
+    n, _ = oracle.run_events(cfg, seed, nev)
+    return n, time.perf_counter() - t
+
+
+def cpu_baseline(seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.lib()  # builds the oracle if needed, before forking
+    cores = max(1, min(16, os.cpu_count() or 1))
+    n1, t1 = _cpu_worker((1, 8))
+    per_core = max(4, int(seconds * n1 / t1))
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_worker, [(1000 + i, per_core) for i in range(cores)])
+    wall = time.perf_counter() - t
+    tot = sum(r[0] for r in res)
+    return dict(value=tot / wall, unit="events/s", cores=cores, kind="port",
+                sample=f"{tot} Pb+Pb events ({per_core} per core) through the FP64 oracle event loop "
+                       f"(sample -> collide -> T_A/T_B -> density -> eps_n, no file output), {wall:.1f} s wall")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--events", type=int, default=100000)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--seed", type=int, default=20261202)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(events shard by counter range across GPUs, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    from supermc_amd import Context, EV_NPART, EV_NCOLL, EV_STATUS, load
+    load()  # fails loudly when libsmc.so is missing
+    nev = args.events
+    ctx = Context(bench_params(nev))
+    print(f"[bench] Pb+Pb 261^2, {nev} events/step, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
+          flush=True)
+    for w in range(args.warmup):
+        ctx.run(args.seed, w * nev, nev, fetch=False)
+    ctx.synchronize()
+    ctx.set_timing(True)
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        ctx.run(args.seed, (args.warmup + k) * nev, nev, fetch=False)
+    ctx.synchronize()
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    value = nev * args.steps / elapsed
+    print(f"[bench] timed region {elapsed:.3f} s -> {value:.1f} events/s", file=sys.stderr, flush=True)
+
+    ev = ctx.fetch_events(nev)  # the last timed batch
+    bad = int((ev[:, EV_STATUS] != 0).sum())
+    sum_k = float((ev[:, EV_NPART] + ev[:, EV_NCOLL]).sum())
+    N = ctx.nx * ctx.ny
+    kernels = {}
+    for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments")):
+        ms, n = ctx.timing(kid)
+        kernels[name] = dict(avg_ms=ms / max(n, 1), launches=n)
+    dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
+    if dom == "k_smear":
+        # dense-equivalent contraction T = Gx diag(w) Gy^T: 2*N*K flops/event
+        work = 2.0 * N * sum_k
+        achieved = work / (kernels[dom]["avg_ms"] * 1e-3)
+        roof = dict(bound="mfma", achieved=achieved / 1e12, peak=FP32_PEAK / 1e12, unit="TFLOP/s",
+                    frac=achieved / FP32_PEAK)
+    else:
+        work = 4.0 * N * nev  # one FP32 grid per event
+        achieved = work / (kernels[dom]["avg_ms"] * 1e-3)
+        roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK)
+    roof["kernel"] = dom
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"{dom}_pbpb261_traffic.json")
+    if os.path.exists(tpath):
+        try:
+            traffic = json.load(open(tpath)).get("bytes_per_launch")
+        except Exception:
+            traffic = None
+    roof["traffic"] = traffic
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(args.cpu_seconds)
+
+    line = {
+        "metric": "events/s Pb+Pb 261^2 grid (Glauber, KLN); fwd+adjoint; eps_n err vs FP64",
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (Philox-sampled nuclei on the device)",
+        "config": {"workload": "batched MC-Glauber Pb+Pb min-bias, 261x261 grid, Gamma fluct k=0.8",
+                   "events_per_step": nev, "grid": [ctx.nx, ctx.ny], "sigma_nn_mb": 64.0, "moments_dtype": "f64",
+                   "mean_sources_per_event": sum_k / nev, "failed_events": bad,
+                   "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()}},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -769,6 +769,7 @@ smc_status smc_synchronize(smc_ctx* c) {
   if (!c) return SMC_ERR_ARG;
   try {
     sync(c);
+    HIPCHK(hipDeviceSynchronize());
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }
