@@ -35,7 +35,7 @@ def bench_params(nev):
         collision_criterion=1, Aproj=208, Atarg=208, proj_ws_R=6.62, proj_ws_a=0.546, targ_ws_R=6.62,
         targ_ws_a=0.546, nucleon_min_dist=0.4, sigma_nn_mb=64.0, alpha=0.14, cc_fluctuation_model=6,
         cc_fluctuation_Gamma_theta=1.25, bmin=0.0, bmax=20.0, Npmin=2, Npmax=500, maxx=13.0, maxy=13.0, dx=0.1,
-        dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev, max_pairs=4096)
+        dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev, max_pairs=4096, smear_path=1)
 
 
 def _cpu_worker(args):
@@ -106,11 +106,12 @@ def main():
     sum_k = float((ev[:, EV_NPART] + ev[:, EV_NCOLL]).sum())
     N = ctx.nx * ctx.ny
     kernels = {}
-    for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments")):
+    for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments"), (4, "k_grid")):
         ms, n = ctx.timing(kid)
-        kernels[name] = dict(avg_ms=ms / max(n, 1), launches=n)
+        if n:
+            kernels[name] = dict(avg_ms=ms / n, launches=n)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    if dom == "k_smear":
+    if dom in ("k_smear", "k_grid"):
         # dense-equivalent contraction T = Gx diag(w) Gy^T: 2*N*K flops/event
         work = 2.0 * N * sum_k
         achieved = work / (kernels[dom]["avg_ms"] * 1e-3)

@@ -402,17 +402,42 @@ SmearSpec thickness_spec(const smc_ctx* c) {
   return s;
 }
 
+// The fused grid kernel holds a grid of up to 272 x 272 cells in registers
+// and covers the single-accumulator densities (MC-Glauber).
+bool fused_ok(const smc_ctx* c) {
+  return c->p.smear_path != 2 && c->p.which_mc_model == 5 && c->d.nx <= 272 && c->d.ny <= 272;
+}
+
+// flags: 2 = eps_n record, 4 = grid sum only; the grid is always written.
+void run_grid(smc_ctx* c, int flags) {
+  float* s = grid_buffer(c, &c->grid);
+  TimerScope ts(c, SMC_K_GRID);
+  check_launch(launch_grid(c->cfg, c->buf, density_spec(c), c->nev, s, c->evout, flags, c->p.smear_path == 1,
+                           c->stream),
+               "k_grid");
+  ts.done();
+}
+
 void run_density(smc_ctx* c) {
   if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
   if (c->p.which_mc_model == 1) throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN density not available yet");
-  float* s = grid_buffer(c, &c->grid);
-  TimerScope ts(c, SMC_K_SMEAR);
-  check_launch(launch_smear(c->cfg, c->buf, density_spec(c), c->nev, s, nullptr, c->stream), "k_smear");
-  ts.done();
+  if (fused_ok(c)) {
+    run_grid(c, 4);
+  } else {
+    float* s = grid_buffer(c, &c->grid);
+    TimerScope ts(c, SMC_K_SMEAR);
+    check_launch(launch_smear(c->cfg, c->buf, density_spec(c), c->nev, s, nullptr, c->stream), "k_smear");
+    ts.done();
+  }
   c->have_density = 1;
 }
 
 void run_moments(smc_ctx* c, int mode) {
+  if (mode == 0 && fused_ok(c)) {
+    run_grid(c, 2);
+    return;
+  }
+  if (!c->have_density) run_density(c);
   TimerScope ts(c, SMC_K_MOMENTS);
   DevCfg g = c->cfg;
   if (mode == 1) g.final_factor = 1.0;
@@ -480,7 +505,7 @@ smc_status smc_params_default(smc_params* p) {
   p->max_events = 1024;
   p->max_pairs = 0;
   p->max_attempts = 10000;
-  p->smear_path = 0;
+  p->smear_path = 1;
   return SMC_OK;
 }
 
@@ -671,7 +696,7 @@ smc_status smc_density(smc_ctx* c, float* s, double* dndy) {
     if (s) HIPCHK(hipMemcpyAsync(s, c->grid, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     int st = SMC_OK;
     if (dndy) {
-      run_moments(c, 1);
+      if (!fused_ok(c)) run_moments(c, 1);  // the fused kernel already summed the grid
       std::vector<double> ev((size_t)c->nev * SMC_EV_STRIDE);
       HIPCHK(hipMemcpyAsync(ev.data(), c->evout, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       sync(c);
@@ -689,7 +714,7 @@ smc_status smc_density(smc_ctx* c, float* s, double* dndy) {
 smc_status smc_moments(smc_ctx* c, double* ev_out) {
   if (!c) return SMC_ERR_ARG;
   try {
-    if (!c->have_density) run_density(c);
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events");
     run_moments(c, 0);
     c->have_moments = 1;
     if (ev_out)
@@ -705,7 +730,6 @@ smc_status smc_run(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* 
   try {
     if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
     run_event(c, nev, seed, event0, 0);
-    run_density(c);
     run_moments(c, 0);
     c->have_moments = 1;
     if (ev_out) {

@@ -113,6 +113,8 @@ int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  float* out0, float* out1, void* stream);
 int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev,
                    double* ev_out, int mode, void* stream);
+int launch_grid(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
+                double* ev_out, int flags, int mfma, void* stream);
 int launch_kln_points(const DevCfg& cfg, int n, const double* ta, const double* tb,
                       double* out, const float* nodes, int npt, int nkt, int nphi,
                       double ecm, double lambda, double ptmin, double ptmax, void* stream);

@@ -11,199 +11,12 @@
 //  k_smear    2D Gaussian smearing of a source list onto the grid, one
 //             workgroup per (event, 16x64 tile), exact reference footprints
 //             via integer threshold tables.  (MCnucl.cpp:398-478, 626-866)
-//  k_moments  FP64 centre of mass, <r^n> and masked eps_n moments, one
-//             workgroup per event.  (MakeDensity.cpp:2244-2430,
-//             MCnucl.cpp:1303-1323)
+//  (moments: smc_grid.hip)
 #include <hip/hip_runtime.h>
 
-#include "smc_internal.h"
+#include "smc_device.h"
 
 namespace smc {
-
-// --------------------------------------------------------------- Philox
-// Philox4x32-10, Salmon, Moraes, Dror & Shaw, "Parallel random numbers: as
-// easy as 1, 2, 3", SC'11.  Counter-based: every draw is a pure function of
-// (seed, event, stream, index, attempt/iteration).
-__device__ __forceinline__ uint4 philox(uint4 c, uint2 k) {
-#pragma unroll
-  for (int r = 0; r < 10; ++r) {
-    const unsigned lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-    const unsigned lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
-    c = make_uint4(hi1 ^ c.y ^ k.x, lo1, hi0 ^ c.w ^ k.y, lo0);
-    k.x += 0x9E3779B9u;
-    k.y += 0xBB67AE85u;
-  }
-  return c;
-}
-
-enum Stream : unsigned {
-  kStrB = 1,
-  kStrOrient = 2,
-  kStrNucleon = 3,
-  kStrFluctA = 4,
-  kStrFluctB = 5,
-  kStrFluctPair = 6,
-  kStrDeuteron = 7,
-};
-
-__device__ __forceinline__ uint4 draw(uint2 key, unsigned long long ev, unsigned stream,
-                                      unsigned nucleus, unsigned idx, unsigned attempt,
-                                      unsigned iter) {
-  uint4 c;
-  c.x = (unsigned)ev;
-  c.y = ((unsigned)(ev >> 32) & 0xFFFFu) | ((stream & 0xFFu) << 16) | ((nucleus & 0xFFu) << 24);
-  c.z = idx;
-  c.w = (attempt << 16) | (iter & 0xFFFFu);
-  return philox(c, key);
-}
-
-// uniform in (0,1) from 24 bits
-__device__ __forceinline__ float u01(unsigned r) { return ((float)(r >> 8) + 0.5f) * (1.0f / 16777216.0f); }
-// uniform in (0,1) from 53 bits
-__device__ __forceinline__ double u53(unsigned a, unsigned b) {
-  unsigned long long m = ((unsigned long long)(a >> 5) << 26) | (b >> 6);
-  return ((double)m + 0.5) * (1.0 / 9007199254740992.0);
-}
-
-constexpr float kPi = 3.14159265358979323846f;
-constexpr float kQf = (float)kQ;
-constexpr float kHf = (float)kH;
-
-// ------------------------------------------------------------ helpers
-__device__ __forceinline__ int step_eval(const StepTable& T, int p) {
-  int k = (int)floorf(fmaf(T.a, (float)p, T.b));
-  k = max(0, min(T.n, k));
-  while (k < T.n && T.thr[k] <= p) ++k;
-  while (k > 0 && T.thr[k - 1] > p) --k;
-  return T.vlo + k;
-}
-
-__device__ __forceinline__ int quant(float v) { return (int)rintf(v * 16777216.0f); }
-
-// Point3D::rotate (MathBasics.cpp:41-50)
-__device__ __forceinline__ void rotate3(float cth, float phi, float& x, float& y, float& z) {
-  const float x0 = x, y0 = y, z0 = z;
-  float sphi, cphi;
-  sincosf(phi, &sphi, &cphi);
-  const float sth = sqrtf(fmaxf(0.f, 1.f - cth * cth));
-  x = cth * cphi * x0 - sphi * y0 + sth * cphi * z0;
-  y = cth * sphi * x0 + cphi * y0 + sth * sphi * z0;
-  z = -sth * x0 + cth * z0;
-}
-
-// Nucleus::SphericalHarmonics (Nucleus.cpp:669-694)
-__device__ __forceinline__ float y20(float c) { return (3.f * c * c - 1.f) * 0.31539156525252005f; }
-__device__ __forceinline__ float y40(float c) {
-  const float c2 = c * c;
-  return (35.f * c2 * c2 - 30.f * c2 + 3.f) * 0.10578554691520431f;
-}
-
-// One Woods-Saxon candidate by rejection (Nucleus.cpp:578-621): the r
-// distribution r^2/(1+exp((r-R)/a)) on [0, R+2.5), isotropic direction,
-// deformed radius R(theta) with the rotation of the nucleus applied.
-__device__ void ws_candidate(const NucSys& ns, uint2 key, unsigned long long ev, unsigned nuc,
-                             unsigned cand, unsigned attempt, float ctr, float phir, float& x,
-                             float& y, float& z) {
-  for (unsigned it = 0; it < 0xFFFFu; ++it) {
-    const uint4 r = draw(key, ev, kStrNucleon, nuc, cand, attempt, it);
-    if (ns.deformed) {
-      const float rr = ns.rmax * cbrtf(u01(r.x));
-      const float cx = 1.f - 2.f * u01(r.y);
-      const float rad1 = ns.R * (1.f + ns.beta2 * y20(cx) + ns.beta4 * y40(cx));
-      const float rw1 = 1.f / (1.f + expf(-rad1 / ns.a));
-      if (u01(r.z) * rw1 > 1.f / (1.f + expf((rr - rad1) / ns.a))) continue;
-      const float sx = sqrtf(fmaxf(0.f, 1.f - cx * cx));
-      float sp, cp;
-      sincosf(2.f * kPi * u01(r.w), &sp, &cp);
-      x = rr * sx * cp;
-      y = rr * sx * sp;
-      z = rr * cx;
-      rotate3(ctr, phir, x, y, z);
-      return;
-    } else {
-      const float rr = ns.rmax * cbrtf(u01(r.x));
-      if (u01(r.y) * ns.rwmax > 1.f / (1.f + expf((rr - ns.R) / ns.a))) continue;
-      const float cx = 1.f - 2.f * u01(r.z);
-      const float sx = sqrtf(fmaxf(0.f, 1.f - cx * cx));
-      float sp, cp;
-      sincosf(2.f * kPi * u01(r.w), &sp, &cp);
-      x = rr * sx * cp;
-      y = rr * sx * sp;
-      z = rr * cx;
-      return;
-    }
-  }
-  x = y = z = 0.f;
-}
-
-// Hulthen CDF (HulthenFunc.cpp:27-36) inverted by safeguarded Newton.
-__device__ float hulthen_inv(float u) {
-  const float al = 0.228f, be = 1.18f;
-  const float c = (al * be * (al + be)) / ((al - be) * (al - be));
-  float lo = 0.f, hi = 100.f, r = 2.f;
-  for (int it = 0; it < 60; ++it) {
-    const float e1 = expf(-r * (al + be)), e2 = expf(-2.f * al * r), e3 = expf(-2.f * be * r);
-    const float F = 2.f * c * (2.f * e1 / (al + be) - 0.5f * e2 / al - 0.5f * e3 / be + 0.5f / al + 0.5f / be -
-                               2.f / (al + be)) - u;
-    if (F > 0.f) hi = r; else lo = r;
-    const float d = expf(-al * r) - expf(-be * r);
-    const float pdf = 2.f * c * d * d;
-    float rn = (pdf > 0.f) ? r - F / pdf : 0.5f * (lo + hi);
-    if (!(rn > lo && rn < hi)) rn = 0.5f * (lo + hi);
-    if (fabsf(rn - r) < 1e-6f) { r = rn; break; }
-    r = rn;
-  }
-  return r;
-}
-
-// Marsaglia & Tsang (2000) Gamma(k, scale); k < 1 via G(k+1) U^(1/k).
-__device__ float gamma_draw(float k, float scale, uint2 key, unsigned long long ev, unsigned stream,
-                            unsigned idx) {
-  unsigned it = 0;
-  uint4 r = draw(key, ev, stream, 0, idx, 0, it++);
-  float boost = 1.f;
-  if (k < 1.f) {
-    boost = powf(u01(r.w), 1.f / k);
-    k += 1.f;
-  }
-  const float d = k - 1.f / 3.f, c = 1.f / sqrtf(9.f * d);
-  for (int guard = 0; guard < 4096; ++guard) {
-    const float u1 = u01(r.x), u2 = u01(r.y), u3 = u01(r.z);
-    const float xn = sqrtf(-2.f * logf(u1)) * cosf(2.f * kPi * u2);
-    float v = 1.f + c * xn;
-    if (v > 0.f) {
-      v = v * v * v;
-      const float x2 = xn * xn;
-      if (u3 < 1.f - 0.0331f * x2 * x2) return scale * d * v * boost;
-      if (logf(u3) < 0.5f * x2 + d * (1.f - v + logf(v))) return scale * d * v * boost;
-    }
-    r = draw(key, ev, stream, 0, idx, 0, it++);
-  }
-  return scale * d * boost;
-}
-
-template <typename T>
-__device__ __forceinline__ T wave_sum(T v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
-}
-
-// 256-thread exclusive scan through LDS (tmp has 256 ints).
-__device__ int block_excl_scan(int v, int* tmp) {
-  const int t = threadIdx.x;
-  tmp[t] = v;
-  __syncthreads();
-  for (int off = 1; off < 256; off <<= 1) {
-    const int add = (t >= off) ? tmp[t - off] : 0;
-    __syncthreads();
-    tmp[t] += add;
-    __syncthreads();
-  }
-  const int incl = tmp[t];
-  __syncthreads();
-  return incl - v;
-}
 
 // ------------------------------------------------------------- k_event
 __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
@@ -593,208 +406,6 @@ __global__ __launch_bounds__(256) void k_smear(DevCfg cfg, EvBufs buf, SmearSpec
   }
 }
 
-// ----------------------------------------------------------- k_moments
-// One workgroup per event.  mode 0: full eps_n record; mode 1: only the
-// grid sum (dndy).
-__device__ __forceinline__ double block_sum_d(double v, double* red) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  __syncthreads();
-  if (lane == 0) red[wv] = v;
-  __syncthreads();
-  return red[0] + red[1] + red[2] + red[3];
-}
-
-constexpr int kMaxOrd = 9;
-
-__global__ __launch_bounds__(256) void k_moments(DevCfg cfg, EvBufs buf, const float* grid, double* evout,
-                                                 int mode) {
-  extern __shared__ unsigned mask[];  // nx * nwy words
-  __shared__ int rect[2 * kAMax][4];
-  __shared__ double red[4];
-  __shared__ int ired[4][4];
-
-  const int e = blockIdx.x, t = threadIdx.x;
-  const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
-  const size_t N = (size_t)nx * ny;
-  const float* g = grid + (size_t)e * N;
-  double* out = evout + (size_t)e * SMC_EV_STRIDE;
-  const int* cnt = buf.counts + (size_t)e * 8;
-  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
-  const double ff = cfg.final_factor;
-
-  // pass 1: centre of mass (MakeDensity.cpp:2273-2282)
-  double sx = 0.0, sy = 0.0, st = 0.0;
-  for (size_t c = t; c < N; c += 256) {
-    const double d = (double)g[c] * ff;
-    if (d == 0.0) continue;
-    const int ix = (int)(c / ny), iy = (int)(c % ny);
-    sx += (cfg.xmin + ix * cfg.dx) * d;
-    sy += (cfg.ymin + iy * cfg.dy) * d;
-    st += d;
-  }
-  const double total = block_sum_d(st, red);
-  const double xsum = block_sum_d(sx, red);
-  const double ysum = block_sum_d(sy, red);
-  if (mode == 1) {
-    if (t == 0) out[SMC_EV_TOTAL] = total;
-    return;
-  }
-  const double xc = xsum / total, yc = ysum / total;
-
-  // hot-spot mask (MCnucl.cpp:1303-1323, MakeDensity.cpp:2341-2359)
-  const int np = npa + npb;
-  const int hbq = cfg.hb_h >> 1;  // box half-width in quanta
-  // the CollisionPair boxes are zero boxes at the origin (CollisionPair.h:12,
-  // Box2D.cpp:14-22), so with Ncoll > 0 the region always contains (0,0)
-  int rxl = (ncoll > 0) ? 0 : 0x7fffffff, rxr = (ncoll > 0) ? 0 : (int)0x80000000;
-  int ryl = rxl, ryr = rxr;
-  for (int i = t; i < 2 * kAMax; i += 256) {
-    const int nu = i / kAMax, j = i % kAMax;
-    if (j >= cfg.nuc[nu].A) continue;
-    const int wr = buf.wound[(size_t)e * 2 * kAMax + i];
-    if (wr < 0) continue;
-    const int x = buf.pos[((size_t)e * 2 * kAMax + i) * 2], y = buf.pos[((size_t)e * 2 * kAMax + i) * 2 + 1];
-    rxl = min(rxl, x - hbq);
-    rxr = max(rxr, x + hbq);
-    ryl = min(ryl, y - hbq);
-    ryr = max(ryr, y + hbq);
-  }
-  {
-    int v0 = rxl, v1 = ryl, v2 = -rxr, v3 = -ryr;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      v0 = min(v0, __shfl_xor(v0, off, 64));
-      v1 = min(v1, __shfl_xor(v1, off, 64));
-      v2 = min(v2, __shfl_xor(v2, off, 64));
-      v3 = min(v3, __shfl_xor(v3, off, 64));
-    }
-    if ((t & 63) == 0) { ired[t >> 6][0] = v0; ired[t >> 6][1] = v1; ired[t >> 6][2] = v2; ired[t >> 6][3] = v3; }
-    __syncthreads();
-    rxl = min(min(ired[0][0], ired[1][0]), min(ired[2][0], ired[3][0]));
-    ryl = min(min(ired[0][1], ired[1][1]), min(ired[2][1], ired[3][1]));
-    rxr = -min(min(ired[0][2], ired[1][2]), min(ired[2][2], ired[3][2]));
-    ryr = -min(min(ired[0][3], ired[1][3]), min(ired[2][3], ired[3][3]));
-  }
-  const int offx = step_eval(cfg.tab[kOffX], rxl), offy = step_eval(cfg.tab[kOffY], ryl);
-  // participant rectangles in grid coordinates
-  for (int i = t; i < 2 * kAMax; i += 256) {
-    const int nu = i / kAMax, j = i % kAMax;
-    int r0 = 0, r1 = 0, r2 = 0, r3 = 0, valid = 0;
-    if (j < cfg.nuc[nu].A && buf.wound[(size_t)e * 2 * kAMax + i] >= 0) {
-      const int x = buf.pos[((size_t)e * 2 * kAMax + i) * 2], y = buf.pos[((size_t)e * 2 * kAMax + i) * 2 + 1];
-      r0 = step_eval(cfg.tab[kMaskX], x - hbq - rxl) + offx;
-      r1 = step_eval(cfg.tab[kMaskX], x + hbq - rxl) + offx;
-      r2 = step_eval(cfg.tab[kMaskY], y - hbq - ryl) + offy;
-      r3 = step_eval(cfg.tab[kMaskX], y + hbq - ryl) + offy;  // divides by dx (MakeDensity.cpp:2357)
-      valid = 1;
-    }
-    rect[i][0] = valid ? max(r0, 0) : 0;
-    rect[i][1] = valid ? min(r1, nx) : 0;
-    rect[i][2] = valid ? max(r2, 0) : 0;
-    rect[i][3] = valid ? min(r3, ny) : 0;
-  }
-  __syncthreads();
-  // rows owned by threads, no atomics
-  for (int ix = t; ix < nx; ix += 256) {
-    unsigned* row = mask + (size_t)ix * nwy;
-    for (int w = 0; w < nwy; ++w) row[w] = 0u;
-    for (int i = 0; i < 2 * kAMax; ++i) {
-      const int x0 = rect[i][0], x1 = rect[i][1];
-      if (ix < x0 || ix >= x1) continue;
-      const int y0 = rect[i][2], y1 = rect[i][3];
-      for (int y = y0; y < y1;) {
-        const int w = y >> 5, b0 = y & 31;
-        const int b1 = min(32, b0 + (y1 - y));
-        const unsigned bits = (b1 - b0 == 32) ? 0xffffffffu : (((1u << (b1 - b0)) - 1u) << b0);
-        row[w] |= bits;
-        y += b1 - b0;
-      }
-    }
-  }
-  __syncthreads();
-
-  // pass 2: <r^n> over the full grid and masked moments
-  const int from = cfg.from_order, to = cfg.to_order;
-  double rn[10];
-  double mre[kMaxOrd], mim[kMaxOrd], mno[kMaxOrd], pre[kMaxOrd], pim[kMaxOrd], pno[kMaxOrd];
-#pragma unroll
-  for (int n = 0; n < 10; ++n) rn[n] = 0.0;
-#pragma unroll
-  for (int o = 0; o < kMaxOrd; ++o) mre[o] = mim[o] = mno[o] = pre[o] = pim[o] = pno[o] = 0.0;
-  for (size_t c = t; c < N; c += 256) {
-    const double d = (double)g[c] * ff;
-    if (d == 0.0) continue;
-    const int ix = (int)(c / ny), iy = (int)(c % ny);
-    const double x = cfg.xmin + ix * cfg.dx - xc;
-    const double y = cfg.ymin + iy * cfg.dy - yc;
-    const double r2 = x * x + y * y;
-    const double r = sqrt(r2);
-    double pw[10];
-    pw[0] = 1.0;
-#pragma unroll
-    for (int n = 1; n < 10; ++n) pw[n] = pw[n - 1] * r;
-#pragma unroll
-    for (int n = 0; n < 10; ++n) rn[n] += pw[n] * d;
-    if (!((mask[(size_t)ix * nwy + (iy >> 5)] >> (iy & 31)) & 1u)) continue;
-    if (r == 0.0) continue;  // every r^2 / r^n weighted term vanishes at r = 0
-    double zr = x, zi = y;  // z^n = r^n e^{i n theta}
-#pragma unroll
-    for (int n = 1; n <= kMaxOrd; ++n) {
-      if (n >= from && n <= to) {
-        const int o = n - 1;
-        const double w2 = r2 / pw[n];  // r^2 / r^n
-        mre[o] += zr * w2 * d;
-        mim[o] += zi * w2 * d;
-        mno[o] += r2 * d;
-        if (n == 1) {
-          pre[o] += r2 * x * d;
-          pim[o] += r2 * y * d;
-          pno[o] += r2 * r * d;
-        } else {
-          pre[o] += zr * d;
-          pim[o] += zi * d;
-          pno[o] += pw[n] * d;
-        }
-      }
-      const double nr = zr * x - zi * y, ni = zr * y + zi * x;
-      zr = nr;
-      zi = ni;
-    }
-  }
-  double rnt[10];
-#pragma unroll
-  for (int n = 0; n < 10; ++n) rnt[n] = block_sum_d(rn[n], red);
-  if (t == 0) {
-    for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = rnt[n] / total;
-    out[SMC_EV_XC] = xc;
-    out[SMC_EV_YC] = yc;
-    out[SMC_EV_TOTAL] = total;
-    out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
-    out[SMC_EV_NPART] = np;
-    out[SMC_EV_NCOLL] = ncoll;
-    out[SMC_EV_B] = buf.bval[e];
-    out[SMC_EV_STATUS] = cnt[3];
-  }
-  const double eps = 1e-15;
-#pragma unroll
-  for (int o = 0; o < kMaxOrd; ++o) {
-    const int n = o + 1;
-    const double a = block_sum_d(mre[o], red), b = block_sum_d(mim[o], red), c = block_sum_d(mno[o], red);
-    const double p = block_sum_d(pre[o], red), q = block_sum_d(pim[o], red), s = block_sum_d(pno[o], red);
-    if (t == 0) {
-      const bool on = n >= from && n <= to;
-      out[SMC_EV_EPS_RE + n] = on ? -a / (c + eps) : 0.0;
-      out[SMC_EV_EPS_IM + n] = on ? -b / (c + eps) : 0.0;
-      out[SMC_EV_EPSP_RE + n] = on ? -p / (s + eps) : 0.0;
-      out[SMC_EV_EPSP_IM + n] = on ? -q / (s + eps) : 0.0;
-    }
-  }
-  if (t == 0) {
-    out[SMC_EV_EPS_RE] = out[SMC_EV_EPS_IM] = out[SMC_EV_EPSP_RE] = out[SMC_EV_EPSP_IM] = 0.0;
-  }
-}
-
 // ------------------------------------------------------------ launchers
 static hipStream_t S(void* s) { return (hipStream_t)s; }
 
@@ -811,14 +422,6 @@ int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
   if (nev <= 0) return 0;
   const int ntx = (cfg.nx + kTX - 1) / kTX, nty = (cfg.ny + kTY - 1) / kTY;
   hipLaunchKernelGGL(k_smear, dim3(ntx * nty, nev), dim3(256), 0, S(stream), cfg, b, s, out0, out1);
-  return (int)hipGetLastError();
-}
-
-int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* ev_out,
-                   int mode, void* stream) {
-  if (nev <= 0) return 0;
-  const size_t shm = (size_t)cfg.nx * ((cfg.ny + 31) / 32) * sizeof(unsigned);
-  hipLaunchKernelGGL(k_moments, dim3(nev), dim3(256), shm, S(stream), cfg, b, grid, ev_out, mode);
   return (int)hipGetLastError();
 }
 

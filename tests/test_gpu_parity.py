@@ -5,7 +5,9 @@ Bars (written here, checked below):
   Npart, Ncoll, participant sets, wound order, pair lists: bit-exact
   T_A, T_B, entropy grid: relative L-inf (max|d| / max|ref|) <= 1e-5
   eps_n (r^2 and r^n weights, n = 2..5): absolute <= 1e-6
-  <r^n>, dS/dy: relative <= 1e-6;  centre of mass: absolute <= 1e-6 fm
+  <r^n>: relative <= 1e-5 (grid-level; r^9 weights the footprint tails);
+  dS/dy, grid sum: relative <= 1e-6 (exact-footprint paths), <= 1e-5 (MFMA
+  path: square window); centre of mass: absolute <= 1e-6 fm
 """
 import numpy as np
 import pytest
@@ -41,20 +43,33 @@ def _check_collision(ctx, ev, o):
     assert (o["ib"] == col["order_b"]).all()
 
 
+def _sum_tol(p):
+    # the MFMA path keeps the square window's corners (each below e^-12.5 of
+    # its source's peak), which moves grid sums by ~1e-6 relative
+    return 1e-5 if p.smear_path == 1 else 1e-6
+
+
 def _check_event_record(rec, o, p):
     m = o["mom"]
     for n in range(p.ecc_from_order, p.ecc_to_order + 1):
         for key, off in (("mom_re", 0), ("mom_im", 10), ("momp_re", 20), ("momp_im", 30)):
             assert abs(rec[off + n] - m[key][n]) <= EPS_TOL, (key, n, rec[off + n], m[key][n])
     for n in range(10):
-        assert rec[40 + n] == pytest.approx(m["rn"][n], rel=1e-6)
+        assert rec[40 + n] == pytest.approx(m["rn"][n], rel=1e-5)
     assert abs(rec[50] - m["xc"]) <= 1e-6 and abs(rec[51] - m["yc"]) <= 1e-6
-    assert rec[52] == pytest.approx(m["total"], rel=1e-6)
+    assert rec[52] == pytest.approx(m["total"], rel=_sum_tol(p))
 
 
+# smear_path 0: fused kernel, VALU, exact reference footprint
+#            1: fused kernel, MFMA Gx.diag(w).Gy^T (square window)
+#            2: separate smear + moment kernels, VALU, exact footprint
+PATHS = [0, 1, 2]
+
+
+@pytest.mark.parametrize("path", PATHS)
 @pytest.mark.parametrize("system", ["AuAu", "PbPb"])
-def test_fed_positions_full_chain(system):
-    p = H.glauber_params(system, max_events=12)
+def test_fed_positions_full_chain(system, path):
+    p = H.glauber_params(system, max_events=12, smear_path=path)
     bvals = np.array([0.0, 1.0, 2.5, 4.0, 5.5, 7.0, 8.5, 10.0, 11.5, 12.5, 13.5, 14.5])
     xa, xb, bs = H.cpu_configs(p, len(bvals), seed=1234 + p.Aproj, bvals=bvals)
     with _ctx(p) as ctx:
@@ -72,15 +87,16 @@ def test_fed_positions_full_chain(system):
             if nc[e] == 0:
                 continue
             _check_collision(ctx, e, o)
-            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL
+            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL, H.rel_linf(s[e], o["s"])
             assert H.rel_linf(ta[e], o["TA"]) <= GRID_TOL
             assert H.rel_linf(tb[e], o["TB"]) <= GRID_TOL
-            assert dndy[e] == pytest.approx(o["dndy"], rel=1e-6)
+            assert dndy[e] == pytest.approx(o["dndy"], rel=_sum_tol(p))
             _check_event_record(rec[e], o, p)
 
 
-def test_sampled_events_match_oracle():
-    p = H.glauber_params("PbPb", max_events=48)
+@pytest.mark.parametrize("path", PATHS)
+def test_sampled_events_match_oracle(path):
+    p = H.glauber_params("PbPb", max_events=48, smear_path=path)
     with _ctx(p) as ctx:
         ev = ctx.run(seed=2026, event0=0, nev=48)
         assert (ev[:, 57] == 0).all()
@@ -98,8 +114,9 @@ def test_sampled_events_match_oracle():
             assert H.rel_linf(s[e], o["s"]) <= GRID_TOL
 
 
-def test_counter_based_events_are_batch_independent():
-    p = H.glauber_params("PbPb", max_events=32)
+@pytest.mark.parametrize("path", PATHS)
+def test_counter_based_events_are_batch_independent(path):
+    p = H.glauber_params("PbPb", max_events=32, smear_path=path)
     with _ctx(p) as ctx:
         a = ctx.run(seed=7, event0=100, nev=32)
         b = ctx.run(seed=7, event0=110, nev=8)
