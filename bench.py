@@ -28,14 +28,14 @@ FP32_PEAK = 157.3e12
 HBM_PEAK = 8.0e12
 
 
-def bench_params(nev):
+def bench_params(nev, smear_path=0):
     from supermc_amd import default_params
     return default_params(
         which_mc_model=5, sub_model=1, shape_of_nucleons=4, gauss_nucl_width=0.5, shape_of_entropy=2,
         collision_criterion=1, Aproj=208, Atarg=208, proj_ws_R=6.62, proj_ws_a=0.546, targ_ws_R=6.62,
         targ_ws_a=0.546, nucleon_min_dist=0.4, sigma_nn_mb=64.0, alpha=0.14, cc_fluctuation_model=6,
         cc_fluctuation_Gamma_theta=1.25, bmin=0.0, bmax=20.0, Npmin=2, Npmax=500, maxx=13.0, maxy=13.0, dx=0.1,
-        dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev, max_pairs=4096, smear_path=1)
+        dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev, max_pairs=4096, smear_path=smear_path)
 
 
 def _cpu_worker(args):
@@ -67,6 +67,25 @@ def cpu_baseline(seconds):
                        f"(sample -> collide -> T_A/T_B -> density -> eps_n, no file output), {wall:.1f} s wall")
 
 
+def eps_error_vs_fp64(ctx, ev, n):
+    """Largest |eps_n - eps_n(FP64 oracle)| (n = 2..5, r^2 and r^n weights)
+    over the first n events of the last batch, the oracle fed the same
+    nucleons and Gamma factors."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import parity_helpers as H
+    worst = {"eps_r2": 0.0, "eps_rn": 0.0}
+    for e in range(min(n, ctx.nev)):
+        o = H.oracle_event(ctx, e)
+        m = o["mom"]
+        for k in range(2, 6):
+            worst["eps_r2"] = max(worst["eps_r2"], abs(ev[e, k] - m["mom_re"][k]), abs(ev[e, 10 + k] - m["mom_im"][k]))
+            worst["eps_rn"] = max(worst["eps_rn"], abs(ev[e, 20 + k] - m["momp_re"][k]),
+                                  abs(ev[e, 30 + k] - m["momp_im"][k]))
+    worst["events"] = min(n, ctx.nev)
+    return worst
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +95,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--smear-path", type=int, default=0,
+                    help="0 exact footprint (VALU strips), 1 MFMA Gx.diag(w).Gy^T, 2 exact (VALU tiles)")
+    ap.add_argument("--parity-events", type=int, default=12,
+                    help="events of the last batch re-evaluated by the FP64 oracle for the eps_n error")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -85,7 +108,7 @@ def main():
     from supermc_amd import Context, EV_NPART, EV_NCOLL, EV_STATUS, load
     load()  # fails loudly when libsmc.so is missing
     nev = args.events
-    ctx = Context(bench_params(nev))
+    ctx = Context(bench_params(nev, args.smear_path))
     print(f"[bench] Pb+Pb 261^2, {nev} events/step, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
           flush=True)
     for w in range(args.warmup):
@@ -106,12 +129,12 @@ def main():
     sum_k = float((ev[:, EV_NPART] + ev[:, EV_NCOLL]).sum())
     N = ctx.nx * ctx.ny
     kernels = {}
-    for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments"), (4, "k_grid")):
+    for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments")):
         ms, n = ctx.timing(kid)
         if n:
             kernels[name] = dict(avg_ms=ms / n, launches=n)
     dom = max(kernels, key=lambda k: kernels[k]["avg_ms"])
-    if dom in ("k_smear", "k_grid"):
+    if dom == "k_smear":
         # dense-equivalent contraction T = Gx diag(w) Gy^T: 2*N*K flops/event
         work = 2.0 * N * sum_k
         achieved = work / (kernels[dom]["avg_ms"] * 1e-3)
@@ -130,6 +153,8 @@ def main():
         except Exception:
             traffic = None
     roof["traffic"] = traffic
+
+    eps_err = eps_error_vs_fp64(ctx, ev, args.parity_events) if args.parity_events > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -151,6 +176,8 @@ def main():
         "config": {"workload": "batched MC-Glauber Pb+Pb min-bias, 261x261 grid, Gamma fluct k=0.8",
                    "events_per_step": nev, "grid": [ctx.nx, ctx.ny], "sigma_nn_mb": 64.0, "moments_dtype": "f64",
                    "mean_sources_per_event": sum_k / nev, "failed_events": bad,
+                   "smear_path": ["exact VALU strips", "MFMA Gx.diag(w).Gy^T", "exact VALU tiles"][args.smear_path],
+                   "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()}},
         "roofline": roof,
         "cpu_baseline": cpu,

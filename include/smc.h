@@ -83,8 +83,10 @@ typedef struct smc_params {
   int max_events;              /* events per batch held on the device */
   int max_pairs;               /* per-event collision-pair capacity (<=0: Aproj*Atarg) */
   int max_attempts;            /* per-event acceptance attempts */
-  int smear_path;              /* 0 fused VALU (exact footprint), 1 fused MFMA fp32 (square
-                                  window, see SMC_SMEAR_*), 2 unfused VALU tiles */
+  int smear_path;              /* 0 exact reference footprint, VALU strips  [default]
+                                  1 MFMA fp32 strips: separable Gx.diag(w).Gy^T over the
+                                    square window (no r <= 5w corner cut)
+                                  2 exact reference footprint, VALU tiles */
 } smc_params;
 
 /* Derived quantities (for the facade and tests) */
@@ -192,7 +194,6 @@ smc_status smc_kln_make_table(smc_ctx* ctx, double* table);
 #define SMC_K_SMEAR 1
 #define SMC_K_MOMENTS 2
 #define SMC_K_KLN 3
-#define SMC_K_GRID 4     /* fused smearing + moments */
 #define SMC_K_COUNT 8
 smc_status smc_set_timing(smc_ctx* ctx, int on);
 smc_status smc_get_timing(smc_ctx* ctx, int kernel, double* total_ms, int* launches);

@@ -119,6 +119,7 @@ struct smc_ctx {
   float* grid = nullptr;
   float* ta = nullptr;
   float* tb = nullptr;
+  float* tmp = nullptr;
   double* evout = nullptr;
   double* kln_table = nullptr;
   // timing
@@ -354,6 +355,7 @@ void alloc_buffers(smc_ctx* c) {
   b.raw = dalloc<int>(E * 2 * kAMax * 2);
   b.src_cap = 2 * kAMax + (int)MP;
   b.src = dalloc<int4>(E * (size_t)b.src_cap);
+  b.swin = dalloc<int4>(E * (size_t)b.src_cap);
   c->evout = dalloc<double>(E * SMC_EV_STRIDE);
   HIPCHK(hipMemset(b.counts, 0, E * 8 * sizeof(int)));
 }
@@ -402,41 +404,30 @@ SmearSpec thickness_spec(const smc_ctx* c) {
   return s;
 }
 
-// The fused grid kernel holds a grid of up to 272 x 272 cells in registers
-// and covers the single-accumulator densities (MC-Glauber).
-bool fused_ok(const smc_ctx* c) {
-  return c->p.smear_path != 2 && c->p.which_mc_model == 5 && c->d.nx <= 272 && c->d.ny <= 272;
-}
-
-// flags: 2 = eps_n record, 4 = grid sum only; the grid is always written.
-void run_grid(smc_ctx* c, int flags) {
-  float* s = grid_buffer(c, &c->grid);
-  TimerScope ts(c, SMC_K_GRID);
-  check_launch(launch_grid(c->cfg, c->buf, density_spec(c), c->nev, s, c->evout, flags, c->p.smear_path == 1,
-                           c->stream),
-               "k_grid");
+// smear_path: 0 exact footprint, VALU strips (tiles for grids wider than
+// 320); 1 MFMA strips Gx.diag(w).Gy^T (square window, no r <= 5w corner cut;
+// grids up to 272 wide, not for sqrt(T_A T_B)); 2 exact footprint, VALU tiles.
+void run_smear(smc_ctx* c, const SmearSpec& spec, float* out0, float* out1) {
+  TimerScope ts(c, SMC_K_SMEAR);
+  const int sp = c->p.smear_path;
+  if (sp == 1 && c->d.ny <= 272 && spec.combine != 1)
+    check_launch(launch_strip(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_strip");
+  else if (sp != 2 && c->d.ny <= 320)
+    check_launch(launch_strip_exact(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_strip_exact");
+  else
+    check_launch(launch_smear(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_smear");
   ts.done();
 }
 
 void run_density(smc_ctx* c) {
   if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
   if (c->p.which_mc_model == 1) throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN density not available yet");
-  if (fused_ok(c)) {
-    run_grid(c, 4);
-  } else {
-    float* s = grid_buffer(c, &c->grid);
-    TimerScope ts(c, SMC_K_SMEAR);
-    check_launch(launch_smear(c->cfg, c->buf, density_spec(c), c->nev, s, nullptr, c->stream), "k_smear");
-    ts.done();
-  }
+  const SmearSpec spec = density_spec(c);
+  run_smear(c, spec, grid_buffer(c, &c->grid), spec.combine ? grid_buffer(c, &c->tmp) : nullptr);
   c->have_density = 1;
 }
 
 void run_moments(smc_ctx* c, int mode) {
-  if (mode == 0 && fused_ok(c)) {
-    run_grid(c, 2);
-    return;
-  }
   if (!c->have_density) run_density(c);
   TimerScope ts(c, SMC_K_MOMENTS);
   DevCfg g = c->cfg;
@@ -505,7 +496,7 @@ smc_status smc_params_default(smc_params* p) {
   p->max_events = 1024;
   p->max_pairs = 0;
   p->max_attempts = 10000;
-  p->smear_path = 1;
+  p->smear_path = 0;
   return SMC_OK;
 }
 
@@ -538,7 +529,7 @@ void smc_destroy(smc_ctx* c) {
   if (!c) return;
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
-                  c->buf.bval, c->buf.raw, c->buf.src, c->dtab, c->grid, c->ta, c->tb, c->evout,
+                  c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
                   c->kln_table};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -676,9 +667,7 @@ smc_status smc_thickness(smc_ctx* c, float* TA, float* TB) {
     if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events");
     float* a = grid_buffer(c, &c->ta);
     float* b = grid_buffer(c, &c->tb);
-    TimerScope ts(c, SMC_K_SMEAR);
-    check_launch(launch_smear(c->cfg, c->buf, thickness_spec(c), c->nev, a, b, c->stream), "k_smear");
-    ts.done();
+    run_smear(c, thickness_spec(c), a, b);
     c->have_thickness = 1;
     const size_t n = (size_t)c->nev * c->d.nx * c->d.ny;
     if (TA) HIPCHK(hipMemcpyAsync(TA, a, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
@@ -696,7 +685,7 @@ smc_status smc_density(smc_ctx* c, float* s, double* dndy) {
     if (s) HIPCHK(hipMemcpyAsync(s, c->grid, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     int st = SMC_OK;
     if (dndy) {
-      if (!fused_ok(c)) run_moments(c, 1);  // the fused kernel already summed the grid
+      run_moments(c, 1);
       std::vector<double> ev((size_t)c->nev * SMC_EV_STRIDE);
       HIPCHK(hipMemcpyAsync(ev.data(), c->evout, ev.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       sync(c);

@@ -192,6 +192,23 @@ __device__ inline int block_excl_scan(int v, int* tmp) {
   return incl - v;
 }
 
+// Exact footprints of a source at (ph, qh) half quanta: the participant box
+// (MCnucl.cpp:831-844) and the +-5w window (MCnucl.cpp:450-457, 729-736),
+// each packed as lo | hi << 16 per axis.
+__device__ __forceinline__ int4 src_windows(const DevCfg& cfg, int ph, int qh, bool box) {
+  int4 w;
+  if (box) {
+    w.x = step_eval(cfg.tab[kBoxLoX], ph) | (step_eval(cfg.tab[kBoxHiX], ph) << 16);
+    w.y = step_eval(cfg.tab[kBoxLoY], qh) | (step_eval(cfg.tab[kBoxHiY], qh) << 16);
+  } else {
+    w.x = 0;
+    w.y = 0;
+  }
+  w.z = step_eval(cfg.tab[kWinLoX], ph) | (step_eval(cfg.tab[kWinHiX], ph) << 16);
+  w.w = step_eval(cfg.tab[kWinLoY], qh) | (step_eval(cfg.tab[kWinHiY], qh) << 16);
+  return w;
+}
+
 // Sum of a double over a 256-thread block (red: 4 doubles of LDS).
 __device__ __forceinline__ double block_sum_d(double v, double* red) {
   v = wave_sum(v);

@@ -84,6 +84,7 @@ struct EvBufs {
   double* bval;    // [ev]
   int* raw;        // [ev][2][kAMax][2] fed positions (quanta), input order
   int4* src;       // [ev][src_cap] {x_h, y_h, fluct bits, kind}
+  int4* swin;      // [ev][src_cap] footprints {box x, box y, 5w x, 5w y}, lo | hi << 16
   int src_cap;
 };
 
@@ -113,8 +114,10 @@ int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  float* out0, float* out1, void* stream);
 int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev,
                    double* ev_out, int mode, void* stream);
-int launch_grid(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
-                double* ev_out, int flags, int mfma, void* stream);
+int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
+                 void* stream);
+int launch_strip_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
+                       float* out1, void* stream);
 int launch_kln_points(const DevCfg& cfg, int n, const double* ta, const double* tb,
                       double* out, const float* nodes, int npt, int nkt, int nphi,
                       double ecm, double lambda, double ptmin, double ptmax, void* stream);

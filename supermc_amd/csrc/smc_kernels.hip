@@ -263,9 +263,16 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
   const int nsrc_part = npa + npb;
   int4* src = buf.src + (size_t)e * buf.src_cap;
   const bool src_ok = nsrc_part + ncoll <= buf.src_cap;
+  int4* swin = buf.swin + (size_t)e * buf.src_cap;
   if (src_ok) {
-    if (woundA >= 0) src[woundA] = make_int4(2 * sx[0][t], 2 * sy[0][t], __float_as_int(fA), kSrcPartA);
-    if (woundB >= 0) src[npa + woundB] = make_int4(2 * sx[1][t], 2 * sy[1][t], __float_as_int(fB), kSrcPartB);
+    if (woundA >= 0) {
+      src[woundA] = make_int4(2 * sx[0][t], 2 * sy[0][t], __float_as_int(fA), kSrcPartA);
+      swin[woundA] = src_windows(cfg, 2 * sx[0][t], 2 * sy[0][t], true);
+    }
+    if (woundB >= 0) {
+      src[npa + woundB] = make_int4(2 * sx[1][t], 2 * sy[1][t], __float_as_int(fB), kSrcPartB);
+      swin[npa + woundB] = src_windows(cfg, 2 * sx[1][t], 2 * sy[1][t], true);
+    }
   }
   if (ncoll > cfg.max_pairs || !src_ok) status |= kStPairOverflow;
   if (t < A && rowcnt[t] > 0) {
@@ -283,8 +290,10 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
           if (cfg.fluct) fp = gamma_draw(cfg.gk_bin, cfg.gs_bin, key, gev, kStrFluctPair, (unsigned)k);
           buf.pairs[pbase + k] = (unsigned)t | ((unsigned)j << 16);
           buf.pfluct[pbase + k] = fp;
-          if (src_ok)
+          if (src_ok) {
             src[nsrc_part + k] = make_int4(xa + sx[1][j], ya + sy[1][j], __float_as_int(fp), kSrcPair);
+            swin[nsrc_part + k] = src_windows(cfg, xa + sx[1][j], ya + sy[1][j], false);
+          }
         }
         ++k;
       }
