@@ -142,6 +142,12 @@ smc_status smc_get_derived(const smc_ctx* ctx, smc_derived* out);
  * `seed`, independent of batch boundaries. */
 smc_status smc_sample(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev);
 
+/* MCnucl::generateNuclei(b) + getBinaryCollision() (MCnucl.cpp:208-308)
+ * for nev events at the given impact parameters [fm], one attempt each (the
+ * caller applies the acceptance of MakeDensity.cpp:2147-2162).  b is rounded
+ * to twice the position quantum; smc_moments reports the value used. */
+smc_status smc_sample_b(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, const double* b);
+
 /* Fed positions (oracle-parity path): nev events of Aproj / Atarg nucleons,
  * xy interleaved in fm, in any order; b per event is recorded only.
  * Runs the collision step on them.  fluct_seed drives the Gamma factors. */

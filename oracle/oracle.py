@@ -254,6 +254,17 @@ def six_point(*a):
     return lib().orc_six_point(*[float(v) for v in a])
 
 
+def density_kln_table(g, ta, tb, table, dT):
+    """MCnucl.cpp:654-687 over the grid g; returns (s, dndy), dndy < 0 when
+    a cell leaves the table."""
+    ta = np.ascontiguousarray(ta, np.float64)
+    tb = np.ascontiguousarray(tb, np.float64)
+    table = np.ascontiguousarray(table, np.float64)
+    s = np.zeros_like(ta)
+    dndy = lib().orc_density_kln_table(C.byref(g), _p(ta), _p(tb), _p(table), table.shape[0], dT, _p(s))
+    return s, dndy
+
+
 class Rng:
     def __init__(self, seed):
         self.r = OrcRng()

@@ -80,6 +80,7 @@ SIGNATURES = {
     "smc_last_error": (C.c_char_p, [C.c_void_p]),
     "smc_get_derived": (C.c_int, [C.c_void_p, _P(SmcDerived)]),
     "smc_sample": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int]),
+    "smc_sample_b": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
     "smc_set_positions": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, C.c_uint64]),
     "smc_collide": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "smc_get_nucleons": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _fp, _dp, _ip, _fp]),
@@ -185,6 +186,11 @@ class Context:
     def sample(self, seed, event0, nev):
         self._check(self.lib.smc_sample(self.h, seed, event0, nev))
         self.nev = nev
+
+    def sample_b(self, seed, event0, b):
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        self._check(self.lib.smc_sample_b(self.h, seed, event0, len(b), _ptr(b, C.c_double)))
+        self.nev = len(b)
 
     def set_positions(self, xy_proj, xy_targ, b=None, fluct_seed=0):
         xy_proj = np.ascontiguousarray(xy_proj, dtype=np.float64)

@@ -122,6 +122,13 @@ struct smc_ctx {
   float* tmp = nullptr;
   double* evout = nullptr;
   double* kln_table = nullptr;
+  // MC-KLN quadrature nodes (FP32 for the per-cell kernel, FP64 for points)
+  void* kln_ptf = nullptr;
+  void* kln_nodes_f = nullptr;
+  void* kln_ptd = nullptr;
+  void* kln_nodes_d = nullptr;
+  int kln_npt = 0, kln_nn = 0;
+  KlnConst kln_kc{};
   // timing
   int timing = 0;
   struct Pending { int k; hipEvent_t a, b; };
@@ -341,6 +348,96 @@ void build_tables(smc_ctx* c) {
   }
 }
 
+// Gauss-Legendre nodes and weights on [0, 1] (Newton iteration on P_n).
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w) {
+  x.assign(n, 0.0);
+  w.assign(n, 0.0);
+  for (int i = 0; i < (n + 1) / 2; ++i) {
+    double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), pp = 1.0;
+    for (int it = 0; it < 100; ++it) {
+      double p1 = 1.0, p2 = 0.0;
+      for (int j = 1; j <= n; ++j) {
+        const double p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * j - 1.0) * z * p2 - (j - 1.0) * p3) / j;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      const double z1 = z;
+      z = z1 - p1 / pp;
+      if (std::fabs(z - z1) < 1e-16) break;
+    }
+    x[i] = 0.5 * (1.0 - z);
+    x[n - 1 - i] = 0.5 * (1.0 + z);
+    w[i] = w[n - 1 - i] = 1.0 / ((1.0 - z * z) * pp * pp);
+  }
+}
+
+// Precomputes the cell-independent parts of KLNModel::func (KLNModel.cpp:
+// 219-277) on the (pT, kT, phi) product rule over the unit cube, with the
+// normalisation of getdNdy (KLNModel.cpp:97-110, g2hfac = 2 x 1.0).
+void build_kln(smc_ctx* c) {
+  const smc_params& p = c->p;
+  const int npt = std::max(1, p.kln_n_pt), nkt = std::max(1, p.kln_n_kt), nphi = std::max(1, p.kln_n_phi);
+  const double ecm = p.ecm, lam = p.lambda, ptmin = p.kln_pt_min, ptmax = p.kln_pt_max;
+  const double CF = (3.0 * 3.0 - 1.0) / (2 * 3.0), hbarc = 0.197327053;
+  const double Norm = 2. / CF / (hbarc * hbarc), g2hfac = 2.0 * 1.0;
+  const double fac = CF * 2. / (3. * M_PI * M_PI);
+  const double lq2 = 0.2 * 0.2, amax = 0.5, beta0 = (33.0 - 2.0 * 3.0) / (12 * M_PI);
+  auto alpha = [&](double q2) { return (q2 <= lq2) ? amax : std::min(amax, 1.0 / (beta0 * std::log(q2 / lq2))); };
+  std::vector<double> xp, wp, xk, wk, xf, wf;
+  gauss_legendre01(npt, xp, wp);
+  gauss_legendre01(nkt, xk, wk);
+  gauss_legendre01(nphi, xf, wf);
+  // phi and 2 pi - phi give the same k1^2, k2^2: merge the symmetric pairs
+  std::vector<double> uf, ww;
+  for (int j = 0; j < nphi / 2; ++j) { uf.push_back(xf[j]); ww.push_back(2.0 * wf[j]); }
+  if (nphi % 2) { uf.push_back(xf[nphi / 2]); ww.push_back(wf[nphi / 2]); }
+  int nn = nkt * (int)uf.size();
+  const int nn_pad = (nn + 1) & ~1;  // the FP32 kernel walks nodes in pairs
+  std::vector<double> ptd(2 * npt), nd(4 * (size_t)npt * nn_pad, 0.0);
+  std::vector<float> ptf(2 * npt), nf(4 * (size_t)npt * nn_pad, 0.f);
+  for (int a = 0; a < npt; ++a) {
+    const double pt = ptmin + xp[a] * (ptmax - ptmin);
+    const double x = pt / ecm;
+    double P = 0.0, g = 0.0;
+    if (x <= 1.0) {
+      g = 2. / 1.53 * std::pow(0.01 / x, lam);
+      P = wp[a] * fac * fac * std::pow(1.0 - x, 8.) * 2.0 * M_PI * (ptmax - ptmin) * 2.0 * M_PI * pt / 4.0 *
+          (1.0 * g2hfac * Norm * 9. / 32.);
+    }
+    ptd[2 * a] = P;
+    ptd[2 * a + 1] = g;
+    ptf[2 * a] = (float)P;
+    ptf[2 * a + 1] = (float)g;
+    int m = 0;
+    for (int b = 0; b < nkt; ++b)
+      for (size_t f = 0; f < uf.size(); ++f, ++m) {
+        const double kt = pt * xk[b], phi = 2 * M_PI * uf[f];
+        const double k1 = 0.25 * (pt * pt + kt * kt + 2 * kt * pt * std::cos(phi));
+        const double k2 = 0.25 * (pt * pt + kt * kt - 2 * pt * kt * std::cos(phi));
+        const double cn = wk[b] * ww[f] * xk[b] * alpha(std::max(std::max(k1, k2), pt * pt));
+        const size_t o = 4 * ((size_t)a * nn_pad + m);
+        nd[o] = cn;
+        nd[o + 1] = 1.0 / k1;
+        nd[o + 2] = 1.0 / k2;
+        nf[o] = (float)cn;
+        nf[o + 1] = (float)(1.0 / k1);
+        nf[o + 2] = (float)(1.0 / k2);
+      }
+  }
+  c->kln_npt = npt;
+  c->kln_nn = nn_pad;
+  c->kln_kc = KlnConst{lq2, amax, 1.0 / beta0};
+  c->kln_ptf = dalloc<float>(ptf.size());
+  c->kln_nodes_f = dalloc<float>(nf.size());
+  c->kln_ptd = dalloc<double>(ptd.size());
+  c->kln_nodes_d = dalloc<double>(nd.size());
+  HIPCHK(hipMemcpy(c->kln_ptf, ptf.data(), ptf.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->kln_nodes_f, nf.data(), nf.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->kln_ptd, ptd.data(), ptd.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->kln_nodes_d, nd.data(), nd.size() * sizeof(double), hipMemcpyHostToDevice));
+}
+
 void alloc_buffers(smc_ctx* c) {
   const size_t E = (size_t)c->cap;
   const size_t MP = (size_t)c->cfg.max_pairs;
@@ -419,9 +516,69 @@ void run_smear(smc_ctx* c, const SmearSpec& spec, float* out0, float* out1) {
   ts.done();
 }
 
+void kln_points(smc_ctx* c, int n, const double* ta_dev, const double* tb_dev, double* out_dev) {
+  TimerScope ts(c, SMC_K_KLN);
+  check_launch(launch_kln_points(ta_dev, tb_dev, out_dev, n, c->kln_ptd, c->kln_nodes_d, c->kln_npt, c->kln_nn,
+                                 c->kln_kc, c->stream),
+               "k_kln_points");
+  ts.done();
+}
+
+// MCnucl::makeTable (MCnucl.cpp:911-960): table[i][j] = getdNdy(0, i dT, j dT),
+// row and column 0 zero.
+void ensure_kln_table(smc_ctx* c) {
+  if (c->kln_table) return;
+  const int tmax = c->d.kln_tmax;
+  const double dT = c->d.kln_dT;
+  std::vector<double> a((size_t)tmax * tmax), b((size_t)tmax * tmax);
+  for (int i = 0; i < tmax; ++i)
+    for (int j = 0; j < tmax; ++j) {
+      const bool in = i > 0 && j > 0;
+      a[(size_t)i * tmax + j] = in ? dT * i : 0.0;
+      b[(size_t)i * tmax + j] = in ? dT * j : 0.0;
+    }
+  double* da = dalloc<double>(a.size());
+  double* db = dalloc<double>(b.size());
+  c->kln_table = dalloc<double>(a.size());
+  HIPCHK(hipMemcpyAsync(da, a.data(), a.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(db, b.data(), b.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  kln_points(c, (int)a.size(), da, db, c->kln_table);
+  sync(c);
+  (void)hipFree(da);
+  (void)hipFree(db);
+}
+
+// MCnucl::setDensity for which_mc_model 1 (MCnucl.cpp:654-687): T_A, T_B
+// (MCnucl.cpp:398-478), then per-cell dN/dy from the table (reference) or
+// from the quadrature in every cell (kln_direct).
+void run_kln_density(smc_ctx* c) {
+  float* a = grid_buffer(c, &c->ta);
+  float* b = grid_buffer(c, &c->tb);
+  run_smear(c, thickness_spec(c), a, b);
+  c->have_thickness = 1;
+  float* s = grid_buffer(c, &c->grid);
+  const size_t ncell_ev = (size_t)c->d.nx * c->d.ny;
+  if (!c->p.kln_direct) ensure_kln_table(c);
+  TimerScope ts(c, SMC_K_KLN);
+  if (c->p.kln_direct) {
+    check_launch(launch_kln_direct(a, b, s, ncell_ev * c->nev, c->kln_ptf, c->kln_nodes_f, c->kln_npt, c->kln_nn,
+                                   c->kln_kc, c->stream),
+                 "k_kln_direct");
+  } else {
+    check_launch(launch_kln_table(a, b, s, (int)ncell_ev, c->nev, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
+                                  c->buf.counts, c->stream),
+                 "k_kln_table");
+  }
+  ts.done();
+}
+
 void run_density(smc_ctx* c) {
   if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
-  if (c->p.which_mc_model == 1) throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN density not available yet");
+  if (c->p.which_mc_model == 1) {
+    run_kln_density(c);
+    c->have_density = 1;
+    return;
+  }
   const SmearSpec spec = density_spec(c);
   run_smear(c, spec, grid_buffer(c, &c->grid), spec.combine ? grid_buffer(c, &c->tmp) : nullptr);
   c->have_density = 1;
@@ -513,6 +670,7 @@ smc_status smc_create(smc_ctx** out, const smc_params* p) {
     validate_and_derive(c);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     build_tables(c);
+    build_kln(c);
     alloc_buffers(c);
     *out = c;
     return SMC_OK;
@@ -530,7 +688,7 @@ void smc_destroy(smc_ctx* c) {
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
-                  c->kln_table};
+                  c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -550,6 +708,19 @@ smc_status smc_sample(smc_ctx* c, uint64_t seed, uint64_t event0, int nev) {
   try {
     if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
     run_event(c, nev, seed, event0, 0);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_sample_b(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, const double* b) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    if (!b) throw SmcError(SMC_ERR_ARG, "null impact parameters");
+    for (int e = 0; e < nev; ++e)
+      if (!(b[e] >= 0.0 && b[e] < 2 * (56.0 - 12.0))) throw SmcError(SMC_ERR_ARG, "impact parameter out of range");
+    HIPCHK(hipMemcpyAsync(c->buf.bval, b, (size_t)nev * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    run_event(c, nev, seed, event0, 2);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }
@@ -741,16 +912,36 @@ smc_status smc_fetch_events(smc_ctx* c, int nev, double* ev_out) {
   } catch (const SmcError& e) { return fail(c, e); }
 }
 
-smc_status smc_kln_dndy(smc_ctx* c, int, const double*, const double*, double*) {
+smc_status smc_kln_dndy(smc_ctx* c, int n, const double* ta, const double* tb, double* out) {
   if (!c) return SMC_ERR_ARG;
-  c->err = "MC-KLN quadrature not built yet";
-  return SMC_ERR_UNSUPPORTED;
+  try {
+    if (n <= 0 || !ta || !tb || !out) throw SmcError(SMC_ERR_ARG, "bad KLN point arrays");
+    double* da = dalloc<double>((size_t)n);
+    double* db = dalloc<double>((size_t)n);
+    double* dout = dalloc<double>((size_t)n);
+    HIPCHK(hipMemcpyAsync(da, ta, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(db, tb, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    kln_points(c, n, da, db, dout);
+    HIPCHK(hipMemcpyAsync(out, dout, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    (void)hipFree(da);
+    (void)hipFree(db);
+    (void)hipFree(dout);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
 }
 
-smc_status smc_kln_make_table(smc_ctx* c, double*) {
+smc_status smc_kln_make_table(smc_ctx* c, double* table) {
   if (!c) return SMC_ERR_ARG;
-  c->err = "MC-KLN table not built yet";
-  return SMC_ERR_UNSUPPORTED;
+  try {
+    ensure_kln_table(c);
+    if (table) {
+      const size_t n = (size_t)c->d.kln_tmax * c->d.kln_tmax;
+      HIPCHK(hipMemcpyAsync(table, c->kln_table, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
 }
 
 smc_status smc_set_timing(smc_ctx* c, int on) {

@@ -93,6 +93,12 @@ constexpr int kStOk = 0;
 constexpr int kStNoEvent = 1;
 constexpr int kStPairOverflow = 2;
 constexpr int kStPosRange = 4;
+constexpr int kStKlnTable = 8;
+
+// running-coupling constants of KLNModel::getAlphaStrong (KLNModel.h:91-96)
+struct KlnConst {
+  double lq2, amax, inv_beta0;
+};
 
 // Smear kernel source kinds and accumulation modes
 enum SrcKind : int { kSrcPartA = 0, kSrcPartB = 1, kSrcPair = 2 };
@@ -109,7 +115,7 @@ struct SmearSpec {
 
 // Launch wrappers (smc_kernels.hip)
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
-                 unsigned long long event0, int fed, void* stream);
+                 unsigned long long event0, int mode, void* stream);
 int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,
                  float* out0, float* out1, void* stream);
 int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev,
@@ -118,8 +124,11 @@ int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  void* stream);
 int launch_strip_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
                        float* out1, void* stream);
-int launch_kln_points(const DevCfg& cfg, int n, const double* ta, const double* tb,
-                      double* out, const float* nodes, int npt, int nkt, int nphi,
-                      double ecm, double lambda, double ptmin, double ptmax, void* stream);
+int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell, const void* ptf,
+                      const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
+int launch_kln_points(const double* ta, const double* tb, double* out, int n, const void* ptd,
+                      const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
+int launch_kln_table(const float* ta, const float* tb, float* out, int ncell_ev, int nev,
+                     const double* table, int tmax, double dT, int* counts, void* stream);
 
 }  // namespace smc

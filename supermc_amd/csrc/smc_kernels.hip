@@ -20,7 +20,10 @@ namespace smc {
 
 // ------------------------------------------------------------- k_event
 __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
-                                               unsigned long long event0, int fed) {
+                                               unsigned long long event0, int mode) {
+  // mode 0: impact parameter sampled, events accepted as MakeDensity.cpp:2147-2162
+  // mode 1: fed positions; mode 2: sampled nuclei at the given b, one attempt
+  const int fed = mode == 1;
   const int e = blockIdx.x;
   const int t = threadIdx.x;
   const int lane = t & 63, wv = t >> 6;
@@ -57,7 +60,7 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
       // nuclei sit at +-b/2 rounded to the position quantum.
       if (t == 0) {
         const uint4 r = draw(key, gev, kStrB, 0, 0, attempt, 0);
-        const double b = sqrt((cfg.bmax2 - cfg.bmin2) * u53(r.x, r.y) + cfg.bmin2);
+        const double b = (mode == 2) ? buf.bval[e] : sqrt((cfg.bmax2 - cfg.bmin2) * u53(r.x, r.y) + cfg.bmin2);
         const int xc = (int)llrint(b * 0.5 / kQ);
         s_xc = xc;
         s_b = 2.0 * (double)xc * kQ;
@@ -218,7 +221,7 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
       ncoll = scan_tmp[0] + scan_tmp[1] + scan_tmp[2] + scan_tmp[3];
       __syncthreads();
     }
-    if (fed) break;
+    if (mode != 0) break;
     const int np = npa + npb;
     if (ncoll > 0 && np >= cfg.npmin && np <= cfg.npmax && !(status & kStNoEvent)) break;
     if (attempt + 1 >= cfg.max_attempts || (status & kStNoEvent)) { status |= kStNoEvent; break; }
@@ -419,10 +422,10 @@ __global__ __launch_bounds__(256) void k_smear(DevCfg cfg, EvBufs buf, SmearSpec
 static hipStream_t S(void* s) { return (hipStream_t)s; }
 
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
-                 unsigned long long event0, int fed, void* stream) {
+                 unsigned long long event0, int mode, void* stream) {
   if (nev <= 0) return 0;
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  hipLaunchKernelGGL(k_event, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, fed);
+  hipLaunchKernelGGL(k_event, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
   return (int)hipGetLastError();
 }
 

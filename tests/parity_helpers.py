@@ -45,7 +45,7 @@ def cpu_configs(p, nev, seed, bvals=None):
     return np.round(xa / Q) * Q, np.round(xb / Q) * Q, bs
 
 
-def oracle_event(ctx, ev):
+def oracle_event(ctx, ev, kln_table=None):
     """Runs the oracle on the nucleons (and fluct factors) of event ev as
     reported by the HIP path.  Returns a dict with the oracle collision
     result, densities and moments, plus the GPU lists for comparison."""
@@ -68,13 +68,18 @@ def oracle_event(ctx, ev):
     mid = (xa[pa] + xb[pb]) / 2.0
     g = oracle.grid(p.maxx, p.dx, p.maxy, p.dy)
     out["grid"] = g
-    if p.which_mc_model == 5:
-        s, dndy = oracle.density_glauber(g, w, p.alpha, xa[ia], fa, xb[ib], fb, mid, pf.astype(np.float64))
-    else:
-        s, dndy = oracle.density_trento(g, w, xa[ia], fa, xb[ib], fb)
-    out["s"], out["dndy"] = s, dndy
     out["TA"] = oracle.thickness(g, w, xa[ia])
     out["TB"] = oracle.thickness(g, w, xb[ib])
+    if p.which_mc_model == 5:
+        s, dndy = oracle.density_glauber(g, w, p.alpha, xa[ia], fa, xb[ib], fb, mid, pf.astype(np.float64))
+    elif p.which_mc_model == 7:
+        s, dndy = oracle.density_trento(g, w, xa[ia], fa, xb[ib], fb)
+    else:
+        # MC-KLN through the dN/dy table; the table itself is checked entry by
+        # entry against the oracle's quadrature (test_kln_table_entries)
+        table = kln_table if kln_table is not None else ctx.kln_make_table()
+        s, dndy = oracle.density_kln_table(g, out["TA"], out["TB"], table, d.kln_dT)
+    out["s"], out["dndy"] = s, dndy
     boxes = np.array([oracle.particle_box(x, y, w) for x, y in np.concatenate([xa[ia], xb[ib]])])
     out["boxes"] = boxes
     out["mom"] = oracle.moments(g, s * p.finalFactor, boxes, len(pa), p.ecc_from_order, p.ecc_to_order)
