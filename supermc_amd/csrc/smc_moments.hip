@@ -283,6 +283,7 @@ __device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const 
     out[SMC_EV_NCOLL] = ncoll;
     out[SMC_EV_B] = buf.bval[e];
     out[SMC_EV_STATUS] = cnt[3];
+    for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
   }
 }
 

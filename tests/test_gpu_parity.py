@@ -10,7 +10,9 @@ Bars (written here, checked below):
   MFMA path (smear_path 1, opt-in, not a parity path): it smears over the
   square +-5w window, whose corners (r > 5w, each below e^-12.5 of the
   source's peak) the reference leaves out, so its moment bars are wider:
-    eps_n, eps'_n: absolute <= 1e-5; <r^n>: relative <= 5e-5;
+    eps_n: absolute <= 1e-5; eps'_n: absolute <= 5e-5 (the r^n weight
+    grows toward the corners); <r^n>: relative <= 5e-4 (r^9 of a
+    peripheral event weights the corners most);
     dS/dy, grid sum: relative <= 1e-5
   centre of mass: absolute <= 1e-6 fm on every path
 """
@@ -58,10 +60,11 @@ def _check_event_record(rec, o, p):
     m = o["mom"]
     mfma = p.smear_path in MFMA_PATHS and p.which_mc_model == 5
     tol = 1e-5 if mfma else EPS_TOL
-    tol_rn = 5e-5 if mfma else 1e-5
+    tol_p = 5e-5 if mfma else EPS_TOL
+    tol_rn = 5e-4 if mfma else 1e-5
     for n in range(p.ecc_from_order, p.ecc_to_order + 1):
-        for key, off in (("mom_re", 0), ("mom_im", 10), ("momp_re", 20), ("momp_im", 30)):
-            assert abs(rec[off + n] - m[key][n]) <= tol, (key, n, rec[off + n], m[key][n])
+        for key, off, t in (("mom_re", 0, tol), ("mom_im", 10, tol), ("momp_re", 20, tol_p), ("momp_im", 30, tol_p)):
+            assert abs(rec[off + n] - m[key][n]) <= t, (key, n, rec[off + n], m[key][n])
     for n in range(10):
         assert rec[40 + n] == pytest.approx(m["rn"][n], rel=tol_rn)
     assert abs(rec[50] - m["xc"]) <= 1e-6 and abs(rec[51] - m["yc"]) <= 1e-6
