@@ -83,10 +83,10 @@ typedef struct smc_params {
   int max_events;              /* events per batch held on the device */
   int max_pairs;               /* per-event collision-pair capacity (<=0: Aproj*Atarg) */
   int max_attempts;            /* per-event acceptance attempts */
-  int smear_path;              /* 0 exact reference footprint, VALU strips  [default]
+  int smear_path;              /* 0 exact reference footprint, separable VALU tiles [default]
                                   1 MFMA fp32 strips: separable Gx.diag(w).Gy^T over the
                                     square window (no r <= 5w corner cut)
-                                  2 exact reference footprint, VALU tiles */
+                                  2 exact reference footprint, per-cell exponentials */
 } smc_params;
 
 /* Derived quantities (for the facade and tests) */

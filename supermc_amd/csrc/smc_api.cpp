@@ -501,16 +501,17 @@ SmearSpec thickness_spec(const smc_ctx* c) {
   return s;
 }
 
-// smear_path: 0 exact footprint, VALU strips (tiles for grids wider than
-// 320); 1 MFMA strips Gx.diag(w).Gy^T (square window, no r <= 5w corner cut;
-// grids up to 272 wide, not for sqrt(T_A T_B)); 2 exact footprint, VALU tiles.
+// smear_path: 0 exact footprint, separable VALU tiles (16 x 64, one wave);
+// 1 MFMA strips Gx.diag(w).Gy^T (square window, no r <= 5w corner cut;
+// grids up to 272 wide, not for sqrt(T_A T_B)); 2 exact footprint, per-cell
+// exponentials (256-thread tiles).
 void run_smear(smc_ctx* c, const SmearSpec& spec, float* out0, float* out1) {
   TimerScope ts(c, SMC_K_SMEAR);
   const int sp = c->p.smear_path;
   if (sp == 1 && c->d.ny <= 272 && spec.combine != 1)
     check_launch(launch_strip(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_strip");
-  else if (sp != 2 && c->d.ny <= 320)
-    check_launch(launch_strip_exact(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_strip_exact");
+  else if (sp != 2)
+    check_launch(launch_tile_exact(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile_exact");
   else
     check_launch(launch_smear(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_smear");
   ts.done();

@@ -122,8 +122,8 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
                    double* ev_out, int mode, void* stream);
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                  void* stream);
-int launch_strip_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
-                       float* out1, void* stream);
+int launch_tile_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
+                      float* out1, void* stream);
 int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell, const void* ptf,
                       const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
 int launch_kln_points(const double* ta, const double* tb, double* out, int n, const void* ptd,

@@ -1,4 +1,5 @@
-// smc_strip.hip -- MFMA smearing, one wavefront per 16-row strip of a grid.
+// smc_strip.hip -- separable Gaussian smearing: MFMA strips (square window)
+// and exact-footprint VALU tiles.
 //
 // The separable form of the Gaussian smearing (MCnucl.cpp:432-478 T_A/T_B,
 // :689-760 MC-Glauber, :762-811 sqrt(T_A T_B)):
@@ -136,38 +137,38 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NACC == 1 ? 
     }
 }
 
-// Exact-footprint smearing, one wavefront per 16-row strip.  Lane l owns
-// the columns l + 64 q (q < kXS) of all 16 rows.  Sources are applied one at
-// a time: each lane computes the column factor for the one or two slots the
-// source's y-window meets, lanes 0-15 compute the row factors, and every
-// cell takes w gx gy when it lies in the integer window and inside the disk
-// dx^2 + dy^2 <= 25 w^2 (MCnucl.cpp:450-475, :732-756, Particle.cpp:122-132).
-constexpr int kXS = 5;  // column slots per lane: grids up to 320 wide
-
-template <int ACC>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_strip_exact(
-    DevCfg cfg, EvBufs buf, SmearSpec spec, float* out) {
+// Exact-footprint smearing, one wavefront per 16 x 64 tile of a grid.
+// Each tile keeps its 16 rows x 64 columns in 16 accumulators per lane
+// (lane = column), selects the sources whose integer window meets the tile
+// (ordered ballot compaction, so the summation order is the source order),
+// and per source: lanes 0-15 compute the row factors {w gx(row),
+// 25 w^2 - dx^2} into LDS, every lane its column factor gy and dy^2, and each
+// cell takes w gx gy when dy^2 <= 25 w^2 - dx^2 (the disk cut of
+// MCnucl.cpp:450-475, :732-756, Particle.cpp:122-132).  Row factors reach
+// all lanes as LDS broadcasts, so a cell costs one compare, one select and
+// one FMA.
+__global__ __launch_bounds__(64) void k_tile_exact(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, float* out) {
   __shared__ StripEntry wl[64];
+  __shared__ float4 rd[8];  // row factors of the current source, two rows per entry
   const int lane = threadIdx.x;
-  const int R = blockIdx.x, e = blockIdx.y;
-  const int r0 = 16 * R;
   const int nx = cfg.nx, ny = cfg.ny;
+  const int ntx = (nx + 15) >> 4;
+  const int r0 = 16 * (blockIdx.x % ntx), c0 = 64 * (blockIdx.x / ntx);
+  const int e = blockIdx.y;
   const float c2 = cfg.inv2w2 * 1.4426950408889634f;
   const float cut2 = cfg.cut2;
-  int ygh[kXS];
-#pragma unroll
-  for (int q = 0; q < kXS; ++q) ygh[q] = (int)llrint((cfg.ymin + (lane + 64 * q) * cfg.dy) / kH);
+  const int col = c0 + lane;
+  const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
   const int myrow = r0 + (lane & 15);
   const int rowh = (int)llrint((cfg.xmin + myrow * cfg.dx) / kH);
   const int K = buf.counts[(size_t)e * 8 + 5];
   const int4* src = buf.src + (size_t)e * buf.src_cap;
   const int4* swin = buf.swin + (size_t)e * buf.src_cap;
+  float* rdf = reinterpret_cast<float*>(rd);
 
-  float acc[16][kXS];
+  float acc[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r)
-#pragma unroll
-    for (int q = 0; q < kXS; ++q) acc[r][q] = 0.f;
+  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 
   for (int sub = 0; sub < K; sub += 64) {
     const int k = sub + lane;
@@ -176,7 +177,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     if (k < K) {
       const int4 s = src[k];
       const int kind = s.w;
-      if (spec.enabled[kind] && spec.acc[kind] == ACC) {
+      if (spec.enabled[kind] && spec.acc[kind] == acc_sel) {
         const int4 wv = swin[k];
         en.xh = s.x;
         en.yh = s.y;
@@ -184,7 +185,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
         en.yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
         en.w = spec.scale[kind] * (spec.use_fluct[kind] ? __int_as_float(s.z) : 1.f);
         const int xlo = en.xw & 0xffff, xhi = en.xw >> 16, ylo = en.yw & 0xffff, yhi = en.yw >> 16;
-        sel = xlo < r0 + 16 && xhi > r0 && ylo < yhi;
+        sel = xlo < r0 + 16 && xhi > r0 && ylo < c0 + 64 && yhi > c0;
       }
     }
     const unsigned long long m = __ballot(sel);
@@ -194,54 +195,33 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) void
     for (int g = 0; g < n; ++g) {
       const StripEntry q = wl[g];
       const int xlo = q.xw & 0xffff, xhi = q.xw >> 16, ylo = q.yw & 0xffff, yhi = q.yw >> 16;
-      // row factors: lane r (< 16) for row r0 + r; the disk cut of a row is
-      // dy^2 <= 25 w^2 - dx^2
-      float gxw = 0.f, lim = 0.f;
-      {
+      if (lane < 16) {
         const float dx = (float)(rowh - q.xh) * kHf;
         const float dx2 = dx * dx;
-        lim = cut2 - dx2;
-        if (myrow >= xlo && myrow < xhi) gxw = q.w * __builtin_amdgcn_exp2f(-dx2 * c2);
+        const float gxw = (myrow >= xlo && myrow < xhi) ? q.w * __builtin_amdgcn_exp2f(-dx2 * c2) : 0.f;
+        rdf[2 * lane] = gxw;
+        rdf[2 * lane + 1] = cut2 - dx2;
       }
-      // column factors for the slots the y-window meets
-      const int q0 = ylo >> 6, q1 = (yhi - 1) >> 6;
-      float gy[kXS], dy2[kXS];
+      const float dy = (float)(ygh - q.yh) * kHf;
+      const float d2 = dy * dy;
+      const float gy = (col >= ylo && col < yhi) ? __builtin_amdgcn_exp2f(-d2 * c2) : 0.f;
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
-      for (int s = 0; s < kXS; ++s) {
-        gy[s] = 0.f;
-        dy2[s] = 0.f;
-        if (s >= q0 && s <= q1) {
-          const int col = lane + 64 * s;
-          const float dy = (float)(ygh[s] - q.yh) * kHf;
-          dy2[s] = dy * dy;
-          if (col >= ylo && col < yhi) gy[s] = __builtin_amdgcn_exp2f(-dy2[s] * c2);
-        }
+      for (int i = 0; i < 8; ++i) {
+        const float4 v = rd[i];
+        acc[2 * i] = fmaf(v.x, (d2 <= v.y) ? gy : 0.f, acc[2 * i]);
+        acc[2 * i + 1] = fmaf(v.z, (d2 <= v.w) ? gy : 0.f, acc[2 * i + 1]);
       }
-      const int rlo = max(xlo - r0, 0), rhi = min(xhi - r0, 16);
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        if (r < rlo || r >= rhi) continue;  // uniform
-        const float gr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(gxw), r));
-        const float lr = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(lim), r));
-#pragma unroll
-        for (int s = 0; s < kXS; ++s) {
-          if (s < q0 || s > q1) continue;  // uniform
-          acc[r][s] = fmaf(gr, (dy2[s] <= lr) ? gy[s] : 0.f, acc[r][s]);
-        }
-      }
+      __builtin_amdgcn_wave_barrier();
     }
     __builtin_amdgcn_wave_barrier();
   }
+  if (col >= ny) return;
   const size_t N = (size_t)nx * ny;
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int ix = r0 + r;
-    if (ix >= nx) continue;
-#pragma unroll
-    for (int s = 0; s < kXS; ++s) {
-      const int iy = lane + 64 * s;
-      if (iy < ny) out[(size_t)e * N + (size_t)ix * ny + iy] = acc[r][s];
-    }
+    if (ix < nx) out[(size_t)e * N + (size_t)ix * ny + col] = acc[r];
   }
 }
 
@@ -250,15 +230,14 @@ __global__ void k_combine_sqrt(float* a, const float* b, size_t n) {
   if (i < n) a[i] = sqrtf(a[i] * b[i]);
 }
 
-int launch_strip_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
+int launch_tile_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                        void* stream) {
   if (nev <= 0) return 0;
-  if (cfg.ny > 64 * kXS) return (int)hipErrorInvalidValue;
-  const dim3 grid((cfg.nx + 15) / 16, nev);
+  const dim3 grid(((cfg.nx + 15) / 16) * ((cfg.ny + 63) / 64), nev);
   hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL((k_strip_exact<0>), grid, dim3(64), 0, st, cfg, b, s, out0);
+  hipLaunchKernelGGL(k_tile_exact, grid, dim3(64), 0, st, cfg, b, s, 0, out0);
   if (s.combine != 0) {
-    hipLaunchKernelGGL((k_strip_exact<1>), grid, dim3(64), 0, st, cfg, b, s, out1);
+    hipLaunchKernelGGL(k_tile_exact, grid, dim3(64), 0, st, cfg, b, s, 1, out1);
     if (s.combine == 1) {
       const size_t n = (size_t)nev * cfg.nx * cfg.ny;
       hipLaunchKernelGGL(k_combine_sqrt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out0, out1, n);

@@ -71,9 +71,9 @@ def _check_event_record(rec, o, p):
     assert rec[52] == pytest.approx(m["total"], rel=_sum_tol(p))
 
 
-# smear_path 0: exact reference footprint, VALU strips (default)
+# smear_path 0: exact reference footprint, separable VALU tiles (default)
 #            1: MFMA strips Gx.diag(w).Gy^T over the square window
-#            2: exact reference footprint, VALU tiles
+#            2: exact reference footprint, per-cell exponentials
 PATHS = [0, 1, 2]
 MFMA_PATHS = (1,)
 
