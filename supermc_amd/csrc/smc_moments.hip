@@ -105,11 +105,187 @@ __device__ __forceinline__ void load_row(const float* g, int ix, int nx, int ny,
   }
 }
 
+// Pass 2 for any order range: <r^n> over the full grid and the masked eps_n
+// sums, rows prefetched.
+template <int NW, int NORD, int MAXC>
+__device__ void pass2_generic(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout,
+                              MomLds& L, const unsigned* mask, double total, double xc, double yc) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nx = cfg.nx, ny = cfg.ny;
+  const int* cnt = buf.counts + (size_t)e * 8;
+  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
+  const double ff = cfg.final_factor;
+  const int nwy = (ny + 31) >> 5;
+  MomAcc<NORD> M;
+#pragma unroll
+  for (int n = 0; n < 10; ++n) M.rn[n] = 0.0;
+#pragma unroll
+  for (int o = 0; o < NORD; ++o) M.mre[o] = M.mim[o] = M.mno[o] = M.pre[o] = M.pim[o] = M.pno[o] = 0.0;
+  const int from = cfg.from_order, to = cfg.to_order;
+  {
+    float cur[MAXC], nxt[MAXC];
+    load_row<MAXC>(g, wv, nx, ny, cur);
+    for (int ix = wv; ix < nx; ix += NW) {
+      load_row<MAXC>(g, ix + NW, nx, ny, nxt);
+      const double x = cfg.xmin + ix * cfg.dx - xc;
+      const unsigned* row = mask + (size_t)ix * nwy;
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) {
+        if (cur[j] != 0.f) {
+          const int iy = lane + 64 * j;
+          const double y = cfg.ymin + iy * cfg.dy - yc;
+          const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
+          moment_cell<NORD>(M, from, to, (double)cur[j] * ff, x, y, masked);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    }
+  }
+  constexpr int NV = 10 + 6 * NORD;
+  double all[NV];
+#pragma unroll
+  for (int n = 0; n < 10; ++n) all[n] = M.rn[n];
+#pragma unroll
+  for (int o = 0; o < NORD; ++o) {
+    all[10 + 6 * o + 0] = M.mre[o];
+    all[10 + 6 * o + 1] = M.mim[o];
+    all[10 + 6 * o + 2] = M.mno[o];
+    all[10 + 6 * o + 3] = M.pre[o];
+    all[10 + 6 * o + 4] = M.pim[o];
+    all[10 + 6 * o + 5] = M.pno[o];
+  }
+  block_sums<NW, NV>(all, L);
+  if (t == 0) {
+    const double* T = L.tot;
+    double* out = evout + (size_t)e * SMC_EV_STRIDE;
+    for (int i = 0; i < 40; ++i) out[i] = 0.0;
+    for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
+    const double eps = 1e-15;
+    for (int o = 0; o < NORD; ++o) {
+      const int n = from + o;
+      if (n > to) break;
+      out[SMC_EV_EPS_RE + n] = -T[10 + 6 * o + 0] / (T[10 + 6 * o + 2] + eps);
+      out[SMC_EV_EPS_IM + n] = -T[10 + 6 * o + 1] / (T[10 + 6 * o + 2] + eps);
+      out[SMC_EV_EPSP_RE + n] = -T[10 + 6 * o + 3] / (T[10 + 6 * o + 5] + eps);
+      out[SMC_EV_EPSP_IM + n] = -T[10 + 6 * o + 4] / (T[10 + 6 * o + 5] + eps);
+    }
+    out[SMC_EV_XC] = xc;
+    out[SMC_EV_YC] = yc;
+    out[SMC_EV_TOTAL] = total;
+    out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
+    out[SMC_EV_NPART] = npa + npb;
+    out[SMC_EV_NCOLL] = ncoll;
+    out[SMC_EV_B] = buf.bval[e];
+    out[SMC_EV_STATUS] = cnt[3];
+    for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
+  }
+}
+
+
+// Pass 2 for the orders FROM..TO (FROM >= 2) fixed at compile time: r and
+// 1/r from one reciprocal square root, d r^n shared between <r^n> and the
+// r^n-weighted sums, one r^2 normalisation for all orders.
+template <int NW, int FROM, int TO, int MAXC>
+__device__ void pass2_fixed(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout,
+                            MomLds& L, const unsigned* mask, double total, double xc, double yc) {
+  static_assert(FROM >= 2 && TO <= 9 && FROM <= TO, "orders");
+  constexpr int NORD = TO - FROM + 1;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nx = cfg.nx, ny = cfg.ny;
+  const int* cnt = buf.counts + (size_t)e * 8;
+  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
+  const double ff = cfg.final_factor;
+  const int nwy = (ny + 31) >> 5;
+  constexpr int NV = 10 + 1 + 5 * NORD;  // rn[10], r^2 norm, per order re/im (r^2), re/im/norm (r^n)
+  double a[NV];
+#pragma unroll
+  for (int i = 0; i < NV; ++i) a[i] = 0.0;
+  {
+    float cur[MAXC], nxt[MAXC];
+    load_row<MAXC>(g, wv, nx, ny, cur);
+    for (int ix = wv; ix < nx; ix += NW) {
+      load_row<MAXC>(g, ix + NW, nx, ny, nxt);
+      const double x = cfg.xmin + ix * cfg.dx - xc;
+      const unsigned* row = mask + (size_t)ix * nwy;
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) {
+        if (cur[j] != 0.f) {
+          const int iy = lane + 64 * j;
+          const double y = cfg.ymin + iy * cfg.dy - yc;
+          const double d = (double)cur[j] * ff;
+          const double r2 = fma(x, x, y * y);
+          const double rs = (r2 > 0.0) ? rsqrt(r2) : 0.0;
+          const double r = r2 * rs;
+          double pw[TO + 1];
+          pw[0] = d;
+#pragma unroll
+          for (int n = 1; n <= TO; ++n) pw[n] = pw[n - 1] * r;
+          double q = pw[TO];
+#pragma unroll
+          for (int n = 0; n <= TO; ++n) a[n] += pw[n];
+#pragma unroll
+          for (int n = TO + 1; n < 10; ++n) {
+            q *= r;
+            a[n] += q;
+          }
+          const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
+          if (masked && r2 > 0.0) {
+            a[10] = fma(r2, d, a[10]);
+            double zr = x, zi = y, dw = d;  // z^n and d r^(2-n)
+#pragma unroll
+            for (int n = 2; n <= TO; ++n) {
+              const double nr = fma(zr, x, -zi * y), ni = fma(zr, y, zi * x);
+              zr = nr;
+              zi = ni;
+              if (n > 2) dw *= rs;
+              if (n >= FROM) {
+                double* o = a + 11 + 5 * (n - FROM);
+                o[0] = fma(zr, dw, o[0]);
+                o[1] = fma(zi, dw, o[1]);
+                o[2] = fma(zr, d, o[2]);
+                o[3] = fma(zi, d, o[3]);
+                o[4] += pw[n];
+              }
+            }
+          }
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    }
+  }
+  block_sums<NW, NV>(a, L);
+  if (t == 0) {
+    const double* T = L.tot;
+    double* out = evout + (size_t)e * SMC_EV_STRIDE;
+    for (int i = 0; i < 40; ++i) out[i] = 0.0;
+    for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
+    const double eps = 1e-15;
+    for (int n = FROM; n <= TO; ++n) {
+      const double* o = T + 11 + 5 * (n - FROM);
+      out[SMC_EV_EPS_RE + n] = -o[0] / (T[10] + eps);
+      out[SMC_EV_EPS_IM + n] = -o[1] / (T[10] + eps);
+      out[SMC_EV_EPSP_RE + n] = -o[2] / (o[4] + eps);
+      out[SMC_EV_EPSP_IM + n] = -o[3] / (o[4] + eps);
+    }
+    out[SMC_EV_XC] = xc;
+    out[SMC_EV_YC] = yc;
+    out[SMC_EV_TOTAL] = total;
+    out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
+    out[SMC_EV_NPART] = npa + npb;
+    out[SMC_EV_NCOLL] = ncoll;
+    out[SMC_EV_B] = buf.bval[e];
+    out[SMC_EV_STATUS] = cnt[3];
+    for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
+  }
+}
+
 // Moments of one event's grid g[nx][ny] (global memory) for a workgroup of
 // NW waves.  s1 holds this thread's partial (sum rho, sum x rho, sum y rho)
 // when the caller computed them (have_s1), otherwise pass 1 reads the grid.
 // ff = finalFactor (setSd, MakeDensity.cpp:2514-2522).
-template <int NW, int NORD, int MAXC>
+template <int NW, int NORD, int MAXC, bool FIX25>
 __device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout,
                              MomLds& L, unsigned* mask, bool have_s1, double (&s1)[3]) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
@@ -220,71 +396,10 @@ __device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const 
   const double total = L.tot[0];
   const double xc = L.tot[1] / total, yc = L.tot[2] / total;
 
-  // pass 2: <r^n> over the full grid and the masked eps_n sums, rows prefetched
-  MomAcc<NORD> M;
-#pragma unroll
-  for (int n = 0; n < 10; ++n) M.rn[n] = 0.0;
-#pragma unroll
-  for (int o = 0; o < NORD; ++o) M.mre[o] = M.mim[o] = M.mno[o] = M.pre[o] = M.pim[o] = M.pno[o] = 0.0;
-  const int from = cfg.from_order, to = cfg.to_order;
-  {
-    float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, wv, nx, ny, cur);
-    for (int ix = wv; ix < nx; ix += NW) {
-      load_row<MAXC>(g, ix + NW, nx, ny, nxt);
-      const double x = cfg.xmin + ix * cfg.dx - xc;
-      const unsigned* row = mask + (size_t)ix * nwy;
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j) {
-        if (cur[j] != 0.f) {
-          const int iy = lane + 64 * j;
-          const double y = cfg.ymin + iy * cfg.dy - yc;
-          const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
-          moment_cell<NORD>(M, from, to, (double)cur[j] * ff, x, y, masked);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
-    }
-  }
-  constexpr int NV = 10 + 6 * NORD;
-  double all[NV];
-#pragma unroll
-  for (int n = 0; n < 10; ++n) all[n] = M.rn[n];
-#pragma unroll
-  for (int o = 0; o < NORD; ++o) {
-    all[10 + 6 * o + 0] = M.mre[o];
-    all[10 + 6 * o + 1] = M.mim[o];
-    all[10 + 6 * o + 2] = M.mno[o];
-    all[10 + 6 * o + 3] = M.pre[o];
-    all[10 + 6 * o + 4] = M.pim[o];
-    all[10 + 6 * o + 5] = M.pno[o];
-  }
-  block_sums<NW, NV>(all, L);
-  if (t == 0) {
-    const double* T = L.tot;
-    double* out = evout + (size_t)e * SMC_EV_STRIDE;
-    for (int i = 0; i < 40; ++i) out[i] = 0.0;
-    for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
-    const double eps = 1e-15;
-    for (int o = 0; o < NORD; ++o) {
-      const int n = from + o;
-      if (n > to) break;
-      out[SMC_EV_EPS_RE + n] = -T[10 + 6 * o + 0] / (T[10 + 6 * o + 2] + eps);
-      out[SMC_EV_EPS_IM + n] = -T[10 + 6 * o + 1] / (T[10 + 6 * o + 2] + eps);
-      out[SMC_EV_EPSP_RE + n] = -T[10 + 6 * o + 3] / (T[10 + 6 * o + 5] + eps);
-      out[SMC_EV_EPSP_IM + n] = -T[10 + 6 * o + 4] / (T[10 + 6 * o + 5] + eps);
-    }
-    out[SMC_EV_XC] = xc;
-    out[SMC_EV_YC] = yc;
-    out[SMC_EV_TOTAL] = total;
-    out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
-    out[SMC_EV_NPART] = npa + npb;
-    out[SMC_EV_NCOLL] = ncoll;
-    out[SMC_EV_B] = buf.bval[e];
-    out[SMC_EV_STATUS] = cnt[3];
-    for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
-  }
+  if constexpr (FIX25)
+    pass2_fixed<NW, 2, 5, MAXC>(cfg, buf, e, g, evout, L, mask, total, xc, yc);
+  else
+    pass2_generic<NW, NORD, MAXC>(cfg, buf, e, g, evout, L, mask, total, xc, yc);
 }
 
 // Sum of the grid only (dndy, MCnucl.cpp:817-820), no finalFactor.
@@ -299,15 +414,15 @@ __device__ void grid_sum(const DevCfg& cfg, int e, const float* g, double* evout
 }
 
 // Moments of grids already in memory (non-fused densities), same code path.
-template <int NORD>
-__global__ __launch_bounds__(kGT) void k_moments2(DevCfg cfg, EvBufs buf, const float* grid, double* evout,
+template <int NORD, int MAXC, bool FIX25>
+__global__ __launch_bounds__(kGT) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 1, 8))) void k_moments2(DevCfg cfg, EvBufs buf, const float* grid, double* evout,
                                                   int flags) {
   __shared__ MomLds L;
   extern __shared__ unsigned dyn_mask[];
   const int e = blockIdx.x;
   const float* g = grid + (size_t)e * cfg.nx * cfg.ny;
   double s1[3] = {0.0, 0.0, 0.0};
-  if (flags & 2) grid_moments<kGW, NORD, 9>(cfg, buf, e, g, evout, L, dyn_mask, false, s1);
+  if (flags & 2) grid_moments<kGW, NORD, MAXC, FIX25>(cfg, buf, e, g, evout, L, dyn_mask, false, s1);
   else grid_sum<kGW>(cfg, e, g, evout, L);
 }
 
@@ -319,8 +434,15 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
   const int nord = cfg.to_order - cfg.from_order + 1;
   hipStream_t st = (hipStream_t)stream;
   const size_t shm = (size_t)cfg.nx * ((cfg.ny + 31) / 32) * sizeof(unsigned);
-  if (nord <= 4) hipLaunchKernelGGL((k_moments2<4>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
-  else hipLaunchKernelGGL((k_moments2<9>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
+  const bool narrow = cfg.ny <= 320;
+  if (cfg.from_order == 2 && cfg.to_order == 5 && narrow)
+    hipLaunchKernelGGL((k_moments2<4, 5, true>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
+  else if (nord <= 4 && narrow)
+    hipLaunchKernelGGL((k_moments2<4, 5, false>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
+  else if (nord <= 4)
+    hipLaunchKernelGGL((k_moments2<4, 9, false>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
+  else
+    hipLaunchKernelGGL((k_moments2<9, 9, false>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
   return (int)hipGetLastError();
 }
 
