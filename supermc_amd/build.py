@@ -28,6 +28,37 @@ def _run(cmd):
     subprocess.run(cmd, check=True)
 
 
+HOST_DIR = os.path.join(HERE, "host")
+FACADE_SOURCES = ["ParameterReader.cpp", "smc_host.cpp", "MCnucl.cpp", "MakeDensity.cpp", "KLNModel.cpp"]
+CXX = os.environ.get("CXX", "g++")
+
+
+def build_facade():
+    """The reference-interface C++ facade (supermc_amd/host) as
+    lib/libsmc_facade.so plus the superMC.e driver in bin/, both linked
+    against lib/libsmc.so (g++ only: the facade calls the C-ABI)."""
+    bindir = os.path.join(HERE, "bin")
+    os.makedirs(bindir, exist_ok=True)
+    inc = ["-I", os.path.join(ROOT, "include"), "-I", HOST_DIR]
+    rpath = "-Wl,-rpath,$ORIGIN:$ORIGIN/../lib"
+    srcs = [os.path.join(HOST_DIR, s) for s in FACADE_SOURCES]
+    lib = os.path.join(LIBDIR, "libsmc_facade.so")
+    _run([CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", *inc, *srcs, "-o", lib + ".tmp",
+          "-L", LIBDIR, "-lsmc", rpath])
+    os.replace(lib + ".tmp", lib)
+    exe = os.path.join(bindir, "superMC.e")
+    _run([CXX, "-O2", "-std=c++17", "-Wall", *inc, os.path.join(HOST_DIR, "superMC.cpp"), "-o", exe + ".tmp",
+          "-L", LIBDIR, "-lsmc_facade", "-lsmc", rpath])
+    os.replace(exe + ".tmp", exe)
+    check_src = os.path.join(ROOT, "tests", "facade", "facade_check.cpp")
+    if os.path.exists(check_src):  # the facade's test driver
+        chk = os.path.join(ROOT, "tests", "facade", "facade_check.bin")
+        _run([CXX, "-O2", "-std=c++17", "-Wall", *inc, check_src, "-o", chk + ".tmp", "-L", LIBDIR,
+              "-lsmc_facade", "-lsmc", "-Wl,-rpath," + LIBDIR + ":$ORIGIN/../../supermc_amd/lib"])
+        os.replace(chk + ".tmp", chk)
+    return lib, exe
+
+
 def build(verbose=True):
     os.makedirs(LIBDIR, exist_ok=True)
     objs = []
@@ -47,6 +78,7 @@ def build(verbose=True):
     os.replace(tmp, out)
     for o in objs:
         os.remove(o)
+    build_facade()
     return out
 
 

@@ -1,0 +1,91 @@
+#include "MCnucl.h"
+
+#include <cstdlib>
+#include <iostream>
+
+#include "smc_host.h"
+
+using smc_host::check;
+
+MCnucl::MCnucl(ParameterReader* paraRdr) {
+  p_ = smc_host::params_from_reader(*paraRdr, 1);
+  seed_ = smc_host::seed_from_reader(*paraRdr);
+  check(nullptr, smc_create(&ctx_, &p_), "MCnucl::MCnucl");
+  smc_derived d;
+  check(ctx_, smc_get_derived(ctx_, &d), "MCnucl::MCnucl");
+  nx_ = d.nx;
+  ny_ = d.ny;
+  dx_ = p_.dx;
+  dy_ = p_.dy;
+  npmin_ = p_.Npmin;
+  npmax_ = p_.Npmax;
+  const size_t n = (size_t)nx_ * ny_;
+  ta_.assign(n, 0.0);
+  tb_.assign(n, 0.0);
+  rho_.assign(n, 0.0);
+}
+
+MCnucl::~MCnucl() { smc_destroy(ctx_); }
+
+// MCnucl.cpp:208-214 with Nucleus::populate: both nuclei at +-b/2, sampled
+// on the device (event counter_ of stream seed_), then the collision step.
+void MCnucl::generateNuclei(double b) {
+  check(ctx_, smc_sample_b(ctx_, seed_, counter_++, 1, &b), "MCnucl::generateNuclei");
+  have_event_ = true;
+  ncoll_ = npart1_ = npart2_ = 0;
+}
+
+void MCnucl::deleteNucleus() { have_event_ = false; }
+
+// MCnucl.cpp:217-308: Npart1/2 and the pair list; returns Ncoll.
+int MCnucl::getBinaryCollision() {
+  if (!have_event_) {
+    std::cerr << "MCnucl::getBinaryCollision: call generateNuclei first" << std::endl;
+    std::exit(1);
+  }
+  check(ctx_, smc_collide(ctx_, &npart1_, &npart2_, &ncoll_), "MCnucl::getBinaryCollision");
+  return ncoll_;
+}
+
+// MCnucl.cpp:388-393
+int MCnucl::CentralityCut() {
+  const int n = npart1_ + npart2_;
+  return (n <= npmax_ && n >= npmin_) ? 1 : 0;
+}
+
+// MCnucl.cpp:398-478
+void MCnucl::calculateThickness() {
+  std::vector<float> a(ta_.size()), b(tb_.size());
+  check(ctx_, smc_thickness(ctx_, a.data(), b.data()), "MCnucl::calculateThickness");
+  for (size_t i = 0; i < a.size(); ++i) {
+    ta_[i] = a[i];
+    tb_[i] = b[i];
+  }
+}
+
+// MCnucl.cpp:626-820 for ipt < 0
+void MCnucl::setDensity(int iy, int ipt) {
+  if (iy != 0 || ipt >= 0) {
+    std::cerr << "MCnucl::setDensity: only iy = 0, ipt < 0 (pT-integrated, y = 0) is on this path" << std::endl;
+    std::exit(1);
+  }
+  std::vector<float> s(rho_.size());
+  check(ctx_, smc_density(ctx_, s.data(), &dndy_), "MCnucl::setDensity");
+  for (size_t i = 0; i < s.size(); ++i) rho_[i] = s[i];
+}
+
+// MCnucl.cpp:911-960 (built on the device, kept in the context)
+void MCnucl::makeTable() { check(ctx_, smc_kln_make_table(ctx_, nullptr), "MCnucl::makeTable"); }
+
+void MCnucl::getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_proj, std::vector<double>& xy_targ,
+                         std::vector<int>& wound_targ) {
+  xy_proj.assign(2 * (size_t)p_.Aproj, 0.0);
+  xy_targ.assign(2 * (size_t)p_.Atarg, 0.0);
+  wound_proj.assign(p_.Aproj, -1);
+  wound_targ.assign(p_.Atarg, -1);
+  std::vector<float> fp(p_.Aproj), ft(p_.Atarg);
+  check(ctx_,
+        smc_get_nucleons(ctx_, 0, xy_proj.data(), wound_proj.data(), fp.data(), xy_targ.data(), wound_targ.data(),
+                         ft.data()),
+        "MCnucl::getNucleons");
+}

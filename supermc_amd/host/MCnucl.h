@@ -1,0 +1,66 @@
+// MCnucl -- one-event-at-a-time interface of the reference's MCnucl
+// (MCnucl.h:87-146) on top of libsmc.  Each call drives the gfx950 kernels
+// for a batch of one event; batched event loops live in MakeDensity.
+//
+// Differences from the reference class (the rest keeps its signatures and
+// meaning):
+//  - generateNuclei(b) samples both nuclei and runs the collision step on
+//    the device; getBinaryCollision() returns that event's Ncoll.
+//  - setDensity(iy, ipt) supports ipt < 0 (pT-integrated) and iy = 0.
+//  - KLN (which_mc_model 1) uses the library's dN/dy table, so setKLN /
+//    setLgX are not needed and getHotSpots is not offered.
+//  - lastCx1 .. lastPh2 are 0 (the reference never assigns them).
+#ifndef SMC_HOST_MCNUCL_H
+#define SMC_HOST_MCNUCL_H
+
+#include <cstdint>
+#include <vector>
+
+#include "ParameterReader.h"
+#include "smc.h"
+
+class MCnucl {
+ public:
+  explicit MCnucl(ParameterReader* paraRdr);
+  ~MCnucl();
+  MCnucl(const MCnucl&) = delete;
+  MCnucl& operator=(const MCnucl&) = delete;
+
+  double lastCx1 = 0, lastPh1 = 0, lastCx2 = 0, lastPh2 = 0;
+
+  double getTA1(int x, int y) { return ta_[(size_t)x * ny_ + y]; }
+  double getTA2(int x, int y) { return tb_[(size_t)x * ny_ + y]; }
+  double getRho(int i, int x, int y) { (void)i; return rho_[(size_t)x * ny_ + y]; }
+  void setRho(int i, int x, int y, double val) { (void)i; rho_[(size_t)x * ny_ + y] = val; }
+  int getNcoll() { return ncoll_; }
+  int getNpart1() { return npart1_; }
+  int getNpart2() { return npart2_; }
+  double getdNdy() { return dndy_ * dx_ * dy_; }
+
+  void generateNuclei(double b);
+  void deleteNucleus();
+  void setDensity(int iy, int ipt);
+  void calculateThickness();
+  int getBinaryCollision();
+  int CentralityCut();
+  void setCentralityCut(int Nmin, int Nmax) { npmax_ = Nmax; npmin_ = Nmin; }
+  void makeTable();
+
+  // the event's nucleons in collision order (x, y in fm; wound rank or -1)
+  void getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_proj, std::vector<double>& xy_targ,
+                   std::vector<int>& wound_targ);
+  smc_ctx* context() { return ctx_; }
+
+ private:
+  smc_ctx* ctx_ = nullptr;
+  smc_params p_;
+  uint64_t seed_ = 0, counter_ = 0;
+  int nx_ = 0, ny_ = 0;
+  double dx_ = 0, dy_ = 0;
+  int npart1_ = 0, npart2_ = 0, ncoll_ = 0, npmin_ = 0, npmax_ = 0;
+  double dndy_ = 0;
+  bool have_event_ = false;
+  std::vector<double> ta_, tb_, rho_;
+};
+
+#endif

@@ -1,0 +1,174 @@
+#include "MakeDensity.h"
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <iomanip>
+#include <iostream>
+
+#include "smc_host.h"
+
+using smc_host::check;
+
+MakeDensity::MakeDensity(ParameterReader* paraRdr) : rdr_(paraRdr) {
+  if (rdr_->exist("gpu_batch_events")) batch_ = std::max(1, (int)rdr_->getVal("gpu_batch_events"));
+  p_ = smc_host::params_from_reader(*rdr_, batch_);
+  seed_ = smc_host::seed_from_reader(*rdr_);
+  check(nullptr, smc_create(&ctx_, &p_), "MakeDensity::MakeDensity");
+  smc_derived d;
+  check(ctx_, smc_get_derived(ctx_, &d), "MakeDensity::MakeDensity");
+  nx_ = d.nx;
+  ny_ = d.ny;
+  xmin_ = d.xmin;
+  ymin_ = d.ymin;
+  // MakeDensity.cpp:2173-2181 (optional dS/dy cut)
+  cut_dsdy_ = rdr_->exist("cutdSdy") ? (int)rdr_->getVal("cutdSdy") : 0;
+  if (cut_dsdy_ == 1) {
+    cut_lo_ = rdr_->getVal("cutdSdy_lowerBound");
+    cut_hi_ = rdr_->getVal("cutdSdy_upperBound");
+  }
+  deformed_ = p_.proj_deformed == 1 || p_.targ_deformed == 1;
+  // MC-KLN: the dN/dy table once, as MakeDensity's constructor does
+  if (p_.which_mc_model == 1 && !p_.kln_direct) check(ctx_, smc_kln_make_table(ctx_, nullptr), "makeTable");
+}
+
+MakeDensity::~MakeDensity() { smc_destroy(ctx_); }
+
+int MakeDensity::next_batch(int want) {
+  const int n = std::min(want, batch_);
+  batch_rec_.resize((size_t)n * SMC_EV_STRIDE);
+  check(ctx_, smc_run(ctx_, seed_, counter_, n, batch_rec_.data()), "MakeDensity: event batch");
+  counter_ += (uint64_t)n;
+  return n;
+}
+
+bool MakeDensity::accept(const double* rec) const {
+  if (rec[SMC_EV_STATUS] != 0.0) return false;  // no accepted configuration / capacity
+  if (cut_dsdy_ == 1) {
+    // gettotaldSdy: sum rho dx dy without finalFactor (MakeDensity.cpp:2557-2567)
+    const double ff = p_.finalFactor;
+    const double s = (ff != 0.0) ? rec[SMC_EV_TOTAL] / ff * p_.dx * p_.dy : 0.0;
+    if (s < cut_lo_ || s > cut_hi_) return false;
+  }
+  return true;
+}
+
+void MakeDensity::dumpEccentricities(const char* base_filename, const double* rec, int from_order, int to_order,
+                                     double Npart_current, double Nbin_current, double b) {
+  const int max_order = 10;
+  char buffer[512];
+  const double dsdy = rec[SMC_EV_DSDY];
+  auto tail = [&](std::ofstream& of) {
+    of << std::setprecision(5) << std::setw(10) << Npart_current << std::setprecision(5) << std::setw(10)
+       << Nbin_current << std::setprecision(8) << std::setw(16) << dsdy << std::setprecision(8) << std::setw(16)
+       << b;
+    if (deformed_)
+      for (int k = 0; k < 4; ++k) of << std::setprecision(8) << std::setw(16) << 0.0;
+    of << std::endl;
+  };
+  for (int order = from_order; order <= to_order; ++order) {
+    std::snprintf(buffer, sizeof(buffer), base_filename, order);
+    std::ofstream of(buffer, std::ios_base::app);
+    of << std::setprecision(8) << std::setw(16) << rec[SMC_EV_EPS_RE + order] << std::setprecision(8)
+       << std::setw(16) << rec[SMC_EV_EPS_IM + order] << std::setprecision(8) << std::setw(16)
+       << rec[SMC_EV_EPSP_RE + order] << std::setprecision(8) << std::setw(16) << rec[SMC_EV_EPSP_IM + order]
+       << std::setprecision(8) << std::setw(16) << rec[SMC_EV_RN + order];
+    tail(of);
+  }
+  std::snprintf(buffer, sizeof(buffer), base_filename, max_order);
+  std::ofstream of(buffer, std::ios_base::app);
+  for (int order = 1; order < max_order; ++order)
+    of << std::setprecision(8) << std::setw(16) << rec[SMC_EV_EPS_RE + order] << std::setprecision(8)
+       << std::setw(16) << rec[SMC_EV_EPS_IM + order] << std::setprecision(8) << std::setw(16)
+       << rec[SMC_EV_EPSP_RE + order] << std::setprecision(8) << std::setw(16) << rec[SMC_EV_EPSP_IM + order]
+       << std::setprecision(8) << std::setw(16) << rec[SMC_EV_RN + order];
+  tail(of);
+}
+
+void MakeDensity::generateEccTable(int nevent) {
+  const int from = p_.ecc_from_order, to = p_.ecc_to_order;
+  const bool use_sd = rdr_->getVal("use_sd") != 0, use_ed = rdr_->getVal("use_ed") != 0;
+  records_.clear();
+  int written = 0;
+  while (written < nevent) {
+    const int n = next_batch(nevent - written);
+    for (int e = 0; e < n && written < nevent; ++e) {
+      const double* rec = &batch_rec_[(size_t)e * SMC_EV_STRIDE];
+      if (!accept(rec)) {
+        ++dropped_;
+        continue;
+      }
+      // setSd / setEd give the same profile here (MakeDensity.cpp:2526-2535)
+      if (use_sd) dumpEccentricities("data/sn_ecc_eccp_%d.dat", rec, from, to, rec[SMC_EV_NPART],
+                                     rec[SMC_EV_NCOLL], rec[SMC_EV_B]);
+      if (use_ed) dumpEccentricities("data/en_ecc_eccp_%d.dat", rec, from, to, rec[SMC_EV_NPART],
+                                     rec[SMC_EV_NCOLL], rec[SMC_EV_B]);
+      records_.insert(records_.end(), rec, rec + SMC_EV_STRIDE);
+      ++written;
+    }
+  }
+}
+
+void MakeDensity::dumpDensityBlock(const char* filename, const float* s, double factor) {
+  // MakeDensity.cpp:2595-2608
+  std::ofstream of(filename, std::ios_base::out);
+  for (int i = 0; i < nx_; ++i) {
+    for (int j = 0; j < ny_; ++j) of << std::setprecision(12) << std::setw(22) << (double)s[(size_t)i * ny_ + j] * factor;
+    of << std::endl;
+  }
+}
+
+void MakeDensity::dumpDensity4Col(const char* filename, const float* s, double factor, double npart) {
+  // MakeDensity.cpp:2573-2593 (one rapidity bin at y = 0)
+  std::ofstream of(filename, std::ios_base::out);
+  of << "# <npart>= " << npart << " xmax= " << nx_ << " ymax= " << ny_ << std::endl;
+  for (int i = 0; i < nx_; ++i)
+    for (int j = 0; j < ny_; ++j) {
+      const double x = xmin_ + i * p_.dx, y = ymin_ + j * p_.dy, rap = 0.0;
+      of << std::setprecision(3) << std::setw(10) << rap << std::setprecision(3) << std::setw(10) << x
+         << std::setprecision(3) << std::setw(10) << y << std::setprecision(12) << std::setw(22)
+         << (double)s[(size_t)i * ny_ + j] * factor << std::endl;
+    }
+}
+
+void MakeDensity::generate_profile_ebe(int nevent) {
+  const bool use_sd = rdr_->getVal("use_sd") != 0, use_ed = rdr_->getVal("use_ed") != 0;
+  const bool use_block = rdr_->getVal("use_block") != 0, use_4col = rdr_->getVal("use_4col") != 0;
+  const size_t ncell = (size_t)nx_ * ny_;
+  std::vector<float> grid;
+  std::vector<double> dndy;
+  records_.clear();
+  int event = 1;
+  char buffer[512];
+  while (event <= nevent) {
+    const int n = next_batch(nevent - event + 1);
+    grid.resize((size_t)n * ncell);
+    dndy.resize(n);
+    const smc_status st = smc_density(ctx_, grid.data(), dndy.data());
+    if (st != SMC_ERR_DNDY_ZERO) check(ctx_, st, "MakeDensity::generate_profile_ebe");
+    for (int e = 0; e < n && event <= nevent; ++e) {
+      const double* rec = &batch_rec_[(size_t)e * SMC_EV_STRIDE];
+      if (!accept(rec)) {
+        ++dropped_;
+        continue;
+      }
+      const float* s = &grid[(size_t)e * ncell];
+      const char* kinds[2] = {"sd", "ed"};
+      const bool on[2] = {use_sd, use_ed};
+      for (int k = 0; k < 2; ++k) {
+        if (!on[k]) continue;
+        if (use_4col) {
+          std::snprintf(buffer, sizeof(buffer), "data/%s_event_%d_4col.dat", kinds[k], event);
+          dumpDensity4Col(buffer, s, p_.finalFactor, rec[SMC_EV_NPART]);
+        }
+        if (use_block) {
+          std::snprintf(buffer, sizeof(buffer), "data/%s_event_%d_block.dat", kinds[k], event);
+          dumpDensityBlock(buffer, s, p_.finalFactor);
+        }
+      }
+      records_.insert(records_.end(), rec, rec + SMC_EV_STRIDE);
+      ++event;
+    }
+  }
+}

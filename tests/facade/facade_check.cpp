@@ -1,0 +1,61 @@
+// facade_check -- drives the reference-interface facade for the tests.
+//   facade_check reader <params-file> [key=value ...]   parse + print names/values (no GPU)
+//   facade_check mcnucl <params-file> b0 b1 ...         one event per b through MCnucl
+//   facade_check kln <ecm> <lambda> ta0 tb0 ta1 tb1 ... KLNModel::getdNdy per point
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "KLNModel.h"
+#include "MCnucl.h"
+#include "ParameterReader.h"
+
+int main(int argc, char* argv[]) {
+  if (argc < 3) return 2;
+  const char* mode = argv[1];
+  if (!std::strcmp(mode, "reader")) {
+    ParameterReader r;
+    r.readFromFile(argv[2]);
+    r.readFromArguments(argc, argv, "#", 3);
+    const char* keys[] = {"which_mc_model", "ecm", "maxx", "gauss_nucl_width", "aproj", "finalfactor"};
+    for (const char* k : keys) std::printf("%s %d %.17g\n", k, r.exist(k) ? 1 : 0, r.exist(k) ? r.getVal(k) : 0.0);
+    std::printf("missing %d\n", r.exist("no_such_key") ? 1 : 0);
+    r.getVal("no_such_key");  // exits with -1 and the reference's message
+    return 0;
+  }
+  if (!std::strcmp(mode, "mcnucl")) {
+    ParameterReader r;
+    r.readFromFile(argv[2]);
+    MCnucl mc(&r);
+    for (int i = 3; i < argc; ++i) {
+      const double b = std::atof(argv[i]);
+      mc.generateNuclei(b);
+      const int nc = mc.getBinaryCollision();
+      const int cut = mc.CentralityCut();
+      double ta = 0, tb = 0, rho = 0;
+      if (nc > 0) {
+        mc.calculateThickness();
+        mc.setDensity(0, -1);
+        for (int x = 0; x < 261; ++x)
+          for (int y = 0; y < 261; ++y) {
+            ta += mc.getTA1(x, y);
+            tb += mc.getTA2(x, y);
+            rho += mc.getRho(0, x, y);
+          }
+      }
+      std::printf("%d %d %d %d %d %.17g %.17g %.17g %.17g\n", i - 3, mc.getNpart1(), mc.getNpart2(), nc, cut,
+                  nc > 0 ? mc.getdNdy() : 0.0, ta, tb, rho);
+      mc.deleteNucleus();
+    }
+    return 0;
+  }
+  if (!std::strcmp(mode, "kln")) {
+    KLNModel k(std::atof(argv[2]), 7);
+    k.setLambdaSaturation(std::atof(argv[3]));
+    for (int i = 4; i + 1 < argc; i += 2)
+      std::printf("%.17g\n", k.getdNdy(0.0, std::atof(argv[i]), std::atof(argv[i + 1])));
+    return 0;
+  }
+  return 2;
+}

@@ -1,0 +1,228 @@
+"""The reference-interface C++ facade (supermc_amd/host): ParameterReader,
+MCnucl, MakeDensity, KLNModel and the superMC.e driver.
+
+CPU tests: parameter parsing semantics (ParameterReader.cpp:42-169) and the
+fatal paths (missing key -> exit -1, unsupported model -> exit 1) that stop
+before any device work.
+GPU tests: the driver's output files (MakeDensity.cpp:2432-2500 formats)
+against the C-ABI records of the same Philox events, the MCnucl per-event
+interface against the batched path, and KLNModel::getdNdy against the
+oracle's quadrature.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "supermc_amd", "bin", "superMC.e")
+CHECK = os.path.join(ROOT, "tests", "facade", "facade_check.bin")
+
+# MC-Glauber Pb+Pb (the bench workload) in parameters.dat form
+PARAMS = """# test parameters
+which_mc_model = 5   # MC-Glauber
+sub_model = 1
+lambda = 0.138
+tmax = 71
+tmax_subdivision = 3
+alpha = 0.14
+Aproj = 208
+Atarg = 208
+proj_deformed = 0
+targ_deformed = 0
+shape_of_nucleons = 4
+collision_criterion = 1
+shape_of_entropy = 2
+quark_width = 0.3
+gauss_nucl_width = 0.5
+ecm = 2760.
+bmin = 0.
+bmax = 20.
+Npmin = 2
+Npmax = 500
+cutdSdy = 0
+nev = 20
+randomSeed = 1234
+operation = 9
+finalFactor = 1.0
+ecc_from_order = 2
+ecc_to_order = 5
+use_sd = 1
+use_ed = 0
+use_4col = 0
+use_block = 1
+maxx = 13.
+maxy = 13.
+dx = 0.1
+dy = 0.1
+cc_fluctuation_model = 6
+cc_fluctuation_Gamma_theta = 1.25
+sigma_nn_mb = 64.0
+proj_ws_R = 6.62
+proj_ws_a = 0.546
+targ_ws_R = 6.62
+targ_ws_a = 0.546
+nucleon_min_dist = 0.4
+gpu_batch_events = 7
+"""
+
+
+def _need(path):
+    if not os.path.exists(path):
+        pytest.fail(f"{path} not built: run python -m supermc_amd.build")
+
+
+def _write_params(d, text=PARAMS, **over):
+    lines = []
+    for line in text.splitlines():
+        key = line.split("=")[0].strip().lower() if "=" in line.split("#")[0] else None
+        if key is not None and key in {k.lower() for k in over}:
+            continue
+        lines.append(line)
+    for k, v in over.items():
+        lines.append(f"{k} = {v}")
+    p = os.path.join(d, "parameters.dat")
+    open(p, "w").write("\n".join(lines) + "\n")
+    os.makedirs(os.path.join(d, "data"), exist_ok=True)
+    return p
+
+
+def test_parameter_reader_semantics(tmp_path):
+    _need(CHECK)
+    p = tmp_path / "p.dat"
+    p.write_text("# header\n  Which_MC_Model = 1   # comment = 5\nECM=5020. \n\nmaxx = 15.  \n"
+                 "gauss_nucl_width = 0.812 # width\naproj = 208\n")
+    r = subprocess.run([CHECK, "reader", str(p), "ecm = 2760", "MAXX=10 # x"], capture_output=True, text=True)
+    out = dict((ln.split()[0], ln.split()[1:]) for ln in r.stdout.splitlines() if ln.strip())
+    assert out["which_mc_model"] == ["1", "1"]
+    assert float(out["ecm"][1]) == 2760.0           # arguments override the file
+    assert float(out["maxx"][1]) == 10.0            # names are case-insensitive
+    assert float(out["gauss_nucl_width"][1]) == 0.812
+    assert out["finalfactor"][0] == "0"
+    assert out["missing"] == ["0"]
+    # getVal on a missing name: message and exit(-1) (ParameterReader.cpp:149-156)
+    assert r.returncode == 255
+    assert "parameter with name no_such_key not found" in r.stdout
+
+
+def test_driver_missing_key_exits_like_reference(tmp_path):
+    _need(EXE)
+    _write_params(tmp_path, PARAMS.replace("Npmax = 500\n", ""))
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 255
+    assert "parameter with name Npmax not found" in r.stdout
+
+
+def test_driver_rejects_model_outside_path(tmp_path):
+    _need(EXE)
+    _write_params(tmp_path, which_mc_model=3)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "which_mc_model must be 1, 5 or 7" in r.stderr
+
+
+def _read_table(path):
+    return np.array([[float(v) for v in ln.split()] for ln in open(path) if ln.strip() and not ln.startswith("#")])
+
+
+@pytest.mark.gpu
+def test_driver_ecc_table_matches_c_abi(tmp_path):
+    """operation 9: data/sn_ecc_eccp_<n>.dat and _10.dat carry the records of
+    events 0..nev-1 of stream randomSeed, in order, whatever the batch size."""
+    _need(EXE)
+    _write_params(tmp_path)
+    r = subprocess.run([EXE, "nev=20"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=20)
+    with Context(p) as ctx:
+        ev = ctx.run(seed=1234, event0=0, nev=20)
+    for n in range(2, 6):
+        t = _read_table(tmp_path / "data" / f"sn_ecc_eccp_{n}.dat")
+        assert t.shape == (20, 9)
+        np.testing.assert_allclose(t[:, 0], ev[:, n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_allclose(t[:, 1], ev[:, 10 + n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_allclose(t[:, 2], ev[:, 20 + n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_allclose(t[:, 3], ev[:, 30 + n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_allclose(t[:, 4], ev[:, 40 + n], rtol=6e-8)
+        np.testing.assert_array_equal(t[:, 5], ev[:, 54])
+        np.testing.assert_array_equal(t[:, 6], ev[:, 55])
+        np.testing.assert_allclose(t[:, 7], ev[:, 53], rtol=6e-8)
+        np.testing.assert_allclose(t[:, 8], ev[:, 56], rtol=6e-8)
+    t10 = _read_table(tmp_path / "data" / "sn_ecc_eccp_10.dat")
+    assert t10.shape == (20, 9 * 5 + 4)
+    assert (t10[:, 0:5] == 0).all()  # order 1 outside [from, to] stays 0 (MakeDensity.cpp:2266-2270)
+    np.testing.assert_allclose(t10[:, 5 + 0], ev[:, 2], rtol=6e-8, atol=1e-9)
+    np.testing.assert_array_equal(t10[:, 45], ev[:, 54])
+
+
+@pytest.mark.gpu
+def test_driver_profiles_match_c_abi(tmp_path):
+    """operation 1: data/sd_event_<i>_block.dat = finalFactor * s."""
+    _need(EXE)
+    _write_params(tmp_path, operation=1, nev=3, finalFactor=2.5, use_4col=1)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=3, finalFactor=2.5)
+    with Context(p) as ctx:
+        ev = ctx.run(seed=1234, event0=0, nev=3)
+        s, _, _ = ctx.density()
+    for e in range(3):
+        blk = _read_table(tmp_path / "data" / f"sd_event_{e + 1}_block.dat")
+        assert blk.shape == (261, 261)
+        np.testing.assert_allclose(blk, 2.5 * s[e].astype(np.float64), rtol=1e-11, atol=1e-300)
+        col = _read_table(tmp_path / "data" / f"sd_event_{e + 1}_4col.dat")
+        assert col.shape == (261 * 261, 4)
+        np.testing.assert_allclose(col[:, 3], 2.5 * s[e].ravel().astype(np.float64), rtol=1e-11)
+        assert col[1, 2] == pytest.approx(-12.9) and col[261, 1] == pytest.approx(-12.9)
+        header = open(tmp_path / "data" / f"sd_event_{e + 1}_4col.dat").readline()
+        assert header.split()[:2] == ["#", "<npart>="] and float(header.split()[2]) == ev[e, 54]
+
+
+@pytest.mark.gpu
+def test_mcnucl_facade_matches_batched_path(tmp_path):
+    _need(CHECK)
+    pfile = _write_params(tmp_path)
+    bs = [0.0, 3.0, 7.5, 11.0, 19.0]
+    r = subprocess.run([CHECK, "mcnucl", pfile, *map(str, bs)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    rows = [ln.split() for ln in r.stdout.splitlines() if ln.strip()]
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=1)
+    with Context(p) as ctx:
+        for i, b in enumerate(bs):
+            # MCnucl::generateNuclei(b) is event i of stream randomSeed
+            ctx.sample_b(seed=1234, event0=i, b=np.array([b]))
+            npa, npb, nc, _ = ctx.collide(allow=(4,))
+            row = rows[i]
+            assert int(row[1]) == npa[0] and int(row[2]) == npb[0] and int(row[3]) == nc[0]
+            assert int(row[4]) == int(2 <= npa[0] + npb[0] <= 500)
+            if nc[0] == 0:
+                continue
+            ta, tb = ctx.thickness()
+            s, dndy, _ = ctx.density()
+            assert float(row[5]) == pytest.approx(dndy[0] * 0.01, rel=1e-12)
+            assert float(row[6]) == pytest.approx(ta[0].astype(np.float64).sum(), rel=1e-9)
+            assert float(row[7]) == pytest.approx(tb[0].astype(np.float64).sum(), rel=1e-9)
+            assert float(row[8]) == pytest.approx(s[0].astype(np.float64).sum(), rel=1e-9)
+    assert int(rows[-1][3]) == 0  # b = 19 fm: no collision
+
+
+@pytest.mark.gpu
+def test_klnmodel_facade_matches_oracle():
+    _need(CHECK)
+    import oracle
+    pts = [(0.5, 0.5), (1.3, 0.4), (3.0, 7.0), (0.0, 1.0)]
+    args = [v for pt in pts for v in map(str, pt)]
+    r = subprocess.run([CHECK, "kln", "2760", "0.138", *args], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = [float(v) for v in r.stdout.split()]
+    k = oracle.kln(2760.0, 0.138)
+    for (ta, tb), g in zip(pts, got):
+        ref = oracle.kln_dndy_gl(k, ta, tb, 32, 32, 48)
+        assert g == pytest.approx(ref, rel=1e-11, abs=0.0)
