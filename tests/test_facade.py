@@ -153,7 +153,9 @@ def test_driver_ecc_table_matches_c_abi(tmp_path):
         np.testing.assert_allclose(t[:, 8], ev[:, 56], rtol=6e-8)
     t10 = _read_table(tmp_path / "data" / "sn_ecc_eccp_10.dat")
     assert t10.shape == (20, 9 * 5 + 4)
-    assert (t10[:, 0:5] == 0).all()  # order 1 outside [from, to] stays 0 (MakeDensity.cpp:2266-2270)
+    # order 1 is outside [from, to]: its moments stay 0 (MakeDensity.cpp:2266-2270), <r> is written
+    assert (t10[:, 0:4] == 0).all()
+    np.testing.assert_allclose(t10[:, 4], ev[:, 41], rtol=6e-8)
     np.testing.assert_allclose(t10[:, 5 + 0], ev[:, 2], rtol=6e-8, atol=1e-9)
     np.testing.assert_array_equal(t10[:, 45], ev[:, 54])
 
