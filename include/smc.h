@@ -195,11 +195,38 @@ smc_status smc_kln_dndy(smc_ctx* ctx, int n, const double* ta, const double* tb,
  * [tmax][tmax] if non-NULL). */
 smc_status smc_kln_make_table(smc_ctx* ctx, double* table);
 
+/* Differentiable batches at frozen collision topology.  No counterpart in
+ * the reference (it has no derivative code); for calibration callers.
+ * theta = {w, R, a, alpha, k}: Gaussian nucleon width [fm], Woods-Saxon
+ * radius and diffuseness of both nuclei [fm], two-component alpha, Gamma
+ * k = 1/cc_fluctuation_Gamma_theta.  Requires which_mc_model 5, spherical
+ * Woods-Saxon nuclei with equal R and a, cc_fluctuation_model 0 or 6.
+ *
+ * smc_diff_setup draws nev events at theta0 (the ctx's parameters, returned
+ * in theta0[5] if non-NULL) with the acceptance of MakeDensity.cpp:2147-2162
+ * and freezes their uniforms, impact parameters, participants, pairs,
+ * footprint windows and hot-spot masks.  Positions are r = F^-1(u; R, a) of
+ * the Woods-Saxon radial CDF, Gamma factors the quantiles of fixed uniforms.
+ * smc_diff_forward evaluates the batch at theta; out [nev][8] (optional):
+ * |eps_2|, |eps_3| (r^2 weight), dS/dy, Re eps_2, Im eps_2, Re eps_3,
+ * Im eps_3, status.  smc_diff_backward takes cot [nev][3] = dL/d(|eps_2|,
+ * |eps_3|, dS/dy) per event and returns grad[5] = dL/dtheta at the last
+ * forward's theta.  smc_diff_topology exports event ev's frozen uniforms:
+ * uvw [A][3] = (u, cos theta, phi) per nucleon in collision order, b, and
+ * the Gamma uniform of each of its nsrc sources (participants in wound
+ * order, then pairs). */
+smc_status smc_diff_setup(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* theta0);
+smc_status smc_diff_forward(smc_ctx* ctx, const double* theta, double* out);
+smc_status smc_diff_backward(smc_ctx* ctx, const double* cot, double* grad);
+smc_status smc_diff_topology(smc_ctx* ctx, int ev, double* uvw_proj, double* uvw_targ, double* b,
+                             double* gamma_v, int cap, int* nsrc);
+
 /* Timing of the ctx's kernels with HIP events on its stream. */
 #define SMC_K_EVENT 0
 #define SMC_K_SMEAR 1
 #define SMC_K_MOMENTS 2
 #define SMC_K_KLN 3
+#define SMC_K_ADJOINT 4
 #define SMC_K_COUNT 8
 smc_status smc_set_timing(smc_ctx* ctx, int on);
 smc_status smc_get_timing(smc_ctx* ctx, int kernel, double* total_ms, int* launches);

@@ -78,6 +78,7 @@ def lib():
             "orc_kln_dndy_gl": (C.c_double, [C.POINTER(OrcKln), C.c_double, C.c_double, C.c_int, C.c_int, C.c_int]),
             "orc_gauss_legendre01": (None, [C.c_int, _dp, _dp]),
             "orc_six_point": (C.c_double, [C.c_double] * 8),
+            "orc_hotspot_mask": (C.c_int, [C.POINTER(OrcGrid), C.c_int, _dp, C.c_int, C.POINTER(C.c_ubyte)]),
             "orc_density_kln_table": (C.c_double, [C.POINTER(OrcGrid), _dp, _dp, _dp, C.c_int, C.c_double, _dp]),
             "orc_rng_seed": (None, [C.POINTER(OrcRng), C.c_uint64]),
             "orc_rng_uniform": (C.c_double, [C.POINTER(OrcRng)]),
@@ -227,6 +228,13 @@ def moments(g, density, boxes, ncoll, from_order, to_order):
     out["xc"], out["yc"], out["total"] = cm
     out["rc"] = rc
     return out
+
+
+def hotspot_mask(g, boxes, ncoll):
+    boxes = np.ascontiguousarray(boxes, np.float64).reshape(-1, 4)
+    m = np.zeros((g.nx, g.ny), np.uint8)
+    lib().orc_hotspot_mask(C.byref(g), len(boxes), _p(boxes), ncoll, m.ctypes.data_as(C.POINTER(C.c_ubyte)))
+    return m.astype(bool)
 
 
 def kln(ecm=2760.0, lam=0.138):

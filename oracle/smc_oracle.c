@@ -329,6 +329,40 @@ double orc_density_trento(const orc_grid* g, double w,
   return dndy;
 }
 
+int orc_hotspot_mask(const orc_grid* g, int nbox, const double* boxes, int ncoll, unsigned char* mask)
+{
+  /* The cells orc_moments weights for eps_n (MCnucl.cpp:1303-1323,
+   * MakeDensity.cpp:2341-2423), as a [nx][ny] 0/1 grid. */
+  int nhot = nbox + ncoll;
+  memset(mask, 0, (size_t)g->nx * g->ny);
+  if (nhot <= 0) return -1;
+  double region[4];
+  if (nbox > 0) memcpy(region, boxes, sizeof(region));
+  else region[0] = region[1] = region[2] = region[3] = 0.0;
+  for (int h = 1; h < nhot; h++) {
+    double zero[4] = {0, 0, 0, 0};
+    const double* o = (h < nbox) ? boxes + 4 * h : zero;
+    region[0] = dmin_(o[0], region[0]);
+    region[1] = dmax_(o[1], region[1]);
+    region[2] = dmin_(o[2], region[2]);
+    region[3] = dmax_(o[3], region[3]);
+  }
+  double dx = g->dx, dy = g->dy;
+  int nYpoints = (int)((region[3] - region[2]) / dy);
+  for (int h = 0; h < nhot; h++) {
+    double zero[4] = {0, 0, 0, 0};
+    const double* hs = (h < nbox) ? boxes + 4 * h : zero;
+    for (int x_i = (int)((hs[0] - region[0]) / dx); x_i < (int)((hs[1] - region[0]) / dx); x_i++)
+      for (int y_i = (int)((hs[2] - region[2]) / dy); y_i < (int)((hs[3] - region[2]) / dx); y_i++) {
+        if (y_i >= nYpoints) continue;
+        int i = x_i + (int)((region[0] - g->xmin) / dx);
+        int j = y_i + (int)((region[2] - g->ymin) / dy);
+        if (i < g->nx && i >= 0 && j < g->ny && j >= 0) mask[(size_t)i * g->ny + j] = 1;
+      }
+  }
+  return 0;
+}
+
 int orc_moments(const orc_grid* g, const double* density,
                 int nbox, const double* boxes, int ncoll,
                 int from_order, int to_order,

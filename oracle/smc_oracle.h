@@ -96,6 +96,9 @@ int orc_moments(const orc_grid* g, const double* density,
                 double* mom_re, double* mom_im, double* momp_re,
                 double* momp_im, double* rn, double* cm);
 
+/* cells of the eps_n hot-spot mask as a 0/1 [nx][ny] grid */
+int orc_hotspot_mask(const orc_grid* g, int nbox, const double* boxes, int ncoll, unsigned char* mask);
+
 /* ---- MC-KLN (which_mc_model 1, sub_model 7) ---- */
 typedef struct {
   double ecm;      /* sqrt(s) [GeV] */

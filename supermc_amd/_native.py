@@ -92,6 +92,10 @@ SIGNATURES = {
     "smc_fetch_events": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "smc_kln_dndy": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "smc_kln_make_table": (C.c_int, [C.c_void_p, _dp]),
+    "smc_diff_setup": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
+    "smc_diff_forward": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "smc_diff_backward": (C.c_int, [C.c_void_p, _dp, _dp]),
+    "smc_diff_topology": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip]),
     "smc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "smc_get_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
     "smc_synchronize": (C.c_int, [C.c_void_p]),
@@ -270,6 +274,39 @@ class Context:
         t = np.zeros((n, n))
         self._check(self.lib.smc_kln_make_table(self.h, _ptr(t, C.c_double)))
         return t
+
+    # differentiable batches at frozen topology: theta = (w, R, a, alpha, k)
+    def diff_setup(self, seed, event0, nev):
+        th = np.zeros(5)
+        self._check(self.lib.smc_diff_setup(self.h, seed, event0, nev, _ptr(th, C.c_double)))
+        self.nev = nev
+        self.diff_nev = nev
+        return th
+
+    def diff_forward(self, theta):
+        th = np.ascontiguousarray(theta, np.float64)
+        out = np.zeros((self.diff_nev, 8))
+        self._check(self.lib.smc_diff_forward(self.h, _ptr(th, C.c_double), _ptr(out, C.c_double)))
+        self._diff_theta = th.copy()
+        return out
+
+    def diff_backward(self, cot):
+        cot = np.ascontiguousarray(cot, np.float64).reshape(self.diff_nev, 3)
+        g = np.zeros(5)
+        self._check(self.lib.smc_diff_backward(self.h, _ptr(cot, C.c_double), _ptr(g, C.c_double)))
+        return g
+
+    def diff_topology(self, ev):
+        p = self.params
+        up = np.zeros((p.Aproj, 3))
+        ut = np.zeros((p.Atarg, 3))
+        b = C.c_double()
+        cap = 2 * 256 + (p.max_pairs if p.max_pairs > 0 else min(p.Aproj * p.Atarg, 8192))
+        gv = np.zeros(cap)
+        n = C.c_int()
+        self._check(self.lib.smc_diff_topology(self.h, ev, _ptr(up, C.c_double), _ptr(ut, C.c_double), C.byref(b),
+                                               _ptr(gv, C.c_double), cap, C.byref(n)))
+        return dict(uvw_proj=up, uvw_targ=ut, b=b.value, gamma_v=gv[:n.value])
 
     def set_timing(self, on=True):
         self._check(self.lib.smc_set_timing(self.h, 1 if on else 0))

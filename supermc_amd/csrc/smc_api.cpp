@@ -129,6 +129,14 @@ struct smc_ctx {
   void* kln_nodes_d = nullptr;
   int kln_npt = 0, kln_nn = 0;
   KlnConst kln_kc{};
+  // differentiable batches
+  DiffBufs dif{};
+  void* dif_alloc[16] = {};
+  int dif_nev = 0, dif_fwd = 0;
+  uint64_t dif_seed = 0, dif_event0 = 0;
+  double dif_theta0[5] = {}, dif_theta[5] = {};
+  double* dif_out = nullptr;
+  double* dif_cot = nullptr;
   // timing
   int timing = 0;
   struct Pending { int k; hipEvent_t a, b; };
@@ -603,6 +611,174 @@ void run_event(smc_ctx* c, int nev, uint64_t seed, uint64_t event0, int fed) {
   c->have_density = c->have_thickness = c->have_moments = 0;
 }
 
+// ---------------------------------------------------------------- differentiable batches
+void diff_check_params(const smc_ctx* c) {
+  const smc_params& p = c->p;
+  const smc_derived& d = c->d;
+  if (p.which_mc_model != 5) throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need which_mc_model 5");
+  for (int nu = 0; nu < 2; ++nu)
+    if (c->cfg.nuc[nu].kind != kWoodsSaxon || c->cfg.nuc[nu].deformed)
+      throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need spherical Woods-Saxon nuclei");
+  if (d.ws_R[0] != d.ws_R[1] || d.ws_a[0] != d.ws_a[1])
+    throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches share R and a between the nuclei");
+  if (!(p.cc_fluctuation_model == 0 || p.cc_fluctuation_model == 6))
+    throw SmcError(SMC_ERR_UNSUPPORTED, "cc_fluctuation_model must be 0 or 6");
+}
+
+template <class T>
+T* dif_alloc(smc_ctx* c, int slot, size_t n) {
+  if (c->dif_alloc[slot]) (void)hipFree(c->dif_alloc[slot]);
+  T* ptr = dalloc<T>(n);
+  c->dif_alloc[slot] = ptr;
+  return ptr;
+}
+
+void diff_setup(smc_ctx* c, uint64_t seed, uint64_t event0, int nev) {
+  diff_check_params(c);
+  const size_t E = (size_t)nev, SC = (size_t)c->buf.src_cap;
+  const int nwy = (c->d.ny + 31) / 32;
+  DiffBufs& d = c->dif;
+  d.pending = dif_alloc<int>(c, 0, E);
+  d.attempt = dif_alloc<int>(c, 1, E);
+  d.topo_ok = dif_alloc<int>(c, 2, E);
+  d.bval = dif_alloc<double>(c, 3, E);
+  d.du = dif_alloc<double>(c, 4, E * 2 * kAMax * 4);
+  d.dpos = dif_alloc<double>(c, 5, E * 2 * kAMax * 6);
+  d.dsrc = dif_alloc<double2>(c, 6, E * SC);
+  d.gam_v = dif_alloc<double>(c, 7, E * SC);
+  d.dsc = dif_alloc<double>(c, 8, E * kDsStride);
+  d.mask = dif_alloc<unsigned>(c, 9, E * c->d.nx * nwy);
+  d.gsrc = dif_alloc<double4>(c, 10, E * SC);
+  d.gev = dif_alloc<double>(c, 11, E * 5);
+  std::vector<double> x, w;
+  gauss_legendre01(64, x, w);
+  std::vector<double> glh(128);
+  for (int i = 0; i < 64; ++i) {
+    glh[2 * i] = x[i];
+    glh[2 * i + 1] = w[i];
+  }
+  double* gl = dif_alloc<double>(c, 12, 128);
+  HIPCHK(hipMemcpy(gl, glh.data(), glh.size() * sizeof(double), hipMemcpyHostToDevice));
+  d.gl = reinterpret_cast<const double2*>(gl);
+  d.ngl = 64;
+  if (c->dif_out) (void)hipFree(c->dif_out);
+  if (c->dif_cot) (void)hipFree(c->dif_cot);
+  c->dif_out = dalloc<double>(E * kDiffOut);
+  c->dif_cot = dalloc<double>(E * 3);
+  HIPCHK(hipMemsetAsync(d.gam_v, 0, E * SC * sizeof(double), c->stream));
+  // theta0 from the context's parameters
+  c->dif_theta0[0] = c->d.width;
+  c->dif_theta0[1] = c->d.ws_R[0];
+  c->dif_theta0[2] = c->d.ws_a[0];
+  c->dif_theta0[3] = c->p.alpha;
+  c->dif_theta0[4] = 1.0 / c->p.cc_fluctuation_Gamma_theta;
+  // draw topologies until every event passes MakeDensity.cpp:2147-2162
+  std::vector<int> pend(E, 1), att(E, 0), cnt(E * 8), tok(E);
+  int left = nev;
+  for (int round = 0; left > 0; ++round) {
+    if (round >= 200) throw SmcError(SMC_ERR_NO_EVENT, "differentiable batch: events not accepted after 200 rounds");
+    HIPCHK(hipMemcpyAsync(d.pending, pend.data(), E * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d.attempt, att.data(), E * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    check_launch(launch_diff_topo(c->cfg, c->buf, d, nev, seed, event0, c->dif_theta0[1], c->dif_theta0[2],
+                                  c->stream),
+                 "k_diff_topo");
+    check_launch(launch_event(c->cfg, c->buf, nev, seed, event0, 1, c->stream), "k_event");
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->buf.counts, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(tok.data(), d.topo_ok, E * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    left = 0;
+    for (size_t e = 0; e < E; ++e) {
+      if (!pend[e]) continue;
+      const int np = cnt[e * 8] + cnt[e * 8 + 1], nc = cnt[e * 8 + 2], st = cnt[e * 8 + 3];
+      const bool ok = tok[e] && st == 0 && nc > 0 && np >= c->p.Npmin && np <= c->p.Npmax;
+      if (ok) {
+        pend[e] = 0;
+      } else {
+        ++att[e];
+        ++left;
+      }
+    }
+  }
+  // the Gamma uniforms (and the theta0 sources) are part of the topology
+  check_launch(launch_diff_forward_pre(c->cfg, c->buf, d, nev, seed, event0, c->dif_theta0,
+                                       c->p.cc_fluctuation_model == 6, c->stream),
+               "k_diff_pos/src");
+  sync(c);
+  c->nev = nev;
+  c->have_events = 1;
+  c->have_density = c->have_thickness = c->have_moments = 0;
+  c->dif_nev = nev;
+  c->dif_seed = seed;
+  c->dif_event0 = event0;
+  c->dif_fwd = 0;
+}
+
+// The grid geometry of the context with the Gaussian at width th[0]; the
+// footprint is the frozen integer window alone (no disk cut), so the
+// density is smooth in theta.
+DevCfg diff_cfg(const smc_ctx* c, const double* th) {
+  DevCfg cw = c->cfg;
+  cw.inv2w2 = (float)(1.0 / (2.0 * th[0] * th[0]));
+  cw.norm = (float)(1.0 / (2.0 * M_PI * th[0] * th[0]));
+  cw.cut2 = 1e30f;
+  return cw;
+}
+
+SmearSpec diff_spec(const smc_ctx* c, const double* th) {
+  SmearSpec s = density_spec(c);
+  const double norm = 1.0 / (2.0 * M_PI * th[0] * th[0]);
+  s.scale[kSrcPartA] = s.scale[kSrcPartB] = (float)(0.5 * (1.0 - th[3]) * norm);
+  s.scale[kSrcPair] = (float)(th[3] * norm);
+  s.enabled[kSrcPair] = th[3] > 1e-8;
+  s.use_fluct[kSrcPartA] = s.use_fluct[kSrcPartB] = s.use_fluct[kSrcPair] = 1;
+  return s;
+}
+
+void diff_forward(smc_ctx* c, const double* th) {
+  if (!c->dif_nev) throw SmcError(SMC_ERR_STATE, "call smc_diff_setup first");
+  if (!(th[0] > 0.0 && th[1] > 0.0 && th[2] > 0.0 && th[3] >= 0.0 && th[3] < 1.0 && th[4] > 0.0))
+    throw SmcError(SMC_ERR_ARG, "theta outside w, R, a, k > 0, 0 <= alpha < 1");
+  const int nev = c->dif_nev;
+  std::copy(th, th + 5, c->dif_theta);
+  const int fluct = c->p.cc_fluctuation_model == 6;
+  TimerScope t0(c, SMC_K_EVENT);
+  check_launch(launch_diff_forward_pre(c->cfg, c->buf, c->dif, nev, c->dif_seed, c->dif_event0, th, fluct, c->stream),
+               "k_diff_pos/src");
+  t0.done();
+  float* grid = grid_buffer(c, &c->grid);
+  TimerScope t1(c, SMC_K_SMEAR);
+  check_launch(launch_tile_exact(diff_cfg(c, th), c->buf, diff_spec(c, th), nev, grid, nullptr, c->stream),
+               "k_tile_exact");
+  t1.done();
+  TimerScope t2(c, SMC_K_MOMENTS);
+  const double ffdxdy = c->p.finalFactor * c->p.dx * c->p.dy;
+  check_launch(launch_diff_moments(c->cfg, c->buf, c->dif, grid, nev, ffdxdy, c->dif_out, c->stream),
+               "k_diff_moments");
+  t2.done();
+  c->have_density = 1;
+  c->dif_fwd = 1;
+}
+
+void diff_backward(smc_ctx* c, const double* cot, double* grad) {
+  if (!c->dif_fwd) throw SmcError(SMC_ERR_STATE, "call smc_diff_forward first");
+  const int nev = c->dif_nev;
+  HIPCHK(hipMemcpyAsync(c->dif_cot, cot, (size_t)nev * 3 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  float* G = grid_buffer(c, &c->tmp);
+  const double ffdxdy = c->p.finalFactor * c->p.dx * c->p.dy;
+  TimerScope ts(c, SMC_K_ADJOINT);
+  check_launch(launch_diff_backward(diff_cfg(c, c->dif_theta), c->buf, diff_spec(c, c->dif_theta), c->dif, nev,
+                                    c->dif_cot, ffdxdy, G,
+                                    c->dif_theta, c->stream),
+               "k_diff_cot/gather/chain");
+  ts.done();
+  std::vector<double> gev((size_t)nev * 5);
+  HIPCHK(hipMemcpyAsync(gev.data(), c->dif.gev, gev.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  for (int q = 0; q < 5; ++q) grad[q] = 0.0;
+  for (int e = 0; e < nev; ++e)
+    for (int q = 0; q < 5; ++q) grad[q] += gev[(size_t)e * 5 + q];
+}
+
 }  // namespace
 
 extern "C" {
@@ -692,6 +868,10 @@ void smc_destroy(smc_ctx* c) {
                   c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  for (void* q : c->dif_alloc)
+    if (q) (void)hipFree(q);
+  if (c->dif_out) (void)hipFree(c->dif_out);
+  if (c->dif_cot) (void)hipFree(c->dif_cot);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -940,6 +1120,71 @@ smc_status smc_kln_make_table(smc_ctx* c, double* table) {
       const size_t n = (size_t)c->d.kln_tmax * c->d.kln_tmax;
       HIPCHK(hipMemcpyAsync(table, c->kln_table, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
       sync(c);
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_diff_setup(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* theta0) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    diff_setup(c, seed, event0, nev);
+    if (theta0) std::copy(c->dif_theta0, c->dif_theta0 + 5, theta0);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_diff_forward(smc_ctx* c, const double* theta, double* out) {
+  if (!c || !theta) return SMC_ERR_ARG;
+  try {
+    diff_forward(c, theta);
+    if (out)
+      HIPCHK(hipMemcpyAsync(out, c->dif_out, (size_t)c->dif_nev * kDiffOut * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_diff_backward(smc_ctx* c, const double* cot, double* grad) {
+  if (!c || !cot || !grad) return SMC_ERR_ARG;
+  try {
+    diff_backward(c, cot, grad);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_diff_topology(smc_ctx* c, int ev, double* uvw_proj, double* uvw_targ, double* b, double* gamma_v,
+                             int cap, int* nsrc) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->dif_nev) throw SmcError(SMC_ERR_STATE, "call smc_diff_setup first");
+    if (ev < 0 || ev >= c->dif_nev) throw SmcError(SMC_ERR_ARG, "event index out of range");
+    std::vector<double> du(2 * kAMax * 4);
+    HIPCHK(hipMemcpyAsync(du.data(), c->dif.du + (size_t)ev * 2 * kAMax * 4, du.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    int cnt[8];
+    HIPCHK(hipMemcpyAsync(cnt, c->buf.counts + (size_t)ev * 8, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    double bb = 0.0;
+    HIPCHK(hipMemcpyAsync(&bb, c->dif.bval + ev, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    std::vector<double> gv((size_t)c->buf.src_cap);
+    HIPCHK(hipMemcpyAsync(gv.data(), c->dif.gam_v + (size_t)ev * c->buf.src_cap, gv.size() * sizeof(double),
+                          hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    for (int nu = 0; nu < 2; ++nu) {
+      double* o = nu ? uvw_targ : uvw_proj;
+      if (!o) continue;
+      const int A = nu ? c->p.Atarg : c->p.Aproj;
+      for (int i = 0; i < A; ++i)
+        for (int q = 0; q < 3; ++q) o[3 * i + q] = du[((size_t)nu * kAMax + i) * 4 + q];
+    }
+    if (b) *b = bb;
+    const int K = cnt[5];
+    if (nsrc) *nsrc = K;
+    if (gamma_v) {
+      if (cap < K) throw SmcError(SMC_ERR_CAPACITY, "gamma_v capacity below the source count");
+      for (int k = 0; k < K; ++k) gamma_v[k] = gv[k];
     }
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }

@@ -31,6 +31,9 @@ enum Stream : unsigned {
   kStrFluctB = 5,
   kStrFluctPair = 6,
   kStrDeuteron = 7,
+  kStrDiffB = 8,        // differentiable batches: impact parameter
+  kStrDiffNucleon = 9,  //   nucleon candidates (radial quantile, direction)
+  kStrDiffGamma = 10,   //   Gamma quantile uniforms
 };
 
 __device__ __forceinline__ uint4 draw(uint2 key, unsigned long long ev, unsigned stream,

@@ -1,0 +1,526 @@
+// smc_diff.hip -- differentiable MC-Glauber batches at frozen collision
+// topology (no counterpart in the reference, which has no derivative code).
+//
+// Parameters theta = (w, R, a, alpha, k): nucleon width, Woods-Saxon radius
+// and diffuseness (both nuclei), two-component mixing, Gamma k = 1/theta.
+// Frozen at theta0: the uniforms of every accepted nucleon, the impact
+// parameter, participants and collision pairs, every source's integer
+// footprint window with its r <= 5 w0 disk cut, and the hot-spot mask.
+//
+//  positions  r = F^{-1}(u; R, a), F the Woods-Saxon radial CDF truncated at
+//             R + 2.5 (the law of Nucleus.cpp:578-621), times a fixed unit
+//             direction; CM subtracted (Nucleus.cpp:296-309); dr/dR, dr/da
+//             by implicit differentiation of F(r) = u.
+//  weights    participant (1-alpha)/2 f, pair alpha f, f = G^{-1}(v; s)/s
+//             with s the Gamma shapes of MCnucl.cpp:1271-1301; df/ds from
+//             dG^{-1}/ds = -(dP/ds)/pdf.
+//  density    the exact-footprint tile kernel with w in the Gaussian and the
+//             frozen windows.
+//  adjoint    cell cotangents G = dL/drho (k_diff_cot), one wavefront per
+//             source gathering G against the recomputed Gaussians
+//             (k_diff_gather), per-event chain rule to theta (k_diff_chain).
+#include "smc_device.h"
+
+namespace smc {
+
+// ---------------------------------------------------------------- math
+// N(r) = int_0^r t^2 rho(t) dt and its R and a derivatives, GL on [0, r].
+struct WsInt {
+  double n, nR, na;
+};
+
+__device__ inline WsInt ws_integrals(double r, double R, double a, const double2* gl, int ngl) {
+  WsInt s{0.0, 0.0, 0.0};
+  for (int j = 0; j < ngl; ++j) {
+    const double t = r * gl[j].x, wt = r * gl[j].y;
+    const double rho = 1.0 / (1.0 + exp((t - R) / a));
+    const double dR = rho * (1.0 - rho) / a;
+    const double t2w = t * t * wt;
+    s.n += t2w * rho;
+    s.nR += t2w * dR;
+    s.na += t2w * dR * (t - R) / a;
+  }
+  return s;
+}
+
+// r = F^{-1}(u; R, a) on [0, R + 2.5] and dr/dR, dr/da.
+__device__ inline void ws_finv(double u, double R, double a, const double2* gl, int ngl, double& r, double& rR,
+                               double& ra) {
+  const double rmax = R + 2.5;
+  const WsInt Z = ws_integrals(rmax, R, a, gl, ngl);
+  const double rhomax = 1.0 / (1.0 + exp(2.5 / a));
+  const double ZR = Z.nR + rmax * rmax * rhomax, Za = Z.na;
+  const double target = u * Z.n;
+  double lo = 0.0, hi = rmax, x = rmax * cbrt(u);
+  for (int it = 0; it < 100; ++it) {
+    const WsInt N = ws_integrals(x, R, a, gl, ngl);
+    const double f = N.n - target;
+    if (f < 0.0) lo = x;
+    else hi = x;
+    const double rho = 1.0 / (1.0 + exp((x - R) / a));
+    double xn = x - f / (x * x * rho);
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    const bool done = fabs(xn - x) <= 1e-14 * rmax;
+    x = xn;
+    if (done) break;
+  }
+  r = x;
+  const WsInt N = ws_integrals(x, R, a, gl, ngl);
+  const double rho = 1.0 / (1.0 + exp((x - R) / a));
+  const double Fr = x * x * rho / Z.n;
+  const double FR = N.nR / Z.n - N.n * ZR / (Z.n * Z.n);
+  const double Fa = N.na / Z.n - N.n * Za / (Z.n * Z.n);
+  rR = -FR / Fr;
+  ra = -Fa / Fr;
+}
+
+// digamma by upward recurrence to z >= 12 and the asymptotic series
+__device__ inline double digamma_d(double z) {
+  double acc = 0.0;
+  while (z < 12.0) {
+    acc -= 1.0 / z;
+    z += 1.0;
+  }
+  const double iz = 1.0 / z, iz2 = iz * iz;
+  return acc + log(z) - 0.5 * iz -
+         iz2 * (1.0 / 12 - iz2 * (1.0 / 120 - iz2 * (1.0 / 252 - iz2 * (1.0 / 240 - iz2 * (1.0 / 132)))));
+}
+
+// Regularised lower incomplete gamma P(s, x) by its power series
+// P = e^{-x} x^s / Gamma(s+1) sum_n x^n / ((s+1)...(s+n)) and, when dPds is
+// requested, dP/ds = P (ln x - psi(s+1)) - prefactor sum_n t_n H_n with
+// H_n = sum_{m<=n} 1/(s+m).
+__device__ inline double gamma_p(double s, double x, double lg1, double psi1, double* dPds) {
+  if (x <= 0.0) {
+    if (dPds) *dPds = 0.0;
+    return 0.0;
+  }
+  const double lx = log(x);
+  const double pref = exp(-x + s * lx - lg1);
+  double t = 1.0, sum = 1.0, H = 0.0, dsum = 0.0;
+  for (int n = 1; n < 2000; ++n) {
+    const double sn = s + n;
+    t *= x / sn;
+    H += 1.0 / sn;
+    sum += t;
+    dsum += t * H;
+    if (t < 1e-17 * sum && n > x) break;
+  }
+  const double P = pref * sum;
+  if (dPds) *dPds = P * (lx - psi1) - pref * dsum;
+  return P;
+}
+
+// Unit-scale Gamma quantile x(v; s) (P(s, x) = v) and dx/ds.
+__device__ inline void gamma_qf(double v, double s, double& x, double& dxds) {
+  const double lg1 = lgamma(s + 1.0), psi1 = digamma_d(s + 1.0), lgs = lg1 - log(s);
+  double lo = 0.0, hi = 1.0;
+  while (gamma_p(s, hi, lg1, psi1, nullptr) < v && hi < 1e3) hi *= 2.0;
+  // small-x form of P as the first guess, inside the bracket
+  double xx = exp((log(v) + lg1) / s);
+  if (!(xx > lo && xx < hi)) xx = 0.5 * (lo + hi);
+  for (int it = 0; it < 200; ++it) {
+    const double f = gamma_p(s, xx, lg1, psi1, nullptr) - v;
+    if (f < 0.0) lo = xx;
+    else hi = xx;
+    const double pdf = exp(-xx + (s - 1.0) * log(xx) - lgs);
+    double xn = xx - f / pdf;
+    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
+    const bool done = fabs(xn - xx) <= 1e-15 * xx;
+    xx = xn;
+    if (done) break;
+  }
+  x = xx;
+  double dP = 0.0;
+  gamma_p(s, x, lg1, psi1, &dP);
+  const double pdf = exp(-x + (s - 1.0) * log(x) - lgs);
+  dxds = -dP / pdf;
+}
+
+// Gamma shape of a source kind (MCnucl.cpp:1271-1301; k = 1/theta)
+__device__ __forceinline__ double diff_shape(int kind, double alpha, double kg) {
+  return (kind == kSrcPair) ? (alpha + 1e-8) * kg : (1.0 - alpha + 1e-8) * 0.5 * kg;
+}
+
+// ------------------------------------------------------------- topology
+// One workgroup per pending event: b, then each nucleus (one lane each)
+// from candidates (u, cos theta, phi) in stream order with the hard core,
+// r from the radial quantile at theta0.  Writes the quantised positions
+// sorted by (x, index) to raw (fed-position input of k_event) and the
+// candidates' uniforms in the same order.
+__global__ __launch_bounds__(64) void k_diff_topo(DevCfg cfg, EvBufs buf, DiffBufs d, uint2 key,
+                                                  unsigned long long event0, double R0, double a0) {
+  const int e = blockIdx.x;
+  const int lane = threadIdx.x;
+  if (!d.pending[e]) return;
+  const unsigned attempt = (unsigned)d.attempt[e];
+  const unsigned long long gev = event0 + (unsigned long long)e;
+  __shared__ double px[2][kAMax], py[2][kAMax], pz[2][kAMax];
+  __shared__ double uu[2][kAMax][3];
+  __shared__ int ok[2];
+  __shared__ int s_xc;
+  __shared__ int ord[2][kAMax], xq[2][kAMax];
+  if (lane == 0) {
+    const uint4 r = draw(key, gev, kStrDiffB, 0, 0, attempt, 0);
+    const double b = sqrt((cfg.bmax2 - cfg.bmin2) * u53(r.x, r.y) + cfg.bmin2);
+    s_xc = (int)llrint(b * 0.5 / kQ);
+    d.bval[e] = 2.0 * (double)s_xc * kQ;
+    buf.bval[e] = 2.0 * (double)s_xc * kQ;
+  }
+  __syncthreads();
+  if (lane < 2) {
+    const int nu = lane;
+    const int A = cfg.nuc[nu].A;
+    int acc = 0;
+    const int max_cand = 64 * A + 64;
+    for (int j = 0; j < max_cand && acc < A; ++j) {
+      const uint4 r1 = draw(key, gev, kStrDiffNucleon, nu, j, attempt, 0);
+      const uint4 r2 = draw(key, gev, kStrDiffNucleon, nu, j, attempt, 1);
+      const double u = u53(r1.x, r1.y);
+      const double c = 2.0 * u53(r2.x, r2.y) - 1.0;
+      const double ph = 2.0 * M_PI * u53(r2.z, r2.w);
+      double r, rR, ra;
+      ws_finv(u, R0, a0, d.gl, d.ngl, r, rR, ra);
+      const double st = sqrt(fmax(0.0, 1.0 - c * c));
+      const double x = r * st * cos(ph), y = r * st * sin(ph), z = r * c;
+      bool good = true;
+      for (int m = 0; m < acc && good; ++m) {
+        const double dx = x - px[nu][m], dy = y - py[nu][m], dz = z - pz[nu][m];
+        if (dx * dx + dy * dy + dz * dz < (double)cfg.dmin2) good = false;
+      }
+      if (!good) continue;
+      px[nu][acc] = x;
+      py[nu][acc] = y;
+      pz[nu][acc] = z;
+      uu[nu][acc][0] = u;
+      uu[nu][acc][1] = c;
+      uu[nu][acc][2] = ph;
+      ++acc;
+    }
+    ok[nu] = acc == A;
+    if (acc < A) {
+      // no topology this attempt: nuclei far apart (no collision), so the
+      // collision step rejects the event and it is drawn again
+      int* raw = buf.raw + ((size_t)e * 2 + nu) * kAMax * 2;
+      const double x0 = (nu == 0) ? -55.0 : 5.0;
+      for (int q = 0; q < A; ++q) {
+        raw[2 * q] = (int)llrint((x0 + 0.19 * q) / kQ);
+        raw[2 * q + 1] = 0;
+      }
+    } else {
+      double sx = 0.0, sy = 0.0;
+      for (int i = 0; i < A; ++i) {
+        sx += px[nu][i];
+        sy += py[nu][i];
+      }
+      const double xcen = (nu == 0 ? 1.0 : -1.0) * (double)s_xc * kQ;
+      int* raw = buf.raw + ((size_t)e * 2 + nu) * kAMax * 2;
+      double* du = d.du + ((size_t)e * 2 + nu) * kAMax * 4;
+      // quantised positions, then a stable insertion sort by x
+      for (int i = 0; i < A; ++i) {
+        xq[nu][i] = (int)llrint((px[nu][i] - sx / A + xcen) / kQ);
+        int p = i;
+        while (p > 0 && xq[nu][ord[nu][p - 1]] > xq[nu][i]) {
+          ord[nu][p] = ord[nu][p - 1];
+          --p;
+        }
+        ord[nu][p] = i;
+      }
+      for (int q = 0; q < A; ++q) {
+        const int i = ord[nu][q];
+        raw[2 * q] = xq[nu][i];
+        raw[2 * q + 1] = (int)llrint((py[nu][i] - sy / A) / kQ);
+        du[4 * q] = uu[nu][i][0];
+        du[4 * q + 1] = uu[nu][i][1];
+        du[4 * q + 2] = uu[nu][i][2];
+        du[4 * q + 3] = 0.0;
+      }
+    }
+  }
+  __syncthreads();
+  if (lane == 0) d.topo_ok[e] = ok[0] && ok[1];
+}
+
+// ------------------------------------------------------------- forward
+// Positions at theta and their R, a derivatives (CM included).
+// dpos[((e*2+nu)*kAMax + i)*6 + {x, y, x_R, y_R, x_a, y_a}]
+__global__ __launch_bounds__(256) void k_diff_pos(DevCfg cfg, DiffBufs d, double R, double a) {
+  const int e = blockIdx.x, t = threadIdx.x;
+  __shared__ double red[4];
+  for (int nu = 0; nu < 2; ++nu) {
+    const int A = cfg.nuc[nu].A;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    if (t < A) {
+      const double* du = d.du + (((size_t)e * 2 + nu) * kAMax + t) * 4;
+      double r, rR, ra;
+      ws_finv(du[0], R, a, d.gl, d.ngl, r, rR, ra);
+      const double st = sqrt(fmax(0.0, 1.0 - du[1] * du[1]));
+      const double nx = st * cos(du[2]), ny = st * sin(du[2]);
+      v[0] = r * nx;
+      v[1] = r * ny;
+      v[2] = rR * nx;
+      v[3] = rR * ny;
+      v[4] = ra * nx;
+      v[5] = ra * ny;
+    }
+    double m[6];
+    for (int q = 0; q < 6; ++q) m[q] = block_sum_d(v[q], red) / A;
+    if (t < A) {
+      const double xcen = (nu == 0 ? 0.5 : -0.5) * d.bval[e];
+      double* o = d.dpos + (((size_t)e * 2 + nu) * kAMax + t) * 6;
+      o[0] = v[0] - m[0] + xcen;
+      o[1] = v[1] - m[1];
+      for (int q = 2; q < 6; ++q) o[q] = v[q] - m[q];
+    }
+  }
+}
+
+// Source positions (half quanta) and Gamma weights at theta into src; f
+// and df/ds per source into dsrc.
+__global__ __launch_bounds__(256) void k_diff_src(DevCfg cfg, EvBufs buf, DiffBufs d, uint2 key,
+                                                  unsigned long long event0, double alpha, double kg, int fluct) {
+  const int e = blockIdx.x, t = threadIdx.x;
+  const int* cnt = buf.counts + (size_t)e * 8;
+  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2], K = cnt[5];
+  __shared__ int inv[2][kAMax];
+  for (int nu = 0; nu < 2; ++nu) {
+    if (t < cfg.nuc[nu].A) {
+      const int w = buf.wound[(size_t)e * 2 * kAMax + nu * kAMax + t];
+      if (w >= 0) inv[nu][w] = t;
+    }
+  }
+  __syncthreads();
+  const unsigned long long gev = event0 + (unsigned long long)e;
+  int4* src = buf.src + (size_t)e * buf.src_cap;
+  double2* ds = d.dsrc + (size_t)e * buf.src_cap;
+  const double* pos = d.dpos + (size_t)e * 2 * kAMax * 6;
+  for (int k = t; k < K && k < npa + npb + ncoll; k += 256) {
+    int4 s = src[k];
+    double x, y;
+    if (k < npa + npb) {
+      const int nu = (k < npa) ? 0 : 1;
+      const int i = inv[nu][nu ? k - npa : k];
+      x = pos[((size_t)nu * kAMax + i) * 6];
+      y = pos[((size_t)nu * kAMax + i) * 6 + 1];
+    } else {
+      const unsigned pr = buf.pairs[(size_t)e * cfg.max_pairs + (k - npa - npb)];
+      const int i = (int)(pr & 0xFFFFu), j = (int)(pr >> 16);
+      x = 0.5 * (pos[(size_t)i * 6] + pos[((size_t)kAMax + j) * 6]);
+      y = 0.5 * (pos[(size_t)i * 6 + 1] + pos[((size_t)kAMax + j) * 6 + 1]);
+    }
+    s.x = (int)llrint(x / kH);
+    s.y = (int)llrint(y / kH);
+    double f = 1.0, dfds = 0.0;
+    if (fluct) {
+      const uint4 r = draw(key, gev, kStrDiffGamma, (unsigned)s.w, (unsigned)k, 0, 0);
+      const double v = u53(r.x, r.y);
+      const double sh = diff_shape(s.w, alpha, kg);
+      double q, dq;
+      gamma_qf(v, sh, q, dq);
+      f = q / sh;
+      dfds = dq / sh - q / (sh * sh);
+      d.gam_v[(size_t)e * buf.src_cap + k] = v;
+    }
+    s.z = __float_as_int((float)f);
+    src[k] = s;
+    ds[k] = make_double2(f, dfds);
+  }
+}
+
+// ------------------------------------------------------------- backward
+// Cell cotangents G = dL/drho for L = sum_e lam2 |eps2| + lam3 |eps3| +
+// mu dS/dy, from the per-event scalars of k_diff_moments (layout kDs*).
+__global__ __launch_bounds__(256) void k_diff_cot(DevCfg cfg, DiffBufs d, const double* cot, double ffdxdy,
+                                                  float* G) {
+  const int e = blockIdx.y;
+  const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)nx * ny) return;
+  const int ix = (int)(i / ny), iy = (int)(i % ny);
+  const double* sc = d.dsc + (size_t)e * kDsStride;
+  const double S = sc[kDsS], xc = sc[kDsXc], yc = sc[kDsYc], N = sc[kDsN] + 1e-15;
+  const double lam[2] = {cot[(size_t)e * 3], cot[(size_t)e * 3 + 1]};
+  const double mu = cot[(size_t)e * 3 + 2];
+  const double x = cfg.xmin + ix * cfg.dx - xc, y = cfg.ymin + iy * cfg.dy - yc;
+  const double r2 = x * x + y * y;
+  const bool m = (d.mask[(size_t)e * nx * nwy + (size_t)ix * nwy + (iy >> 5)] >> (iy & 31)) & 1u;
+  double g = mu * ffdxdy;
+  // n = 2, 3: A_n, dA_n/dxc, dA_n/dyc (complex), dN/dxc, dN/dyc
+  const double dNx = -2.0 * sc[kDsMx], dNy = -2.0 * sc[kDsMy];
+  for (int q = 0; q < 2; ++q) {
+    if (lam[q] == 0.0) continue;
+    const double Ar = sc[kDsA2r + 2 * q], Ai = sc[kDsA2i + 2 * q];
+    const double Am = sqrt(Ar * Ar + Ai * Ai);
+    if (!(Am > 0.0)) continue;
+    double dAxr, dAxi, dAyr, dAyi;
+    if (q == 0) {  // d(z^2)/dx = 2z, d(z^2)/dy = 2iz
+      dAxr = -2.0 * sc[kDsMx];
+      dAxi = -2.0 * sc[kDsMy];
+      dAyr = 2.0 * sc[kDsMy];
+      dAyi = -2.0 * sc[kDsMx];
+    } else {
+      dAxr = -sc[kDsB3xr];
+      dAxi = -sc[kDsB3xi];
+      dAyr = -sc[kDsB3yr];
+      dAyi = -sc[kDsB3yi];
+    }
+    const double cx = (Ar * dAxr + Ai * dAxi) / (Am * N) - Am * dNx / (N * N);
+    const double cy = (Ar * dAyr + Ai * dAyi) / (Am * N) - Am * dNy / (N * N);
+    double v = (cx * x + cy * y) / S;
+    if (m && r2 > 0.0) {
+      // g_n(z) = z^n r^(2-n)
+      double gr = x * x - y * y, gi = 2.0 * x * y;
+      if (q == 1) {
+        const double ir = 1.0 / sqrt(r2);
+        const double tr = gr * x - gi * y, ti = gr * y + gi * x;
+        gr = tr * ir;
+        gi = ti * ir;
+      }
+      v += (Ar * gr + Ai * gi) / (Am * N) - Am * r2 / (N * N);
+    }
+    g += lam[q] * v;
+  }
+  G[(size_t)e * nx * ny + i] = (float)g;
+}
+
+// One wavefront per source: sums of G E, G E dx, G E dy, G E d^2 over the
+// frozen footprint (E = exp(-d^2 / 2w^2)), then dL/dW, dL/dx, dL/dy, dL/dw.
+__global__ __launch_bounds__(256) void k_diff_gather(DevCfg cfg, EvBufs buf, SmearSpec spec, DiffBufs d,
+                                                     const float* G, double w, double alpha) {
+  const int e = blockIdx.y;
+  const int lane = threadIdx.x & 63;
+  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int K = buf.counts[(size_t)e * 8 + 5];
+  if (k >= K) return;
+  const int nx = cfg.nx, ny = cfg.ny;
+  const int4 s = buf.src[(size_t)e * buf.src_cap + k];
+  const int4 wv = buf.swin[(size_t)e * buf.src_cap + k];
+  const int kind = s.w;
+  const int xw = (spec.win[kind] == kWinBox) ? wv.x : wv.z;
+  const int yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
+  const int xlo = xw & 0xffff, xhi = xw >> 16, ylo = yw & 0xffff, yhi = yw >> 16;
+  const float c2 = (float)(1.0 / (2.0 * w * w) * 1.4426950408889634);
+  const float cut2 = cfg.cut2;
+  const float* g = G + (size_t)e * nx * ny;
+  float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+  for (int c0 = ylo; c0 < yhi; c0 += 64) {
+    const int col = c0 + lane;
+    if (col >= yhi) continue;
+    const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
+    const float dy = (float)(ygh - s.y) * kHf;
+    const float dy2 = dy * dy;
+    for (int ix = xlo; ix < xhi; ++ix) {
+      const int rowh = (int)llrint((cfg.xmin + ix * cfg.dx) / kH);
+      const float dx = (float)(rowh - s.x) * kHf;
+      const float dx2 = dx * dx;
+      if (!(dy2 <= cut2 - dx2)) continue;
+      const float E = __builtin_amdgcn_exp2f(-dx2 * c2) * __builtin_amdgcn_exp2f(-dy2 * c2);
+      const float gv = g[(size_t)ix * ny + col] * E;
+      e0 += gv;
+      e1 = fmaf(gv, dx, e1);  // dx, dy: cell minus source
+      e2 = fmaf(gv, dy, e2);
+      e3 = fmaf(gv, dx2 + dy2, e3);
+    }
+  }
+  const double s0 = wave_sum((double)e0), s1 = wave_sum((double)e1), s2 = wave_sum((double)e2),
+               s3 = wave_sum((double)e3);
+  if (lane == 0) {
+    const double norm = 1.0 / (2.0 * M_PI * w * w);
+    const double f = d.dsrc[(size_t)e * buf.src_cap + k].x;
+    const double W = ((kind == kSrcPair) ? alpha : 0.5 * (1.0 - alpha)) * f;
+    double4 o;
+    o.x = norm * s0;                                  // dL/dW
+    o.y = W * norm * s1 / (w * w);                    // dL/dx_source
+    o.z = W * norm * s2 / (w * w);                    // dL/dy_source
+    o.w = W * norm * (s3 / (w * w * w) - 2.0 * s0 / w);  // dL/dw through this source
+    d.gsrc[(size_t)e * buf.src_cap + k] = o;
+  }
+}
+
+// Per-event chain rule: gev[e][5] = dL/d(w, R, a, alpha, k).
+__global__ __launch_bounds__(256) void k_diff_chain(DevCfg cfg, EvBufs buf, DiffBufs d, double alpha, double kg) {
+  const int e = blockIdx.x, t = threadIdx.x;
+  __shared__ double red[4];
+  const int* cnt = buf.counts + (size_t)e * 8;
+  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2], K = cnt[5];
+  const double4* gs = d.gsrc + (size_t)e * buf.src_cap;
+  const double2* ds = d.dsrc + (size_t)e * buf.src_cap;
+  double gw = 0.0, ga = 0.0, gk = 0.0;
+  for (int k = t; k < K; k += 256) {
+    const double4 o = gs[k];
+    gw += o.w;
+    const double f = ds[k].x, fs = ds[k].y;
+    if (k >= npa + npb) {  // W = alpha f(s), s = (alpha + eps) k
+      ga += o.x * (f + alpha * fs * kg);
+      gk += o.x * alpha * fs * (alpha + 1e-8);
+    } else {  // W = (1 - alpha)/2 f(s), s = (1 - alpha + eps) k / 2
+      ga += o.x * (-0.5 * f - 0.5 * (1.0 - alpha) * fs * 0.5 * kg);
+      gk += o.x * 0.5 * (1.0 - alpha) * fs * 0.5 * (1.0 - alpha + 1e-8);
+    }
+  }
+  // positions: participants carry their source, pairs half to each nucleon
+  double gR = 0.0, gA = 0.0;
+  const unsigned* pairs = buf.pairs + (size_t)e * cfg.max_pairs;
+  for (int nu = 0; nu < 2; ++nu) {
+    if (t < cfg.nuc[nu].A) {
+      double gx = 0.0, gy = 0.0;
+      const int wnd = buf.wound[(size_t)e * 2 * kAMax + nu * kAMax + t];
+      if (wnd >= 0) {
+        const double4 o = gs[nu ? npa + wnd : wnd];
+        gx += o.y;
+        gy += o.z;
+      }
+      if (wnd >= 0) {
+        const int np = min(ncoll, cfg.max_pairs);
+        for (int q = 0; q < np; ++q) {
+          const unsigned pr = pairs[q];
+          const int i = nu ? (int)(pr >> 16) : (int)(pr & 0xFFFFu);
+          if (i == t) {
+            const double4 o = gs[npa + npb + q];
+            gx += 0.5 * o.y;
+            gy += 0.5 * o.z;
+          }
+        }
+      }
+      const double* p = d.dpos + (((size_t)e * 2 + nu) * kAMax + t) * 6;
+      gR += gx * p[2] + gy * p[3];
+      gA += gx * p[4] + gy * p[5];
+    }
+  }
+  const double v[5] = {gw, gR, gA, ga, gk};
+  for (int q = 0; q < 5; ++q) {
+    const double sum = block_sum_d(v[q], red);
+    if (t == 0) d.gev[(size_t)e * 5 + q] = sum;
+  }
+}
+
+// ------------------------------------------------------------ launchers
+int launch_diff_topo(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
+                     unsigned long long event0, double R0, double a0, void* stream) {
+  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  hipLaunchKernelGGL(k_diff_topo, dim3(nev), dim3(64), 0, (hipStream_t)stream, cfg, b, d, key, event0, R0, a0);
+  return (int)hipGetLastError();
+}
+
+int launch_diff_forward_pre(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
+                            unsigned long long event0, const double* theta, int fluct, void* stream) {
+  const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_diff_pos, dim3(nev), dim3(256), 0, st, cfg, d, theta[1], theta[2]);
+  hipLaunchKernelGGL(k_diff_src, dim3(nev), dim3(256), 0, st, cfg, b, d, key, event0, theta[3], theta[4], fluct);
+  return (int)hipGetLastError();
+}
+
+int launch_diff_backward(const DevCfg& cfg, const EvBufs& b, const SmearSpec& spec, const DiffBufs& d, int nev,
+                         const double* cot_dev, double ffdxdy, float* G, const double* theta, void* stream) {
+  hipStream_t st = (hipStream_t)stream;
+  const size_t ncell = (size_t)cfg.nx * cfg.ny;
+  hipLaunchKernelGGL(k_diff_cot, dim3((unsigned)((ncell + 255) / 256), nev), dim3(256), 0, st, cfg, d, cot_dev,
+                     ffdxdy, G);
+  hipLaunchKernelGGL(k_diff_gather, dim3((unsigned)((b.src_cap + 3) / 4), nev), dim3(256), 0, st, cfg, b, spec, d,
+                     (const float*)G, theta[0], theta[3]);
+  hipLaunchKernelGGL(k_diff_chain, dim3(nev), dim3(256), 0, st, cfg, b, d, theta[3], theta[4]);
+  return (int)hipGetLastError();
+}
+
+}  // namespace smc

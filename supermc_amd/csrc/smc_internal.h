@@ -95,6 +95,30 @@ constexpr int kStPairOverflow = 2;
 constexpr int kStPosRange = 4;
 constexpr int kStKlnTable = 8;
 
+// Differentiable batches (frozen topology): device buffers of a context
+struct DiffBufs {
+  int* pending;       // [ev] topology still to be drawn
+  int* attempt;       // [ev] attempt counter of the topology draw
+  int* topo_ok;       // [ev] both nuclei filled
+  double* bval;       // [ev] impact parameter used
+  double* du;         // [ev][2][kAMax][4] (u, cos theta, phi, 0), collision order
+  double* dpos;       // [ev][2][kAMax][6] x, y, x_R, y_R, x_a, y_a at theta
+  double2* dsrc;      // [ev][src_cap] Gamma factor f and df/ds
+  double* gam_v;      // [ev][src_cap] Gamma quantile uniforms
+  double* dsc;        // [ev][kDsStride] per-event moment scalars
+  unsigned* mask;     // [ev][nx][(ny+31)/32] hot-spot mask at theta0
+  double4* gsrc;      // [ev][src_cap] dL/dW, dL/dx, dL/dy, dL/dw per source
+  double* gev;        // [ev][5] dL/dtheta per event
+  const double2* gl;  // Gauss-Legendre nodes/weights on [0, 1]
+  int ngl;
+};
+// per-event scalars of the differentiable moments
+enum : int {
+  kDsS = 0, kDsXc, kDsYc, kDsA2r, kDsA2i, kDsA3r, kDsA3i, kDsN, kDsMx, kDsMy,
+  kDsB3xr, kDsB3xi, kDsB3yr, kDsB3yi, kDsStride = 16
+};
+constexpr int kDiffOut = 8;  // per-event forward outputs: |e2|, |e3|, dSdy, e2 re/im, e3 re/im, status
+
 // running-coupling constants of KLNModel::getAlphaStrong (KLNModel.h:91-96)
 struct KlnConst {
   double lq2, amax, inv_beta0;
@@ -124,6 +148,14 @@ int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  void* stream);
 int launch_tile_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
                       float* out1, void* stream);
+int launch_diff_topo(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
+                     unsigned long long event0, double R0, double a0, void* stream);
+int launch_diff_forward_pre(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
+                            unsigned long long event0, const double* theta, int fluct, void* stream);
+int launch_diff_moments(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, const float* grid, int nev,
+                        double ffdxdy, double* out, void* stream);
+int launch_diff_backward(const DevCfg& cfg, const EvBufs& b, const SmearSpec& spec, const DiffBufs& d, int nev,
+                         const double* cot_dev, double ffdxdy, float* G, const double* theta, void* stream);
 int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell, const void* ptf,
                       const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
 int launch_kln_points(const double* ta, const double* tb, double* out, int n, const void* ptd,

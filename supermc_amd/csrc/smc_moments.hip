@@ -105,6 +105,92 @@ __device__ __forceinline__ void load_row(const float* g, int ix, int nx, int ny,
   }
 }
 
+// Hot-spot mask of event e as an LDS bitmap mask[nx][(ny+31)/32]
+// (MCnucl.cpp:1303-1323, MakeDensity.cpp:2341-2359).  The caller orders the
+// atomicOr writes before any read with a barrier.
+template <int NW>
+__device__ void build_mask(const DevCfg& cfg, const EvBufs& buf, int e, unsigned* mask, MomLds& L) {
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nx = cfg.nx, ny = cfg.ny;
+  const int ncoll = buf.counts[(size_t)e * 8 + 2];
+  const int nwy = (ny + 31) >> 5;
+  // union
+  // of participant boxes in region coordinates.  CollisionPair boxes are
+  // zero boxes at the origin (CollisionPair.h:12, Box2D.cpp:14-22).
+  for (int i = t; i < nx * nwy; i += NW * 64) mask[i] = 0u;
+  const int hbq = cfg.hb_h >> 1;
+  int rxl = (ncoll > 0) ? 0 : 0x7fffffff, rxr = (ncoll > 0) ? 0 : (int)0x80000000;
+  int ryl = rxl, ryr = rxr;
+  const int* pos = buf.pos + (size_t)e * 2 * kAMax * 2;
+  const int* wound = buf.wound + (size_t)e * 2 * kAMax;
+  const int i0 = t;  // one nucleon slot per thread (2 * kAMax <= NW * 64)
+  const bool part = i0 < 2 * kAMax && (i0 % kAMax) < cfg.nuc[i0 / kAMax].A && wound[i0] >= 0;
+  int px = 0, py = 0;
+  if (part) {
+    px = pos[2 * i0];
+    py = pos[2 * i0 + 1];
+    rxl = px - hbq;
+    rxr = px + hbq;
+    ryl = py - hbq;
+    ryr = py + hbq;
+  }
+  {
+    int v0 = rxl, v1 = ryl, v2 = -rxr, v3 = -ryr;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      v0 = min(v0, __shfl_xor(v0, off, 64));
+      v1 = min(v1, __shfl_xor(v1, off, 64));
+      v2 = min(v2, __shfl_xor(v2, off, 64));
+      v3 = min(v3, __shfl_xor(v3, off, 64));
+    }
+    if (lane == 0) { L.ired[wv][0] = v0; L.ired[wv][1] = v1; L.ired[wv][2] = v2; L.ired[wv][3] = v3; }
+    __syncthreads();
+    rxl = ryl = 0x7fffffff;
+    int nrxr = 0x7fffffff, nryr = 0x7fffffff;
+    for (int q = 0; q < NW; ++q) {
+      rxl = min(rxl, L.ired[q][0]);
+      ryl = min(ryl, L.ired[q][1]);
+      nrxr = min(nrxr, L.ired[q][2]);
+      nryr = min(nryr, L.ired[q][3]);
+    }
+    rxr = -nrxr;
+    ryr = -nryr;
+  }
+  // participant rectangles in grid cells, compacted (OR is order-free)
+  {
+    const int offx = step_eval(cfg.tab[kOffX], rxl), offy = step_eval(cfg.tab[kOffY], ryl);
+    const unsigned long long m = __ballot(part);
+    if (lane == 0) L.ired[wv][0] = __popcll(m);
+    __syncthreads();
+    int base = 0, np = 0;
+    for (int q = 0; q < NW; ++q) {
+      if (q < wv) base += L.ired[q][0];
+      np += L.ired[q][0];
+    }
+    if (part) {
+      const int k = base + __popcll(m & ((1ull << lane) - 1ull));
+      L.rect[k][0] = max(step_eval(cfg.tab[kMaskX], px - hbq - rxl) + offx, 0);
+      L.rect[k][1] = min(step_eval(cfg.tab[kMaskX], px + hbq - rxl) + offx, nx);
+      L.rect[k][2] = max(step_eval(cfg.tab[kMaskY], py - hbq - ryl) + offy, 0);
+      L.rect[k][3] = min(step_eval(cfg.tab[kMaskX], py + hbq - ryl) + offy, ny);  // /dx (MakeDensity.cpp:2357)
+    }
+    __syncthreads();
+    // one wave per rectangle, one lane per row
+    for (int k = wv; k < np; k += NW) {
+      const int x0 = L.rect[k][0], x1 = L.rect[k][1], y0 = L.rect[k][2], y1 = L.rect[k][3];
+      for (int ix = x0 + lane; ix < x1; ix += 64) {
+        for (int y = y0; y < y1;) {
+          const int w = y >> 5, b0 = y & 31;
+          const int b1 = min(32, b0 + (y1 - y));
+          const unsigned bits = (b1 - b0 == 32) ? 0xffffffffu : (((1u << (b1 - b0)) - 1u) << b0);
+          atomicOr(&mask[(size_t)ix * nwy + w], bits);
+          y += b1 - b0;
+        }
+      }
+    }
+  }
+}
+
 // Pass 2 for any order range: <r^n> over the full grid and the masked eps_n
 // sums, rows prefetched.
 template <int NW, int NORD, int MAXC>
@@ -317,81 +403,7 @@ __device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const 
     }
   }
 
-  // hot-spot mask (MCnucl.cpp:1303-1323, MakeDensity.cpp:2341-2359): union
-  // of participant boxes in region coordinates.  CollisionPair boxes are
-  // zero boxes at the origin (CollisionPair.h:12, Box2D.cpp:14-22).
-  for (int i = t; i < nx * nwy; i += NW * 64) mask[i] = 0u;
-  const int hbq = cfg.hb_h >> 1;
-  int rxl = (ncoll > 0) ? 0 : 0x7fffffff, rxr = (ncoll > 0) ? 0 : (int)0x80000000;
-  int ryl = rxl, ryr = rxr;
-  const int* pos = buf.pos + (size_t)e * 2 * kAMax * 2;
-  const int* wound = buf.wound + (size_t)e * 2 * kAMax;
-  const int i0 = t;  // one nucleon slot per thread (2 * kAMax <= NW * 64)
-  const bool part = i0 < 2 * kAMax && (i0 % kAMax) < cfg.nuc[i0 / kAMax].A && wound[i0] >= 0;
-  int px = 0, py = 0;
-  if (part) {
-    px = pos[2 * i0];
-    py = pos[2 * i0 + 1];
-    rxl = px - hbq;
-    rxr = px + hbq;
-    ryl = py - hbq;
-    ryr = py + hbq;
-  }
-  {
-    int v0 = rxl, v1 = ryl, v2 = -rxr, v3 = -ryr;
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      v0 = min(v0, __shfl_xor(v0, off, 64));
-      v1 = min(v1, __shfl_xor(v1, off, 64));
-      v2 = min(v2, __shfl_xor(v2, off, 64));
-      v3 = min(v3, __shfl_xor(v3, off, 64));
-    }
-    if (lane == 0) { L.ired[wv][0] = v0; L.ired[wv][1] = v1; L.ired[wv][2] = v2; L.ired[wv][3] = v3; }
-    __syncthreads();
-    rxl = ryl = 0x7fffffff;
-    int nrxr = 0x7fffffff, nryr = 0x7fffffff;
-    for (int q = 0; q < NW; ++q) {
-      rxl = min(rxl, L.ired[q][0]);
-      ryl = min(ryl, L.ired[q][1]);
-      nrxr = min(nrxr, L.ired[q][2]);
-      nryr = min(nryr, L.ired[q][3]);
-    }
-    rxr = -nrxr;
-    ryr = -nryr;
-  }
-  // participant rectangles in grid cells, compacted (OR is order-free)
-  {
-    const int offx = step_eval(cfg.tab[kOffX], rxl), offy = step_eval(cfg.tab[kOffY], ryl);
-    const unsigned long long m = __ballot(part);
-    if (lane == 0) L.ired[wv][0] = __popcll(m);
-    __syncthreads();
-    int base = 0, np = 0;
-    for (int q = 0; q < NW; ++q) {
-      if (q < wv) base += L.ired[q][0];
-      np += L.ired[q][0];
-    }
-    if (part) {
-      const int k = base + __popcll(m & ((1ull << lane) - 1ull));
-      L.rect[k][0] = max(step_eval(cfg.tab[kMaskX], px - hbq - rxl) + offx, 0);
-      L.rect[k][1] = min(step_eval(cfg.tab[kMaskX], px + hbq - rxl) + offx, nx);
-      L.rect[k][2] = max(step_eval(cfg.tab[kMaskY], py - hbq - ryl) + offy, 0);
-      L.rect[k][3] = min(step_eval(cfg.tab[kMaskX], py + hbq - ryl) + offy, ny);  // /dx (MakeDensity.cpp:2357)
-    }
-    __syncthreads();
-    // one wave per rectangle, one lane per row
-    for (int k = wv; k < np; k += NW) {
-      const int x0 = L.rect[k][0], x1 = L.rect[k][1], y0 = L.rect[k][2], y1 = L.rect[k][3];
-      for (int ix = x0 + lane; ix < x1; ix += 64) {
-        for (int y = y0; y < y1;) {
-          const int w = y >> 5, b0 = y & 31;
-          const int b1 = min(32, b0 + (y1 - y));
-          const unsigned bits = (b1 - b0 == 32) ? 0xffffffffu : (((1u << (b1 - b0)) - 1u) << b0);
-          atomicOr(&mask[(size_t)ix * nwy + w], bits);
-          y += b1 - b0;
-        }
-      }
-    }
-  }
+  build_mask<NW>(cfg, buf, e, mask, L);
   block_sums<NW, 3>(s1, L);  // also orders the mask writes before the reads below
   const double total = L.tot[0];
   const double xc = L.tot[1] / total, yc = L.tot[2] / total;
@@ -424,6 +436,100 @@ __global__ __launch_bounds__(kGT) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 :
   double s1[3] = {0.0, 0.0, 0.0};
   if (flags & 2) grid_moments<kGW, NORD, MAXC, FIX25>(cfg, buf, e, g, evout, L, dyn_mask, false, s1);
   else grid_sum<kGW>(cfg, e, g, evout, L);
+}
+
+// Differentiable batches: per-event FP64 scalars for |eps_2|, |eps_3| and
+// their adjoint (layout kDs* in smc_internal.h), the forward outputs, and
+// the hot-spot mask (frozen at theta0: built from the stored positions).
+//   A_n = sum_mask rho z^n r^(2-n), N = sum_mask rho r^2, z = (x-xc) + i(y-yc)
+//   Mx, My = sum_mask rho x, rho y;  B3x, B3y = sum_mask rho d(z^3/r)/dx, /dy
+__global__ __launch_bounds__(kGT) void k_diff_moments(DevCfg cfg, EvBufs buf, DiffBufs d, const float* grid,
+                                                      double ffdxdy, double* out) {
+  __shared__ MomLds L;
+  extern __shared__ unsigned dyn_mask[];
+  const int e = blockIdx.x;
+  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
+  const float* g = grid + (size_t)e * nx * ny;
+  double s1[3] = {0.0, 0.0, 0.0};
+  for (int ix = wv; ix < nx; ix += kGW) {
+    const double xg = cfg.xmin + ix * cfg.dx;
+    for (int iy = lane; iy < ny; iy += 64) {
+      const double v = g[(size_t)ix * ny + iy];
+      s1[0] += v;
+      s1[1] += xg * v;
+      s1[2] += (cfg.ymin + iy * cfg.dy) * v;
+    }
+  }
+  build_mask<kGW>(cfg, buf, e, dyn_mask, L);
+  block_sums<kGW, 3>(s1, L);
+  const double S = L.tot[0], xc = L.tot[1] / S, yc = L.tot[2] / S;
+  double a[11];
+#pragma unroll
+  for (int i = 0; i < 11; ++i) a[i] = 0.0;
+  for (int ix = wv; ix < nx; ix += kGW) {
+    const double x = cfg.xmin + ix * cfg.dx - xc;
+    const unsigned* row = dyn_mask + (size_t)ix * nwy;
+    for (int iy = lane; iy < ny; iy += 64) {
+      const double v = g[(size_t)ix * ny + iy];
+      if (v == 0.0 || !((row[iy >> 5] >> (iy & 31)) & 1u)) continue;
+      const double y = cfg.ymin + iy * cfg.dy - yc;
+      const double r2 = x * x + y * y;
+      if (r2 == 0.0) continue;
+      const double ir = 1.0 / sqrt(r2), ir3 = ir * ir * ir;
+      const double z2r = x * x - y * y, z2i = 2.0 * x * y;
+      const double z3r = z2r * x - z2i * y, z3i = z2r * y + z2i * x;
+      a[0] += v * z2r;
+      a[1] += v * z2i;
+      a[2] += v * z3r * ir;
+      a[3] += v * z3i * ir;
+      a[4] += v * r2;
+      a[5] += v * x;
+      a[6] += v * y;
+      a[7] += v * (3.0 * z2r * ir - z3r * x * ir3);
+      a[8] += v * (3.0 * z2i * ir - z3i * x * ir3);
+      a[9] += v * (-3.0 * z2i * ir - z3r * y * ir3);
+      a[10] += v * (3.0 * z2r * ir - z3i * y * ir3);
+    }
+  }
+  block_sums<kGW, 11>(a, L);
+  if (t == 0) {
+    const double* T = L.tot;
+    double* sc = d.dsc + (size_t)e * kDsStride;
+    sc[kDsS] = S;
+    sc[kDsXc] = xc;
+    sc[kDsYc] = yc;
+    sc[kDsA2r] = T[0];
+    sc[kDsA2i] = T[1];
+    sc[kDsA3r] = T[2];
+    sc[kDsA3i] = T[3];
+    sc[kDsN] = T[4];
+    sc[kDsMx] = T[5];
+    sc[kDsMy] = T[6];
+    sc[kDsB3xr] = T[7];
+    sc[kDsB3xi] = T[8];
+    sc[kDsB3yr] = T[9];
+    sc[kDsB3yi] = T[10];
+    const double N = T[4] + 1e-15;
+    double* o = out + (size_t)e * kDiffOut;
+    o[0] = sqrt(T[0] * T[0] + T[1] * T[1]) / N;
+    o[1] = sqrt(T[2] * T[2] + T[3] * T[3]) / N;
+    o[2] = S * ffdxdy;
+    o[3] = -T[0] / N;
+    o[4] = -T[1] / N;
+    o[5] = -T[2] / N;
+    o[6] = -T[3] / N;
+    o[7] = buf.counts[(size_t)e * 8 + 3];
+  }
+  for (int i = t; i < nx * nwy; i += kGT) d.mask[(size_t)e * nx * nwy + i] = dyn_mask[i];
+}
+
+int launch_diff_moments(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, const float* grid, int nev,
+                        double ffdxdy, double* out, void* stream) {
+  if (nev <= 0) return 0;
+  const size_t shm = (size_t)cfg.nx * ((cfg.ny + 31) / 32) * sizeof(unsigned);
+  hipLaunchKernelGGL(k_diff_moments, dim3(nev), dim3(kGT), shm, (hipStream_t)stream, cfg, b, d, grid, ffdxdy, out);
+  return (int)hipGetLastError();
 }
 
 int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* ev_out, int mode,
