@@ -126,6 +126,33 @@ __device__ inline void ws_candidate(const NucSys& ns, uint2 key, unsigned long l
   x = y = z = 0.f;
 }
 
+// One proposal of the rejection sampler above (Nucleus.cpp:578-621): one
+// draw per proposal; returns whether it is accepted and its position.
+__device__ inline bool ws_proposal(const NucSys& ns, uint2 key, unsigned long long ev, unsigned nuc, unsigned idx,
+                                   unsigned attempt, unsigned step, float ctr, float phir, float& x, float& y,
+                                   float& z) {
+  const uint4 r = draw(key, ev, kStrNucleon, nuc, idx, attempt, step);
+  const float rr = ns.rmax * cbrtf(u01(r.x));
+  float cx, acc;
+  if (ns.deformed) {
+    cx = 1.f - 2.f * u01(r.y);
+    const float rad1 = ns.R * (1.f + ns.beta2 * y20(cx) + ns.beta4 * y40(cx));
+    const float rw1 = 1.f / (1.f + expf(-rad1 / ns.a));
+    acc = (u01(r.z) * rw1 <= 1.f / (1.f + expf((rr - rad1) / ns.a))) ? 1.f : 0.f;
+  } else {
+    acc = (u01(r.y) * ns.rwmax <= 1.f / (1.f + expf((rr - ns.R) / ns.a))) ? 1.f : 0.f;
+    cx = 1.f - 2.f * u01(r.z);
+  }
+  const float sx = sqrtf(fmaxf(0.f, 1.f - cx * cx));
+  float sp, cp;
+  sincosf(2.f * kPi * u01(r.w), &sp, &cp);
+  x = rr * sx * cp;
+  y = rr * sx * sp;
+  z = rr * cx;
+  if (ns.deformed) rotate3(ctr, phir, x, y, z);
+  return acc != 0.f;
+}
+
 // Hulthen CDF (HulthenFunc.cpp:27-36) inverted by safeguarded Newton.
 __device__ inline float hulthen_inv(float u) {
   const float al = 0.228f, be = 1.18f;

@@ -29,8 +29,12 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
   const int lane = t & 63, wv = t >> 6;
   const unsigned long long gev = event0 + (unsigned long long)e;
 
-  __shared__ int px[2][kAMax], py[2][kAMax];  // input/sample order (quanta)
-  __shared__ float pz[2][kAMax];              // local z (hard core only)
+  __shared__ __align__(16) int px[2][kAMax], py[2][kAMax];  // input/sample order (quanta)
+  __shared__ __align__(16) float pz[2][kAMax];              // local z (hard core only)
+  __shared__ int cqx[2][64], cqy[2][64];                     // candidates of a round
+  __shared__ float cqz[2][64];
+  __shared__ int s_nc[2], s_wc[4];
+  __shared__ unsigned long long s_okm[4];
   __shared__ int sx[2][kAMax], sy[2][kAMax];  // sorted
   __shared__ unsigned hits[kAMax][kAMax / 32];
   __shared__ int rowcnt[kAMax], fh[kAMax], scan_tmp[256];
@@ -98,42 +102,116 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
       // nucleons (hard core, Nucleus.cpp:276-294), 64 candidates per round
       // taken in stream order -- the same law as the reference's one-at-a-
       // time resampling.
+      // Rounds of 64 candidates per nucleus.  Proposals come from all four
+      // waves (waves nu and nu + 2 serve nucleus nu), one draw per lane per
+      // step, accepted ones compacted in stream order (step, wave, lane)
+      // into the round's candidate list; then the hard core against the
+      // nucleons accepted so far (split over the two waves) and the
+      // sequential acceptance among the round's candidates in list order.
+      const int nu = wv & 1, sub = wv >> 1;
       for (unsigned round = 0;; ++round) {
         const bool need0 = cfg.nuc[0].kind == kWoodsSaxon && s_acc[0] < A;
         const bool need1 = cfg.nuc[1].kind == kWoodsSaxon && s_acc[1] < B;
         if (!need0 && !need1) break;
         if (round > 4096) { status |= kStNoEvent; break; }
-        if (wv < 2 && ((wv == 0) ? need0 : need1)) {
-          const NucSys& ns = cfg.nuc[wv];
-          const int An = NN[wv];
-          const int acc0 = s_acc[wv];
-          float x, y, z;
-          ws_candidate(ns, key, gev, wv, round * 64u + lane, attempt, ctr[wv], phr[wv], x, y, z);
-          const int xq = quant(x), yq = quant(y);
-          bool ok = true;
-          for (int j = 0; j < acc0 && ok; ++j) {
-            const float dx = (float)(xq - px[wv][j]) * kQf, dy = (float)(yq - py[wv][j]) * kQf;
-            const float dz = z - pz[wv][j];
+        const bool need = nu ? need1 : need0;
+        const NucSys& ns = cfg.nuc[nu];
+        if (t < 2) s_nc[t] = 0;
+        __syncthreads();
+        for (unsigned it = 0;; ++it) {
+          bool take = false;
+          int xq = 0, yq = 0;
+          float z = 0.f;
+          const int nc0 = s_nc[nu];
+          if (need && nc0 < 64) {
+            float x, y;
+            take = ws_proposal(ns, key, gev, nu, round * 128u + sub * 64u + lane, attempt, it, ctr[nu], phr[nu],
+                               x, y, z);
+            xq = quant(x);
+            yq = quant(y);
+          }
+          const unsigned long long m = __ballot(take);
+          if (lane == 0) s_wc[wv] = __popcll(m);
+          __syncthreads();
+          if (take) {
+            const int pos = nc0 + (sub ? s_wc[nu] : 0) + __popcll(m & ((1ull << lane) - 1ull));
+            if (pos < 64) {
+              cqx[nu][pos] = xq;
+              cqy[nu][pos] = yq;
+              cqz[nu][pos] = z;
+            }
+          }
+          __syncthreads();
+          if (t < 2) s_nc[t] = min(64, s_nc[t] + s_wc[t] + s_wc[t + 2]);
+          __syncthreads();
+          const bool more0 = need0 && s_nc[0] < 64, more1 = need1 && s_nc[1] < 64;
+          if (!(more0 || more1) || it >= 0xFFFEu) break;
+        }
+        // hard core against the nucleons accepted in earlier rounds: wave
+        // nu + 2*sub checks every candidate against half of them
+        const int acc0 = s_acc[nu];
+        bool ok = lane < s_nc[nu];  // a full list unless the step cap was hit
+        if (need && ok) {
+          const int xq = cqx[nu][lane], yq = cqy[nu][lane];
+          const float z = cqz[nu][lane];
+          const int half = ((acc0 + 7) >> 3) << 2;  // multiple of 4
+          const int j0 = sub ? half : 0, j1 = sub ? acc0 : min(half, acc0);
+          const int4* px4 = reinterpret_cast<const int4*>(px[nu]);
+          const int4* py4 = reinterpret_cast<const int4*>(py[nu]);
+          const float4* pz4 = reinterpret_cast<const float4*>(pz[nu]);
+          int j = j0;
+          for (; j + 4 <= j1 && ok; j += 4) {
+            const int4 ax = px4[j >> 2], ay = py4[j >> 2];
+            const float4 az = pz4[j >> 2];
+            const int axs[4] = {ax.x, ax.y, ax.z, ax.w}, ays[4] = {ay.x, ay.y, ay.z, ay.w};
+            const float azs[4] = {az.x, az.y, az.z, az.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const float dx = (float)(xq - axs[u]) * kQf, dy = (float)(yq - ays[u]) * kQf;
+              const float dz = z - azs[u];
+              if (dx * dx + dy * dy + dz * dz < cfg.dmin2) ok = false;
+            }
+          }
+          for (; j < j1 && ok; ++j) {
+            const float dx = (float)(xq - px[nu][j]) * kQf, dy = (float)(yq - py[nu][j]) * kQf;
+            const float dz = z - pz[nu][j];
             if (dx * dx + dy * dy + dz * dz < cfg.dmin2) ok = false;
           }
-          bool mine = false;
-          int slot = 0, nacc = acc0;
+        }
+        const unsigned long long okw = __ballot(ok);
+        if (lane == 0) s_okm[wv] = okw;
+        __syncthreads();
+        if (need && sub == 0) {
+          const int xq = cqx[nu][lane], yq = cqy[nu][lane];
+          const float z = cqz[nu][lane];
+          // conflicts with the earlier candidates of this round: bit l of cm
+          unsigned long long cm = 0ull;
+          for (int l = 0; l < 64; ++l) {
+            const float dx = (float)(xq - cqx[nu][l]) * kQf, dy = (float)(yq - cqy[nu][l]) * kQf;
+            const float dz = z - cqz[nu][l];
+            if (l < lane && dx * dx + dy * dy + dz * dz < cfg.dmin2) cm |= 1ull << l;
+          }
+          // sequential acceptance in list order on wave-uniform masks
+          const unsigned long long okm = s_okm[nu] & s_okm[nu + 2];
+          const int An = NN[nu];
+          unsigned long long accm = 0ull;
+          int nacc = acc0;
           for (int l = 0; l < 64 && nacc < An; ++l) {
-            const int okl = __shfl((int)ok, l, 64);
-            if (!okl) continue;
-            const int xl = __shfl(xq, l, 64), yl = __shfl(yq, l, 64);
-            const float zl = __shfl(z, l, 64);
-            bool conflict = false;
-            if (mine) {
-              const float dx = (float)(xq - xl) * kQf, dy = (float)(yq - yl) * kQf, dz = z - zl;
-              conflict = dx * dx + dy * dy + dz * dz < cfg.dmin2;
-            }
-            if (__any(conflict)) continue;
-            if (lane == l) { mine = true; slot = nacc; }
+            if (!((okm >> l) & 1ull)) continue;
+            const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)cm, l);
+            const unsigned hi32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(cm >> 32), l);
+            const unsigned long long cl = ((unsigned long long)hi32 << 32) | lo32;
+            if (cl & accm) continue;
+            accm |= 1ull << l;
             ++nacc;
           }
-          if (mine) { px[wv][slot] = xq; py[wv][slot] = yq; pz[wv][slot] = z; }
-          if (lane == 0) s_acc[wv] = nacc;
+          if ((accm >> lane) & 1ull) {
+            const int slot = acc0 + __popcll(accm & ((1ull << lane) - 1ull));
+            px[nu][slot] = xq;
+            py[nu][slot] = yq;
+            pz[nu][slot] = z;
+          }
+          if (lane == 0) s_acc[nu] = nacc;
         }
         __syncthreads();
       }
@@ -278,28 +356,33 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
     }
   }
   if (ncoll > cfg.max_pairs || !src_ok) status |= kStPairOverflow;
+  // pair list in reference order (projectile rows, then target index)
+  const unsigned long long pbase = (unsigned long long)e * cfg.max_pairs;
   if (t < A && rowcnt[t] > 0) {
-    const unsigned long long pbase = (unsigned long long)e * cfg.max_pairs;
     int k = off;
-    const int xa = sx[0][t], ya = sy[0][t];
     for (int w = 0; w < kAMax / 32; ++w) {
       unsigned m = hits[t][w];
       while (m) {
-        const int b = __ffs(m) - 1;
+        const int j = w * 32 + __ffs(m) - 1;
         m &= m - 1;
-        const int j = w * 32 + b;
-        if (k < cfg.max_pairs) {
-          float fp = 1.f;
-          if (cfg.fluct) fp = gamma_draw(cfg.gk_bin, cfg.gs_bin, key, gev, kStrFluctPair, (unsigned)k);
-          buf.pairs[pbase + k] = (unsigned)t | ((unsigned)j << 16);
-          buf.pfluct[pbase + k] = fp;
-          if (src_ok) {
-            src[nsrc_part + k] = make_int4(xa + sx[1][j], ya + sy[1][j], __float_as_int(fp), kSrcPair);
-            swin[nsrc_part + k] = src_windows(cfg, xa + sx[1][j], ya + sy[1][j], false);
-          }
-        }
+        if (k < cfg.max_pairs) buf.pairs[pbase + k] = (unsigned)t | ((unsigned)j << 16);
         ++k;
       }
+    }
+  }
+  __syncthreads();  // the pair list (global) is read back by every thread below
+  // Gamma factors, sources and windows of the pairs, spread over the workgroup
+  const int npairs = min(ncoll, cfg.max_pairs);
+  for (int k = t; k < npairs; k += 256) {
+    const unsigned pr = buf.pairs[pbase + k];
+    const int i = (int)(pr & 0xFFFFu), j = (int)(pr >> 16);
+    float fp = 1.f;
+    if (cfg.fluct) fp = gamma_draw(cfg.gk_bin, cfg.gs_bin, key, gev, kStrFluctPair, (unsigned)k);
+    buf.pfluct[pbase + k] = fp;
+    if (src_ok) {
+      const int xh = sx[0][i] + sx[1][j], yh = sy[0][i] + sy[1][j];
+      src[nsrc_part + k] = make_int4(xh, yh, __float_as_int(fp), kSrcPair);
+      swin[nsrc_part + k] = src_windows(cfg, xh, yh, false);
     }
   }
   if (t == 0) {

