@@ -222,6 +222,26 @@ __device__ inline int block_excl_scan(int v, int* tmp) {
   return incl - v;
 }
 
+// Ascending bitonic sort of key[0][0..255] and key[1][0..255] by a
+// 256-thread workgroup (threads 0-127 sort key[0], 128-255 key[1]); ends
+// with a barrier.
+__device__ inline void bitonic_sort_2x256(unsigned long long (*key)[256]) {
+  const int t = threadIdx.x, a = t >> 7, p = t & 127;
+  __syncthreads();
+  for (int k = 2; k <= 256; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      const int i = ((p & ~(j - 1)) << 1) | (p & (j - 1));
+      const int l = i + j;
+      const bool up = (i & k) == 0;
+      const unsigned long long x = key[a][i], y = key[a][l];
+      if ((x > y) == up) {
+        key[a][i] = y;
+        key[a][l] = x;
+      }
+      __syncthreads();
+    }
+}
+
 // Exact footprints of a source at (ph, qh) half quanta: the participant box
 // (MCnucl.cpp:831-844) and the +-5w window (MCnucl.cpp:450-457, 729-736),
 // each packed as lo | hi << 16 per axis.

@@ -34,6 +34,8 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
   __shared__ int cqx[2][64], cqy[2][64];                     // candidates of a round
   __shared__ float cqz[2][64];
   __shared__ int s_nc[2], s_wc[4];
+  __shared__ unsigned long long skey[2][kAMax];  // sort keys
+  __shared__ int wrank[kAMax];
   __shared__ unsigned long long s_okm[4];
   __shared__ int sx[2][kAMax], sy[2][kAMax];  // sorted
   __shared__ unsigned hits[kAMax][kAMax / 32];
@@ -230,20 +232,19 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
       __syncthreads();
     }
 
-    // ---- sort each nucleus by x, ties by input index (Nucleus.cpp:314)
-    for (int nu = 0; nu < 2; ++nu) {
-      const int An = NN[nu];
-      for (int i = t; i < An; i += 256) {
-        const int xi = px[nu][i];
-        int rank = 0;
-        for (int j = 0; j < An; ++j) {
-          const int xj = px[nu][j];
-          rank += (xj < xi) || (xj == xi && j < i);
-        }
-        sx[nu][rank] = xi;
-        sy[nu][rank] = py[nu][i];
+    // ---- sort each nucleus by x, ties by input index (Nucleus.cpp:314):
+    // bitonic sort of the keys (x, index)
+    for (int nu = 0; nu < 2; ++nu)
+      skey[nu][t] = (t < NN[nu]) ? (((unsigned long long)((unsigned)px[nu][t] ^ 0x80000000u) << 32) | (unsigned)t)
+                                 : ~0ull;
+    bitonic_sort_2x256(skey);
+    for (int nu = 0; nu < 2; ++nu)
+      if (t < NN[nu]) {
+        const int i = (int)(skey[nu][t] & 0xFFFFFFFFull);
+        sx[nu][t] = px[nu][i];
+        sy[nu][t] = py[nu][i];
       }
-    }
+    if (t < B) fh[t] = 0x7FFFFFFF;
     __syncthreads();
 
     // ---- A x B disk test (MCnucl.cpp:359-366), exact in integers:
@@ -272,6 +273,7 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
             for (int w = 0; w < kAMax / 32; ++w)
               if (w == (j >> 5)) bits[w] |= 1u << (j & 31);
             ++cnt;
+            atomicMin(&fh[j], t);  // first hitting projectile row
           }
         }
 #pragma unroll
@@ -282,13 +284,7 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
       }
       __syncthreads();
       // first hitting projectile row of each target nucleon: its wound order
-      int first = -1;
-      if (t < B) {
-        const unsigned m = 1u << (t & 31);
-        const int wd = t >> 5;
-        for (int i = 0; i < A; ++i)
-          if (hits[i][wd] & m) { first = i; break; }
-      }
+      const int first = (t < B && fh[t] != 0x7FFFFFFF) ? fh[t] : -1;
       fh[t] = first;
       npa = __syncthreads_count(t < A && cnt > 0);
       npb = __syncthreads_count(first >= 0);
@@ -313,15 +309,13 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
     const int wA = (t < A && rowcnt[t] > 0) ? 1 : 0;
     const int rA = block_excl_scan(wA, scan_tmp);
     if (wA) woundA = rA;
-    if (t < B && fh[t] >= 0) {
-      int r = 0;
-      const int f = fh[t];
-      for (int j = 0; j < B; ++j) {
-        const int g = fh[j];
-        r += (g >= 0) && (g < f || (g == f && j < t));
-      }
-      woundB = r;
-    }
+    // target wound order: by first hitting row, then index (bitonic sort)
+    skey[0][t] = ~0ull;
+    skey[1][t] = (t < B && fh[t] >= 0) ? (((unsigned long long)fh[t] << 32) | (unsigned)t) : ~0ull;
+    bitonic_sort_2x256(skey);
+    if (skey[1][t] != ~0ull) wrank[skey[1][t] & 0xFFFFu] = t;
+    __syncthreads();
+    if (t < B && fh[t] >= 0) woundB = wrank[t];
   }
   const int off = block_excl_scan(t < A ? rowcnt[t] : 0, scan_tmp);
   float fA = 1.f, fB = 1.f;
