@@ -249,6 +249,7 @@ void validate_and_derive(smc_ctx* c) {
     ns.beta4 = (float)b4;
     ns.rmax = (float)(R + 2.5);
     ns.rwmax = (float)(1.0 / (1.0 + std::exp(-R / a)));
+    ns.inv_a = (float)(1.0 / a);
   }
   d.dmin = (p.nucleon_min_dist >= 0) ? p.nucleon_min_dist : 0.9;
 

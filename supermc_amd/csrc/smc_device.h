@@ -140,12 +140,14 @@ __device__ inline bool ws_proposal(const NucSys& ns, uint2 key, unsigned long lo
     const float rw1 = 1.f / (1.f + expf(-rad1 / ns.a));
     acc = (u01(r.z) * rw1 <= 1.f / (1.f + expf((rr - rad1) / ns.a))) ? 1.f : 0.f;
   } else {
-    acc = (u01(r.y) * ns.rwmax <= 1.f / (1.f + expf((rr - ns.R) / ns.a))) ? 1.f : 0.f;
+    // u rho(0) <= 1/(1+e^((r-R)/a))  <=>  u rho(0) (1 + e^((r-R)/a)) <= 1
+    acc = (u01(r.y) * ns.rwmax * (1.f + __expf((rr - ns.R) * ns.inv_a)) <= 1.f) ? 1.f : 0.f;
     cx = 1.f - 2.f * u01(r.z);
   }
   const float sx = sqrtf(fmaxf(0.f, 1.f - cx * cx));
-  float sp, cp;
-  sincosf(2.f * kPi * u01(r.w), &sp, &cp);
+  // sin and cos of 2 pi u: the hardware sine and cosine take revolutions
+  const float rev = u01(r.w);
+  const float sp = __builtin_amdgcn_sinf(rev), cp = __builtin_amdgcn_cosf(rev);
   x = rr * sx * cp;
   y = rr * sx * sp;
   z = rr * cx;

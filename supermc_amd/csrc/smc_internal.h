@@ -22,6 +22,7 @@ struct NucSys {
   float R, a, beta2, beta4;
   float rmax;   // R + 2.5 (Nucleus.cpp:118)
   float rwmax;  // 1/(1+exp(-R/a)) (Nucleus.cpp:119)
+  float inv_a;  // 1/a
 };
 
 // Monotone step function f: int32 -> [vlo, vlo+n] given by thresholds:

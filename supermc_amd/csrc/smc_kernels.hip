@@ -19,7 +19,7 @@
 namespace smc {
 
 // ------------------------------------------------------------- k_event
-__global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
                                                unsigned long long event0, int mode) {
   // mode 0: impact parameter sampled, events accepted as MakeDensity.cpp:2147-2162
   // mode 1: fed positions; mode 2: sampled nuclei at the given b, one attempt
@@ -31,9 +31,9 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
 
   __shared__ __align__(16) int px[2][kAMax], py[2][kAMax];  // input/sample order (quanta)
   __shared__ __align__(16) float pz[2][kAMax];              // local z (hard core only)
-  __shared__ int cqx[2][64], cqy[2][64];                     // candidates of a round
-  __shared__ float cqz[2][64];
-  __shared__ int s_nc[2], s_wc[4];
+  __shared__ __align__(16) int cqx[2][64], cqy[2][64];       // candidates of a round
+  __shared__ __align__(16) float cqz[2][64];
+  __shared__ int s_nc[2], s_wc[2][4];
   __shared__ unsigned long long skey[2][kAMax];  // sort keys
   __shared__ int wrank[kAMax];
   __shared__ unsigned long long s_okm[4];
@@ -118,14 +118,19 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
         if (round > 4096) { status |= kStNoEvent; break; }
         const bool need = nu ? need1 : need0;
         const NucSys& ns = cfg.nuc[nu];
-        if (t < 2) s_nc[t] = 0;
-        __syncthreads();
+        int nc[2] = {0, 0};  // candidates so far (uniform over the workgroup)
+        // candidates wanted this round: the nucleons still missing plus a
+        // margin for the hard core, at most 64
+        int want[2];
+        for (int q = 0; q < 2; ++q) {
+          const int miss = NN[q] - s_acc[q];
+          want[q] = min(64, miss + (miss >> 3) + 4);
+        }
         for (unsigned it = 0;; ++it) {
           bool take = false;
           int xq = 0, yq = 0;
           float z = 0.f;
-          const int nc0 = s_nc[nu];
-          if (need && nc0 < 64) {
+          if (need && nc[nu] < want[nu]) {
             float x, y;
             take = ws_proposal(ns, key, gev, nu, round * 128u + sub * 64u + lane, attempt, it, ctr[nu], phr[nu],
                                x, y, z);
@@ -133,22 +138,24 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
             yq = quant(y);
           }
           const unsigned long long m = __ballot(take);
-          if (lane == 0) s_wc[wv] = __popcll(m);
+          int* wc = s_wc[it & 1];  // double-buffered: one barrier per step
+          if (lane == 0) wc[wv] = __popcll(m);
           __syncthreads();
           if (take) {
-            const int pos = nc0 + (sub ? s_wc[nu] : 0) + __popcll(m & ((1ull << lane) - 1ull));
+            const int pos = nc[nu] + (sub ? wc[nu] : 0) + __popcll(m & ((1ull << lane) - 1ull));
             if (pos < 64) {
               cqx[nu][pos] = xq;
               cqy[nu][pos] = yq;
               cqz[nu][pos] = z;
             }
           }
-          __syncthreads();
-          if (t < 2) s_nc[t] = min(64, s_nc[t] + s_wc[t] + s_wc[t + 2]);
-          __syncthreads();
-          const bool more0 = need0 && s_nc[0] < 64, more1 = need1 && s_nc[1] < 64;
+          nc[0] = min(64, nc[0] + wc[0] + wc[2]);
+          nc[1] = min(64, nc[1] + wc[1] + wc[3]);
+          const bool more0 = need0 && nc[0] < want[0], more1 = need1 && nc[1] < want[1];
           if (!(more0 || more1) || it >= 0xFFFEu) break;
         }
+        if (t < 2) s_nc[t] = nc[t];
+        __syncthreads();
         // hard core against the nucleons accepted in earlier rounds: wave
         // nu + 2*sub checks every candidate against half of them
         const int acc0 = s_acc[nu];
@@ -162,13 +169,14 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
           const int4* py4 = reinterpret_cast<const int4*>(py[nu]);
           const float4* pz4 = reinterpret_cast<const float4*>(pz[nu]);
           int j = j0;
-          for (; j + 4 <= j1 && ok; j += 4) {
-            const int4 ax = px4[j >> 2], ay = py4[j >> 2];
-            const float4 az = pz4[j >> 2];
-            const int axs[4] = {ax.x, ax.y, ax.z, ax.w}, ays[4] = {ay.x, ay.y, ay.z, ay.w};
-            const float azs[4] = {az.x, az.y, az.z, az.w};
+          for (; j + 8 <= j1 && ok; j += 8) {
+            const int4 ax0 = px4[j >> 2], ay0 = py4[j >> 2], ax1 = px4[(j >> 2) + 1], ay1 = py4[(j >> 2) + 1];
+            const float4 az0 = pz4[j >> 2], az1 = pz4[(j >> 2) + 1];
+            const int axs[8] = {ax0.x, ax0.y, ax0.z, ax0.w, ax1.x, ax1.y, ax1.z, ax1.w};
+            const int ays[8] = {ay0.x, ay0.y, ay0.z, ay0.w, ay1.x, ay1.y, ay1.z, ay1.w};
+            const float azs[8] = {az0.x, az0.y, az0.z, az0.w, az1.x, az1.y, az1.z, az1.w};
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
+            for (int u = 0; u < 8; ++u) {
               const float dx = (float)(xq - axs[u]) * kQf, dy = (float)(yq - ays[u]) * kQf;
               const float dz = z - azs[u];
               if (dx * dx + dy * dy + dz * dz < cfg.dmin2) ok = false;
@@ -188,16 +196,38 @@ __global__ __launch_bounds__(256) void k_event(DevCfg cfg, EvBufs buf, uint2 key
           const float z = cqz[nu][lane];
           // conflicts with the earlier candidates of this round: bit l of cm
           unsigned long long cm = 0ull;
-          for (int l = 0; l < 64; ++l) {
-            const float dx = (float)(xq - cqx[nu][l]) * kQf, dy = (float)(yq - cqy[nu][l]) * kQf;
-            const float dz = z - cqz[nu][l];
-            if (l < lane && dx * dx + dy * dy + dz * dz < cfg.dmin2) cm |= 1ull << l;
+          const int4* cx4 = reinterpret_cast<const int4*>(cqx[nu]);
+          const int4* cy4 = reinterpret_cast<const int4*>(cqy[nu]);
+          const float4* cz4 = reinterpret_cast<const float4*>(cqz[nu]);
+#pragma unroll 4
+          for (int l4 = 0; l4 < 16; ++l4) {
+            const int4 bx = cx4[l4], by = cy4[l4];
+            const float4 bz = cz4[l4];
+            const int bxs[4] = {bx.x, bx.y, bx.z, bx.w}, bys[4] = {by.x, by.y, by.z, by.w};
+            const float bzs[4] = {bz.x, bz.y, bz.z, bz.w};
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int l = 4 * l4 + u;
+              const float dx = (float)(xq - bxs[u]) * kQf, dy = (float)(yq - bys[u]) * kQf;
+              const float dz = z - bzs[u];
+              if (l < lane && dx * dx + dy * dy + dz * dz < cfg.dmin2) cm |= 1ull << l;
+            }
           }
-          // sequential acceptance in list order on wave-uniform masks
+          // sequential acceptance in list order on wave-uniform masks; with
+          // no conflict inside the round it is the first ok candidates
           const unsigned long long okm = s_okm[nu] & s_okm[nu + 2];
           const int An = NN[nu];
           unsigned long long accm = 0ull;
           int nacc = acc0;
+          if (__ballot(cm != 0ull) == 0ull) {
+            accm = okm;
+            int nok = __popcll(okm);
+            while (nok > An - acc0) {  // keep the first An - acc0
+              accm &= ~(1ull << (63 - __clzll(accm)));
+              --nok;
+            }
+            nacc = acc0 + nok;
+          } else
           for (int l = 0; l < 64 && nacc < An; ++l) {
             if (!((okm >> l) & 1ull)) continue;
             const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)cm, l);
