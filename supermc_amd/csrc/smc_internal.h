@@ -81,7 +81,8 @@ struct EvBufs {
   int* wound;      // [ev][2][kAMax] participant rank (wound order) or -1
   unsigned* pairs; // [ev][max_pairs] proj index | targ index << 16
   float* pfluct;   // [ev][max_pairs]
-  int* counts;     // [ev][8]: npartA, npartB, ncoll, status, attempts, nsrc
+  int* counts;     // [ev][8]: npartA, npartB, ncoll, status, attempts, nsrc,
+                   //   source rectangle x0 | x1 << 16, y0 | y1 << 16 (cells)
   double* bval;    // [ev]
   int* raw;        // [ev][2][kAMax][2] fed positions (quanta), input order
   int4* src;       // [ev][src_cap] {x_h, y_h, fluct bits, kind}

@@ -36,6 +36,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   __shared__ int s_nc[2], s_wc[2][4];
   __shared__ unsigned long long skey[2][kAMax];  // sort keys
   __shared__ int wrank[kAMax];
+  __shared__ int s_bb[4];
   __shared__ unsigned long long s_okm[4];
   __shared__ int sx[2][kAMax], sy[2][kAMax];  // sorted
   __shared__ unsigned hits[kAMax][kAMax / 32];
@@ -369,14 +370,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   int4* src = buf.src + (size_t)e * buf.src_cap;
   const bool src_ok = nsrc_part + ncoll <= buf.src_cap;
   int4* swin = buf.swin + (size_t)e * buf.src_cap;
+  // bounding rectangle of every source window: all grid cells outside it
+  // are zero for every density of this event
+  int bb[4] = {0x7FFFFFFF, -1, 0x7FFFFFFF, -1};  // x0, x1, y0, y1
+  auto grow = [&bb](int4 w, bool box) {
+    const int v[4] = {w.z, w.w, box ? w.x : w.z, box ? w.y : w.w};
+    for (int q = 0; q < 4; q += 2) {
+      const int x0 = v[q] & 0xffff, x1 = v[q] >> 16, y0 = v[q + 1] & 0xffff, y1 = v[q + 1] >> 16;
+      if (x0 < x1 && y0 < y1) {
+        bb[0] = min(bb[0], x0);
+        bb[1] = max(bb[1], x1);
+        bb[2] = min(bb[2], y0);
+        bb[3] = max(bb[3], y1);
+      }
+    }
+  };
+  if (t < 4) s_bb[t] = (t & 1) ? -1 : 0x7FFFFFFF;
   if (src_ok) {
     if (woundA >= 0) {
       src[woundA] = make_int4(2 * sx[0][t], 2 * sy[0][t], __float_as_int(fA), kSrcPartA);
-      swin[woundA] = src_windows(cfg, 2 * sx[0][t], 2 * sy[0][t], true);
+      const int4 w = src_windows(cfg, 2 * sx[0][t], 2 * sy[0][t], true);
+      swin[woundA] = w;
+      grow(w, true);
     }
     if (woundB >= 0) {
       src[npa + woundB] = make_int4(2 * sx[1][t], 2 * sy[1][t], __float_as_int(fB), kSrcPartB);
-      swin[npa + woundB] = src_windows(cfg, 2 * sx[1][t], 2 * sy[1][t], true);
+      const int4 w = src_windows(cfg, 2 * sx[1][t], 2 * sy[1][t], true);
+      swin[npa + woundB] = w;
+      grow(w, true);
     }
   }
   if (ncoll > cfg.max_pairs || !src_ok) status |= kStPairOverflow;
@@ -406,9 +427,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     if (src_ok) {
       const int xh = sx[0][i] + sx[1][j], yh = sy[0][i] + sy[1][j];
       src[nsrc_part + k] = make_int4(xh, yh, __float_as_int(fp), kSrcPair);
-      swin[nsrc_part + k] = src_windows(cfg, xh, yh, false);
+      const int4 w = src_windows(cfg, xh, yh, false);
+      swin[nsrc_part + k] = w;
+      grow(w, false);
     }
   }
+  if (!src_ok) {  // windows not stored: the whole grid
+    bb[0] = bb[2] = 0;
+    bb[1] = cfg.nx;
+    bb[3] = cfg.ny;
+  }
+  atomicMin(&s_bb[0], bb[0]);
+  atomicMax(&s_bb[1], bb[1]);
+  atomicMin(&s_bb[2], bb[2]);
+  atomicMax(&s_bb[3], bb[3]);
+  __syncthreads();
   if (t == 0) {
     int* c = buf.counts + (size_t)e * 8;
     c[0] = npa;
@@ -417,6 +450,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     c[3] = status;
     c[4] = attempt + 1;
     c[5] = src_ok ? nsrc_part + ncoll : 0;
+    const bool none = s_bb[0] >= s_bb[1];
+    c[6] = none ? 0 : (s_bb[0] | (s_bb[1] << 16));
+    c[7] = none ? 0 : (s_bb[2] | (s_bb[3] << 16));
     if (!fed) buf.bval[e] = s_b;
   }
 }

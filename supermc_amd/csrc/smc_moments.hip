@@ -94,14 +94,25 @@ __device__ __forceinline__ void moment_cell(MomAcc<NORD>& M, int from, int to, d
   }
 }
 
-// Loads row ix of the grid, MAXC values per lane (columns lane + 64 j).
+// Rectangle [x0, x1) x [y0, y1) of grid cells that holds every non-zero
+// cell of event e (the union of its source windows, from k_event).
+struct Rect {
+  int x0, x1, y0, y1;
+};
+__device__ __forceinline__ Rect event_rect(const EvBufs& buf, int e) {
+  const int a = buf.counts[(size_t)e * 8 + 6], b = buf.counts[(size_t)e * 8 + 7];
+  return Rect{a & 0xffff, a >> 16, b & 0xffff, b >> 16};
+}
+
+// Loads row ix of the grid inside the rectangle, MAXC values per lane
+// (columns y0 + lane + 64 j); zero outside it.
 template <int MAXC>
-__device__ __forceinline__ void load_row(const float* g, int ix, int nx, int ny, float (&v)[MAXC]) {
+__device__ __forceinline__ void load_row(const float* g, int ix, const Rect& R, int ny, float (&v)[MAXC]) {
   const int lane = threadIdx.x & 63;
 #pragma unroll
   for (int j = 0; j < MAXC; ++j) {
-    const int iy = lane + 64 * j;
-    v[j] = (ix < nx && iy < ny) ? g[(size_t)ix * ny + iy] : 0.f;
+    const int iy = R.y0 + lane + 64 * j;
+    v[j] = (ix < R.x1 && iy < R.y1) ? g[(size_t)ix * ny + iy] : 0.f;
   }
 }
 
@@ -202,6 +213,7 @@ __device__ void pass2_generic(const DevCfg& cfg, const EvBufs& buf, int e, const
   const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
   const double ff = cfg.final_factor;
   const int nwy = (ny + 31) >> 5;
+  const Rect R = event_rect(buf, e);
   MomAcc<NORD> M;
 #pragma unroll
   for (int n = 0; n < 10; ++n) M.rn[n] = 0.0;
@@ -210,15 +222,15 @@ __device__ void pass2_generic(const DevCfg& cfg, const EvBufs& buf, int e, const
   const int from = cfg.from_order, to = cfg.to_order;
   {
     float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, wv, nx, ny, cur);
-    for (int ix = wv; ix < nx; ix += NW) {
-      load_row<MAXC>(g, ix + NW, nx, ny, nxt);
+    load_row<MAXC>(g, R.x0 + wv, R, ny, cur);
+    for (int ix = R.x0 + wv; ix < R.x1; ix += NW) {
+      load_row<MAXC>(g, ix + NW, R, ny, nxt);
       const double x = cfg.xmin + ix * cfg.dx - xc;
       const unsigned* row = mask + (size_t)ix * nwy;
 #pragma unroll
       for (int j = 0; j < MAXC; ++j) {
         if (cur[j] != 0.f) {
-          const int iy = lane + 64 * j;
+          const int iy = R.y0 + lane + 64 * j;
           const double y = cfg.ymin + iy * cfg.dy - yc;
           const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
           moment_cell<NORD>(M, from, to, (double)cur[j] * ff, x, y, masked);
@@ -283,21 +295,22 @@ __device__ void pass2_fixed(const DevCfg& cfg, const EvBufs& buf, int e, const f
   const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
   const double ff = cfg.final_factor;
   const int nwy = (ny + 31) >> 5;
+  const Rect R = event_rect(buf, e);
   constexpr int NV = 10 + 1 + 5 * NORD;  // rn[10], r^2 norm, per order re/im (r^2), re/im/norm (r^n)
   double a[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) a[i] = 0.0;
   {
     float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, wv, nx, ny, cur);
-    for (int ix = wv; ix < nx; ix += NW) {
-      load_row<MAXC>(g, ix + NW, nx, ny, nxt);
+    load_row<MAXC>(g, R.x0 + wv, R, ny, cur);
+    for (int ix = R.x0 + wv; ix < R.x1; ix += NW) {
+      load_row<MAXC>(g, ix + NW, R, ny, nxt);
       const double x = cfg.xmin + ix * cfg.dx - xc;
       const unsigned* row = mask + (size_t)ix * nwy;
 #pragma unroll
       for (int j = 0; j < MAXC; ++j) {
         if (cur[j] != 0.f) {
-          const int iy = lane + 64 * j;
+          const int iy = R.y0 + lane + 64 * j;
           const double y = cfg.ymin + iy * cfg.dy - yc;
           const double d = (double)cur[j] * ff;
           const double r2 = fma(x, x, y * y);
@@ -380,20 +393,21 @@ __device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const 
   const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
   const double ff = cfg.final_factor;
   const int nwy = (ny + 31) >> 5;
+  const Rect R = event_rect(buf, e);
 
   // pass 1: centre of mass (MakeDensity.cpp:2273-2282), rows prefetched
   if (!have_s1) {
     float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, wv, nx, ny, cur);
-    for (int ix = wv; ix < nx; ix += NW) {
-      load_row<MAXC>(g, ix + NW, nx, ny, nxt);
+    load_row<MAXC>(g, R.x0 + wv, R, ny, cur);
+    for (int ix = R.x0 + wv; ix < R.x1; ix += NW) {
+      load_row<MAXC>(g, ix + NW, R, ny, nxt);
       const double xg = cfg.xmin + ix * cfg.dx;
       double rs = 0.0, ys = 0.0;
 #pragma unroll
       for (int j = 0; j < MAXC; ++j) {
         const double d = (double)cur[j] * ff;
         rs += d;
-        ys += (cfg.ymin + (lane + 64 * j) * cfg.dy) * d;
+        ys += (cfg.ymin + (R.y0 + lane + 64 * j) * cfg.dy) * d;
       }
       s1[0] += rs;
       s1[1] += xg * rs;
@@ -416,11 +430,12 @@ __device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const 
 
 // Sum of the grid only (dndy, MCnucl.cpp:817-820), no finalFactor.
 template <int NW>
-__device__ void grid_sum(const DevCfg& cfg, int e, const float* g, double* evout, MomLds& L) {
+__device__ void grid_sum(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout, MomLds& L) {
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   double s[1] = {0.0};
-  for (int ix = wv; ix < cfg.nx; ix += NW)
-    for (int iy = lane; iy < cfg.ny; iy += 64) s[0] += (double)g[(size_t)ix * cfg.ny + iy];
+  const Rect R = event_rect(buf, e);
+  for (int ix = R.x0 + wv; ix < R.x1; ix += NW)
+    for (int iy = R.y0 + lane; iy < R.y1; iy += 64) s[0] += (double)g[(size_t)ix * cfg.ny + iy];
   block_sums<NW, 1>(s, L);
   if (t == 0) evout[(size_t)e * SMC_EV_STRIDE + SMC_EV_TOTAL] = L.tot[0];
 }
@@ -435,7 +450,7 @@ __global__ __launch_bounds__(kGT) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 :
   const float* g = grid + (size_t)e * cfg.nx * cfg.ny;
   double s1[3] = {0.0, 0.0, 0.0};
   if (flags & 2) grid_moments<kGW, NORD, MAXC, FIX25>(cfg, buf, e, g, evout, L, dyn_mask, false, s1);
-  else grid_sum<kGW>(cfg, e, g, evout, L);
+  else grid_sum<kGW>(cfg, buf, e, g, evout, L);
 }
 
 // Differentiable batches: per-event FP64 scalars for |eps_2|, |eps_3| and

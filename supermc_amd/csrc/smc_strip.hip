@@ -169,8 +169,11 @@ __global__ __launch_bounds__(64) void k_tile_exact(DevCfg cfg, EvBufs buf, Smear
   float acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+  // tiles outside the event's source rectangle stay zero
+  const int rx = buf.counts[(size_t)e * 8 + 6], ry = buf.counts[(size_t)e * 8 + 7];
+  const bool live = (rx & 0xffff) < r0 + 16 && (rx >> 16) > r0 && (ry & 0xffff) < c0 + 64 && (ry >> 16) > c0;
 
-  for (int sub = 0; sub < K; sub += 64) {
+  for (int sub = 0; live && sub < K; sub += 64) {
     const int k = sub + lane;
     bool sel = false;
     StripEntry en{0, 0, 0.f, 0, 0, 0};
