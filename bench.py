@@ -21,6 +21,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -65,6 +67,58 @@ def cpu_baseline(seconds):
     return dict(value=tot / wall, unit="events/s", cores=cores, kind="port",
                 sample=f"{tot} Pb+Pb events ({per_core} per core) through the FP64 oracle event loop "
                        f"(sample -> collide -> T_A/T_B -> density -> eps_n, no file output), {wall:.1f} s wall")
+
+
+def kln_params(nev):
+    """MC-KLN Pb+Pb at sqrt(s) = 2.76 TeV, lambda = 0.138, T_A/T_B from
+    Gaussian participants (w = 0.5 fm), dN/dy table (tmax 71, subdivision 3)
+    with the 32 x 32 x 48 Gauss-Legendre quadrature, 261^2 cells."""
+    from supermc_amd import default_params
+    return default_params(
+        which_mc_model=1, sub_model=7, ecm=2760.0, lambda_=0.138, tmax=71, tmax_subdivision=3,
+        shape_of_nucleons=4, gauss_nucl_width=0.5, shape_of_entropy=2, collision_criterion=1, Aproj=208,
+        Atarg=208, proj_ws_R=6.62, proj_ws_a=0.546, targ_ws_R=6.62, targ_ws_a=0.546, nucleon_min_dist=0.4,
+        sigma_nn_mb=64.0, cc_fluctuation_model=0, bmin=0.0, bmax=20.0, Npmin=2, Npmax=500, maxx=13.0,
+        maxy=13.0, dx=0.1, dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, kln_n_pt=32,
+        kln_n_kt=32, kln_n_phi=48, max_events=nev, max_pairs=4096)
+
+
+def time_kln(nev, steps, seed):
+    from supermc_amd import Context
+    with Context(kln_params(nev)) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)  # warm-up, builds the dN/dy table
+        ctx.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ev = ctx.fetch_events(nev)
+        ker = {name: ctx.timing(kid)[0] / max(ctx.timing(kid)[1], 1)
+               for kid, name in ((0, "k_event"), (1, "k_smear(T_A,T_B)"), (3, "k_kln_table"), (2, "k_moments"))}
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, kernels_avg_ms=ker,
+                failed_events=int((ev[:, 57] != 0).sum()), mean_eps2=float(np.hypot(ev[:, 2], ev[:, 12]).mean()))
+
+
+def time_adjoint(nev, steps, seed):
+    """Differentiable Pb+Pb batch: forward (|eps_2|, |eps_3|, dS/dy) plus
+    the adjoint d/d(w, R, a, alpha, k), per event."""
+    from supermc_amd import Context
+    with Context(bench_params(nev)) as ctx:
+        th0 = ctx.diff_setup(seed, 0, nev)
+        cot = np.tile([1.0 / nev, 1.0 / nev, 0.0], (nev, 1))
+        ctx.diff_forward(th0)
+        ctx.diff_backward(cot)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.diff_forward(th0)
+            g = ctx.diff_backward(cot)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps,
+                grad_eps2_plus_eps3=dict(zip(("w", "R", "a", "alpha", "k"), map(float, g))))
 
 
 def footprint_cells(w, dx, xmin, seed=1, n=4000):
@@ -128,6 +182,10 @@ def main():
                     help="0 exact footprint (separable VALU tiles), 1 MFMA Gx.diag(w).Gy^T, 2 exact (per-cell exp)")
     ap.add_argument("--parity-events", type=int, default=12,
                     help="events of the last batch re-evaluated by the FP64 oracle for the eps_n error")
+    ap.add_argument("--kln-events", type=int, default=20000, help="MC-KLN events per step (0: skip)")
+    ap.add_argument("--adjoint-events", type=int, default=4096, help="differentiable batch size (0: skip)")
+    ap.add_argument("--only", choices=("kln", "adjoint"), default=None,
+                    help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
     args = ap.parse_args()
@@ -140,6 +198,17 @@ def main():
     load()  # fails loudly when libsmc.so is missing
     if args.copy_probe:
         copy_probe()
+    if args.only:
+        if args.only == "kln":
+            r = time_kln(args.kln_events, args.steps, args.seed)
+            metric, wl = "events/s MC-KLN Pb+Pb 2.76 TeV 261^2 grid", "MC-KLN Pb+Pb, dN/dy table, 261x261"
+        else:
+            r = time_adjoint(args.adjoint_events, args.steps, args.seed)
+            metric, wl = "events/s differentiable Pb+Pb 261^2 (forward + adjoint)", "frozen-topology Pb+Pb batch"
+        print(json.dumps({"metric": metric, "value": r["events_per_s"], "unit": "events/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": 1, "higher_is_better": True, "dtype": "f32",
+                          "data": "synthetic", "config": dict(workload=wl, **r)}), flush=True)
+        return
     nev = args.events
     ctx = Context(bench_params(nev, args.smear_path))
     print(f"[bench] Pb+Pb 261^2, {nev} events/step, warmup {args.warmup}, steps {args.steps}", file=sys.stderr,
@@ -196,6 +265,9 @@ def main():
 
     eps_err = eps_error_vs_fp64(ctx, ev, args.parity_events) if args.parity_events > 0 else None
 
+    kln = time_kln(args.kln_events, 2, args.seed) if args.kln_events > 0 else None
+    adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
@@ -218,7 +290,8 @@ def main():
                    "mean_sources_per_event": sum_k / nev, "failed_events": bad,
                    "smear_path": ["exact separable VALU tiles", "MFMA Gx.diag(w).Gy^T", "exact per-cell exp tiles"][args.smear_path],
                    "eps_err_vs_fp64": eps_err,
-                   "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()}},
+                   "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
+                   "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

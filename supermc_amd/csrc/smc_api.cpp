@@ -575,7 +575,7 @@ void run_kln_density(smc_ctx* c) {
                                    c->kln_kc, c->stream),
                  "k_kln_direct");
   } else {
-    check_launch(launch_kln_table(a, b, s, (int)ncell_ev, c->nev, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
+    check_launch(launch_kln_table(a, b, s, (int)ncell_ev, c->nev, c->d.ny, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
                                   c->buf.counts, c->stream),
                  "k_kln_table");
   }
@@ -651,6 +651,7 @@ void diff_setup(smc_ctx* c, uint64_t seed, uint64_t event0, int nev) {
   d.mask = dif_alloc<unsigned>(c, 9, E * c->d.nx * nwy);
   d.gsrc = dif_alloc<double4>(c, 10, E * SC);
   d.gev = dif_alloc<double>(c, 11, E * 5);
+  d.wstab = dif_alloc<double>(c, 13, 2049);
   std::vector<double> x, w;
   gauss_legendre01(64, x, w);
   std::vector<double> glh(128);

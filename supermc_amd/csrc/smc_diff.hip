@@ -43,15 +43,23 @@ __device__ inline WsInt ws_integrals(double r, double R, double a, const double2
   return s;
 }
 
-// r = F^{-1}(u; R, a) on [0, R + 2.5] and dr/dR, dr/da.
+// r = F^{-1}(u; R, a) on [0, R + 2.5] and dr/dR, dr/da.  tab (optional):
+// r at v = u^(1/3) = i/(kWsTab-1) for this (R, a), the Newton start.
+constexpr int kWsTab = 2049;
 __device__ inline void ws_finv(double u, double R, double a, const double2* gl, int ngl, double& r, double& rR,
-                               double& ra) {
+                               double& ra, const double* tab = nullptr) {
   const double rmax = R + 2.5;
   const WsInt Z = ws_integrals(rmax, R, a, gl, ngl);
   const double rhomax = 1.0 / (1.0 + exp(2.5 / a));
   const double ZR = Z.nR + rmax * rmax * rhomax, Za = Z.na;
   const double target = u * Z.n;
   double lo = 0.0, hi = rmax, x = rmax * cbrt(u);
+  if (tab) {
+    const double v = cbrt(u) * (kWsTab - 1);
+    const int i = min((int)v, kWsTab - 2);
+    const double f = v - i;
+    x = tab[i] + f * (tab[i + 1] - tab[i]);
+  }
   for (int it = 0; it < 100; ++it) {
     const WsInt N = ws_integrals(x, R, a, gl, ngl);
     const double f = N.n - target;
@@ -142,6 +150,16 @@ __device__ __forceinline__ double diff_shape(int kind, double alpha, double kg) 
   return (kind == kSrcPair) ? (alpha + 1e-8) * kg : (1.0 - alpha + 1e-8) * 0.5 * kg;
 }
 
+// The radial quantile at v = u^(1/3) = i/(kWsTab-1), i < kWsTab.
+__global__ __launch_bounds__(256) void k_ws_table(double R, double a, const double2* gl, int ngl, double* tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= kWsTab) return;
+  const double v = (double)i / (kWsTab - 1);
+  double r, rR, ra;
+  ws_finv(v * v * v, R, a, gl, ngl, r, rR, ra);
+  tab[i] = r;
+}
+
 // ------------------------------------------------------------- topology
 // One workgroup per pending event: b, then each nucleus (one lane each)
 // from candidates (u, cos theta, phi) in stream order with the hard core,
@@ -180,7 +198,7 @@ __global__ __launch_bounds__(64) void k_diff_topo(DevCfg cfg, EvBufs buf, DiffBu
       const double c = 2.0 * u53(r2.x, r2.y) - 1.0;
       const double ph = 2.0 * M_PI * u53(r2.z, r2.w);
       double r, rR, ra;
-      ws_finv(u, R0, a0, d.gl, d.ngl, r, rR, ra);
+      ws_finv(u, R0, a0, d.gl, d.ngl, r, rR, ra, d.wstab);
       const double st = sqrt(fmax(0.0, 1.0 - c * c));
       const double x = r * st * cos(ph), y = r * st * sin(ph), z = r * c;
       bool good = true;
@@ -253,7 +271,7 @@ __global__ __launch_bounds__(256) void k_diff_pos(DevCfg cfg, DiffBufs d, double
     if (t < A) {
       const double* du = d.du + (((size_t)e * 2 + nu) * kAMax + t) * 4;
       double r, rR, ra;
-      ws_finv(du[0], R, a, d.gl, d.ngl, r, rR, ra);
+      ws_finv(du[0], R, a, d.gl, d.ngl, r, rR, ra, d.wstab);
       const double st = sqrt(fmax(0.0, 1.0 - du[1] * du[1]));
       const double nx = st * cos(du[2]), ny = st * sin(du[2]);
       v[0] = r * nx;
@@ -498,6 +516,8 @@ __global__ __launch_bounds__(256) void k_diff_chain(DevCfg cfg, EvBufs buf, Diff
 int launch_diff_topo(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
                      unsigned long long event0, double R0, double a0, void* stream) {
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+  hipLaunchKernelGGL(k_ws_table, dim3((kWsTab + 255) / 256), dim3(256), 0, (hipStream_t)stream, R0, a0, d.gl, d.ngl,
+                     d.wstab);
   hipLaunchKernelGGL(k_diff_topo, dim3(nev), dim3(64), 0, (hipStream_t)stream, cfg, b, d, key, event0, R0, a0);
   return (int)hipGetLastError();
 }
@@ -506,6 +526,8 @@ int launch_diff_forward_pre(const DevCfg& cfg, const EvBufs& b, const DiffBufs& 
                             unsigned long long event0, const double* theta, int fluct, void* stream) {
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
   hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_ws_table, dim3((kWsTab + 255) / 256), dim3(256), 0, st, theta[1], theta[2], d.gl, d.ngl,
+                     d.wstab);
   hipLaunchKernelGGL(k_diff_pos, dim3(nev), dim3(256), 0, st, cfg, d, theta[1], theta[2]);
   hipLaunchKernelGGL(k_diff_src, dim3(nev), dim3(256), 0, st, cfg, b, d, key, event0, theta[3], theta[4], fluct);
   return (int)hipGetLastError();

@@ -113,6 +113,7 @@ struct DiffBufs {
   double* gev;        // [ev][5] dL/dtheta per event
   const double2* gl;  // Gauss-Legendre nodes/weights on [0, 1]
   int ngl;
+  double* wstab;      // radial quantile table of the current (R, a): Newton starts
 };
 // per-event scalars of the differentiable moments
 enum : int {
@@ -162,7 +163,7 @@ int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell
                       const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
 int launch_kln_points(const double* ta, const double* tb, double* out, int n, const void* ptd,
                       const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
-int launch_kln_table(const float* ta, const float* tb, float* out, int ncell_ev, int nev,
+int launch_kln_table(const float* ta, const float* tb, float* out, int ncell_ev, int nev, int ny,
                      const double* table, int tmax, double dT, int* counts, void* stream);
 
 }  // namespace smc

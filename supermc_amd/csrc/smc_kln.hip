@@ -87,9 +87,19 @@ __global__ __launch_bounds__(256) void k_kln_points(const double* ta, const doub
 // MCnucl.cpp:654-687: status bit set when a cell leaves the table (exit(1)
 // in the reference).
 __global__ __launch_bounds__(256) void k_kln_table(const float* ta, const float* tb, float* out, int ncell_ev,
-                                                   int nev, const double* table, int tmax, double dT, int* counts) {
+                                                   int nev, int ny, const double* table, int tmax, double dT,
+                                                   int* counts) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)ncell_ev * nev) return;
+  // cells outside the event's source rectangle have T_A = T_B = 0, and the
+  // table's row and column 0 are 0
+  const size_t e = i / ncell_ev;
+  const int c = (int)(i - e * ncell_ev), ix = c / ny, iy = c - ix * ny;
+  const int rx = counts[e * 8 + 6], ry = counts[e * 8 + 7];
+  if (ix < (rx & 0xffff) || ix >= (rx >> 16) || iy < (ry & 0xffff) || iy >= (ry >> 16)) {
+    out[i] = 0.f;
+    return;
+  }
   const double di = (double)ta[i] / dT, dj = (double)tb[i] / dT;
   if (!(di >= 0 && di < tmax - 2) || !(dj >= 0 && dj < tmax - 2)) {  // also catches NaN
     out[i] = 0.f;
@@ -126,12 +136,12 @@ int launch_kln_points(const double* ta, const double* tb, double* out, int n, co
   return (int)hipGetLastError();
 }
 
-int launch_kln_table(const float* ta, const float* tb, float* out, int ncell_ev, int nev, const double* table,
+int launch_kln_table(const float* ta, const float* tb, float* out, int ncell_ev, int nev, int ny, const double* table,
                      int tmax, double dT, int* counts, void* stream) {
   const size_t n = (size_t)ncell_ev * nev;
   if (n == 0) return 0;
   hipLaunchKernelGGL(k_kln_table, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ta, tb, out,
-                     ncell_ev, nev, table, tmax, dT, counts);
+                     ncell_ev, nev, ny, table, tmax, dT, counts);
   return (int)hipGetLastError();
 }
 
