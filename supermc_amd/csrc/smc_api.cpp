@@ -105,6 +105,24 @@ HostTable build_steps(const std::function<long long(long long)>& f, long long pm
 
 }  // namespace
 
+// Scales of the tile smearing's flush-to-zero disk cut.  With the cut the
+// product (K gx) gy reaches the smallest normal float 2^-126 exactly at
+// gx gy = exp2(-cut2 c2), c2 = inv2w2 log2 e as the kernel forms it;
+// weights carry 2^64 so that w p stays normal.  Without a cut (cut2
+// beyond ~100 decades of the Gaussian) K = 2^64 and nothing flushes.
+void smc::set_tile_scales(DevCfg& g) {
+  const float c2 = g.inv2w2 * 1.4426950408889634f;
+  const double arg = (double)g.cut2 * (double)c2;
+  if (arg < 100.0) {
+    g.tile_k = (float)std::ldexp(std::exp2(arg), -126);
+    g.tile_w = 0x1p64f;
+  } else {
+    g.tile_k = 0x1p64f;
+    g.tile_w = 1.f;
+  }
+  g.tile_out = (float)(1.0 / ((double)g.tile_k * (double)g.tile_w));
+}
+
 struct smc_ctx {
   smc_params p{};
   smc_derived d{};
@@ -302,6 +320,7 @@ void validate_and_derive(smc_ctx* c) {
   g.inv2w2 = (float)(1.0 / (2 * w * w));
   g.norm = (float)(1 / (2 * M_PI * w * w));
   g.cut2 = (float)(25. * w * w);
+  set_tile_scales(g);
   const double hb = 4 * std::max(w, p.quark_width);
   g.hb_h = (int)std::llround(hb / kH);
   g.alpha = (float)p.alpha;
@@ -510,17 +529,18 @@ SmearSpec thickness_spec(const smc_ctx* c) {
   return s;
 }
 
-// smear_path: 0 exact footprint, separable VALU tiles (16 x 64, one wave);
-// 1 MFMA strips Gx.diag(w).Gy^T (square window, no r <= 5w corner cut;
-// grids up to 272 wide, not for sqrt(T_A T_B)); 2 exact footprint, per-cell
-// exponentials (256-thread tiles).
+// smear_path: 0 exact footprint on VALU tiles (32 x 64, one wave, disk cut
+// by flushing denormal products); 1 MFMA strips Gx.diag(w).Gy^T (square
+// window, no r <= 5w corner cut; grids up to 272 wide, not for
+// sqrt(T_A T_B)); 2 exact footprint, per-cell exponentials (256-thread
+// tiles).
 void run_smear(smc_ctx* c, const SmearSpec& spec, float* out0, float* out1) {
   TimerScope ts(c, SMC_K_SMEAR);
   const int sp = c->p.smear_path;
   if (sp == 1 && c->d.ny <= 272 && spec.combine != 1)
     check_launch(launch_strip(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_strip");
   else if (sp != 2)
-    check_launch(launch_tile_exact(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile_exact");
+    check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
   else
     check_launch(launch_smear(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_smear");
   ts.done();
@@ -723,6 +743,7 @@ DevCfg diff_cfg(const smc_ctx* c, const double* th) {
   cw.inv2w2 = (float)(1.0 / (2.0 * th[0] * th[0]));
   cw.norm = (float)(1.0 / (2.0 * M_PI * th[0] * th[0]));
   cw.cut2 = 1e30f;
+  set_tile_scales(cw);
   return cw;
 }
 
@@ -749,8 +770,7 @@ void diff_forward(smc_ctx* c, const double* th) {
   t0.done();
   float* grid = grid_buffer(c, &c->grid);
   TimerScope t1(c, SMC_K_SMEAR);
-  check_launch(launch_tile_exact(diff_cfg(c, th), c->buf, diff_spec(c, th), nev, grid, nullptr, c->stream),
-               "k_tile_exact");
+  check_launch(launch_tile(diff_cfg(c, th), c->buf, diff_spec(c, th), nev, grid, nullptr, c->stream), "k_tile");
   t1.done();
   TimerScope t2(c, SMC_K_MOMENTS);
   const double ffdxdy = c->p.finalFactor * c->p.dx * c->p.dy;

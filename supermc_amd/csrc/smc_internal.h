@@ -66,6 +66,10 @@ struct DevCfg {
   int nx, ny;
   double xmin, ymin, dx, dy;
   float w, inv2w2, norm, cut2;  // width, 1/(2w^2), 1/(2 pi w^2), 25 w^2
+  // flush-to-zero disk cut of the tile smearing (smc_tile.hip): row factor
+  // scale K = 2^-126 / exp2(-cut2 * inv2w2 * log2 e), weight scale, and the
+  // output scale 1 / (K tile_w)
+  float tile_k, tile_w, tile_out;
   int hb_h;              // box half-width 4*max(w,qw) in half-quanta (rounded)
   float alpha, pref_part;
   int use_binary;        // alpha > 1e-8 (MCnucl.cpp:724)
@@ -149,8 +153,10 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
                    double* ev_out, int mode, void* stream);
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                  void* stream);
-int launch_tile_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0,
-                      float* out1, void* stream);
+int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
+                void* stream);
+int launch_combine_sqrt(float* a, const float* b, size_t n, void* stream);
+void set_tile_scales(DevCfg& g);
 int launch_diff_topo(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
                      unsigned long long event0, double R0, double a0, void* stream);
 int launch_diff_forward_pre(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,

@@ -1,5 +1,5 @@
-// smc_strip.hip -- separable Gaussian smearing: MFMA strips (square window)
-// and exact-footprint VALU tiles.
+// smc_strip.hip -- separable Gaussian smearing on MFMA strips (square
+// window; the exact footprint is smc_tile.hip's).
 //
 // The separable form of the Gaussian smearing (MCnucl.cpp:432-478 T_A/T_B,
 // :689-760 MC-Glauber, :762-811 sqrt(T_A T_B)):
@@ -137,115 +137,13 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NACC == 1 ? 
     }
 }
 
-// Exact-footprint smearing, one wavefront per 16 x 64 tile of a grid.
-// Each tile keeps its 16 rows x 64 columns in 16 accumulators per lane
-// (lane = column), selects the sources whose integer window meets the tile
-// (ordered ballot compaction, so the summation order is the source order),
-// and per source: lanes 0-15 compute the row factors {w gx(row),
-// 25 w^2 - dx^2} into LDS, every lane its column factor gy and dy^2, and each
-// cell takes w gx gy when dy^2 <= 25 w^2 - dx^2 (the disk cut of
-// MCnucl.cpp:450-475, :732-756, Particle.cpp:122-132).  Row factors reach
-// all lanes as LDS broadcasts, so a cell costs one compare, one select and
-// one FMA.
-__global__ __launch_bounds__(64) void k_tile_exact(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, float* out) {
-  __shared__ StripEntry wl[64];
-  __shared__ float4 rd[8];  // row factors of the current source, two rows per entry
-  const int lane = threadIdx.x;
-  const int nx = cfg.nx, ny = cfg.ny;
-  const int ntx = (nx + 15) >> 4;
-  const int r0 = 16 * (blockIdx.x % ntx), c0 = 64 * (blockIdx.x / ntx);
-  const int e = blockIdx.y;
-  const float c2 = cfg.inv2w2 * 1.4426950408889634f;
-  const float cut2 = cfg.cut2;
-  const int col = c0 + lane;
-  const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
-  const int myrow = r0 + (lane & 15);
-  const int rowh = (int)llrint((cfg.xmin + myrow * cfg.dx) / kH);
-  const int K = buf.counts[(size_t)e * 8 + 5];
-  const int4* src = buf.src + (size_t)e * buf.src_cap;
-  const int4* swin = buf.swin + (size_t)e * buf.src_cap;
-  float* rdf = reinterpret_cast<float*>(rd);
-
-  float acc[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.f;
-  // tiles outside the event's source rectangle stay zero
-  const int rx = buf.counts[(size_t)e * 8 + 6], ry = buf.counts[(size_t)e * 8 + 7];
-  const bool live = (rx & 0xffff) < r0 + 16 && (rx >> 16) > r0 && (ry & 0xffff) < c0 + 64 && (ry >> 16) > c0;
-
-  for (int sub = 0; live && sub < K; sub += 64) {
-    const int k = sub + lane;
-    bool sel = false;
-    StripEntry en{0, 0, 0.f, 0, 0, 0};
-    if (k < K) {
-      const int4 s = src[k];
-      const int kind = s.w;
-      if (spec.enabled[kind] && spec.acc[kind] == acc_sel) {
-        const int4 wv = swin[k];
-        en.xh = s.x;
-        en.yh = s.y;
-        en.xw = (spec.win[kind] == kWinBox) ? wv.x : wv.z;
-        en.yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
-        en.w = spec.scale[kind] * (spec.use_fluct[kind] ? __int_as_float(s.z) : 1.f);
-        const int xlo = en.xw & 0xffff, xhi = en.xw >> 16, ylo = en.yw & 0xffff, yhi = en.yw >> 16;
-        sel = xlo < r0 + 16 && xhi > r0 && ylo < c0 + 64 && yhi > c0;
-      }
-    }
-    const unsigned long long m = __ballot(sel);
-    if (sel) wl[__popcll(m & ((1ull << lane) - 1ull))] = en;
-    const int n = __popcll(m);
-    __builtin_amdgcn_wave_barrier();
-    for (int g = 0; g < n; ++g) {
-      const StripEntry q = wl[g];
-      const int xlo = q.xw & 0xffff, xhi = q.xw >> 16, ylo = q.yw & 0xffff, yhi = q.yw >> 16;
-      if (lane < 16) {
-        const float dx = (float)(rowh - q.xh) * kHf;
-        const float dx2 = dx * dx;
-        const float gxw = (myrow >= xlo && myrow < xhi) ? q.w * __builtin_amdgcn_exp2f(-dx2 * c2) : 0.f;
-        rdf[2 * lane] = gxw;
-        rdf[2 * lane + 1] = cut2 - dx2;
-      }
-      const float dy = (float)(ygh - q.yh) * kHf;
-      const float d2 = dy * dy;
-      const float gy = (col >= ylo && col < yhi) ? __builtin_amdgcn_exp2f(-d2 * c2) : 0.f;
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const float4 v = rd[i];
-        acc[2 * i] = fmaf(v.x, (d2 <= v.y) ? gy : 0.f, acc[2 * i]);
-        acc[2 * i + 1] = fmaf(v.z, (d2 <= v.w) ? gy : 0.f, acc[2 * i + 1]);
-      }
-      __builtin_amdgcn_wave_barrier();
-    }
-    __builtin_amdgcn_wave_barrier();
-  }
-  if (col >= ny) return;
-  const size_t N = (size_t)nx * ny;
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int ix = r0 + r;
-    if (ix < nx) out[(size_t)e * N + (size_t)ix * ny + col] = acc[r];
-  }
-}
-
 __global__ void k_combine_sqrt(float* a, const float* b, size_t n) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) a[i] = sqrtf(a[i] * b[i]);
 }
 
-int launch_tile_exact(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
-                       void* stream) {
-  if (nev <= 0) return 0;
-  const dim3 grid(((cfg.nx + 15) / 16) * ((cfg.ny + 63) / 64), nev);
-  hipStream_t st = (hipStream_t)stream;
-  hipLaunchKernelGGL(k_tile_exact, grid, dim3(64), 0, st, cfg, b, s, 0, out0);
-  if (s.combine != 0) {
-    hipLaunchKernelGGL(k_tile_exact, grid, dim3(64), 0, st, cfg, b, s, 1, out1);
-    if (s.combine == 1) {
-      const size_t n = (size_t)nev * cfg.nx * cfg.ny;
-      hipLaunchKernelGGL(k_combine_sqrt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, out0, out1, n);
-    }
-  }
+int launch_combine_sqrt(float* a, const float* b, size_t n, void* stream) {
+  hipLaunchKernelGGL(k_combine_sqrt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a, b, n);
   return (int)hipGetLastError();
 }
 

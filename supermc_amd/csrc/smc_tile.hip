@@ -1,0 +1,172 @@
+// smc_tile.hip -- exact-footprint Gaussian smearing on VALU tiles.
+//
+// The smearing of MCnucl.cpp:432-478 (T_A / T_B), :689-760 (MC-Glauber
+// entropy) and :762-811 (sqrt(T_A T_B)) adds, for every source k inside its
+// integer footprint window and inside the disk dx^2 + dy^2 <= 25 w^2
+// (Particle.cpp:122-132),
+//     T[x][y] += w_k gx_k(x) gy_k(y),   g(d) = exp(-d^2 / (2 w^2)).
+//
+// The disk cut without a compare.  Inside the disk gx gy >= theta =
+// exp(-cut2 / (2 w^2)), outside it gx gy < theta.  The row factor is stored
+// as K gx with K = 2^-126 / theta, so the product p = (K gx) gy is a normal
+// float exactly when the cell is inside the disk and a denormal otherwise.
+// This file is compiled with f32 denormals flushed (the kernel's MODE
+// register flushes the outputs of v_mul_f32), so cells outside the disk
+// get p = 0 and every cell costs one multiply and one FMA:
+//     acc += p * (w_k 2^64),    T = acc * 2^-64 / K
+// (2^64 keeps w_k p far above the denormal range for any w_k > 2^-64).
+// The boundary is that of the product in float, within a few ulp of the
+// exact disk.  Without a cut (cut2 = 1e30) K = 2^64, nothing flushes and
+// the footprint is the integer window alone.
+//
+// One wavefront owns a tile of TR rows x 64 columns in TR accumulators per
+// lane (lane = column).  It selects the sources whose window meets the tile
+// in source order (ballot compaction, so the summation order is fixed),
+// and per selected source: lanes 0..TR-1 write K gx(row) into LDS, every
+// lane computes gy(column), and groups of four rows that the source's
+// window misses are skipped by a uniform branch.
+#include "smc_device.h"
+
+namespace smc {
+
+struct TileEntry {
+  int xh, yh;
+  float w;
+  int xw, yw;
+  int pad;
+};
+
+template <int TR, int CH, bool PAIR>
+__global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, float* out) {
+  static_assert(TR % CH == 0 && CH % 4 == 0 && TR <= 64 && (!PAIR || TR <= 32), "tile rows");
+  __shared__ TileEntry wl[64];
+  __shared__ float4 rd[2][TR / 4];  // K gx(row) of the current source(s), 0 outside the window
+  const int lane = threadIdx.x;
+  const int nx = cfg.nx, ny = cfg.ny;
+  const int ntx = (nx + TR - 1) / TR;
+  const int r0 = TR * (blockIdx.x % ntx), c0 = 64 * (blockIdx.x / ntx);
+  const int e = blockIdx.y;
+  // exp2 argument -(d_h)^2 kH^2 inv2w2 log2 e with d_h in half-quanta (kH^2 is a power of two)
+  const float cq = cfg.inv2w2 * 1.4426950408889634f * (kHf * kHf);
+  const float kk = cfg.tile_k;
+  const int col = c0 + lane;
+  const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
+  const int rl = PAIR ? (lane & 31) : lane;     // row this lane computes a factor for
+  const int rs = PAIR ? (lane >> 5) : 0;        // which of the pair
+  const int myrow = r0 + (rl < TR ? rl : 0);
+  const int rowh = (int)llrint((cfg.xmin + myrow * cfg.dx) / kH);
+  const int K = buf.counts[(size_t)e * 8 + 5];
+  const int4* src = buf.src + (size_t)e * buf.src_cap;
+  const int4* swin = buf.swin + (size_t)e * buf.src_cap;
+  float* rdf = reinterpret_cast<float*>(&rd[0][0]);
+
+  float acc[TR];
+#pragma unroll
+  for (int r = 0; r < TR; ++r) acc[r] = 0.f;
+  // tiles outside the event's source rectangle stay zero
+  const int rx = buf.counts[(size_t)e * 8 + 6], ry = buf.counts[(size_t)e * 8 + 7];
+  const bool live = (rx & 0xffff) < r0 + TR && (rx >> 16) > r0 && (ry & 0xffff) < c0 + 64 && (ry >> 16) > c0;
+
+  auto fma_phase = [&](const float4* rr, float gy, float wq, int xw) {
+    const int xlo = xw & 0xffff, xhi = xw >> 16;
+#pragma unroll
+    for (int c = 0; c < TR / CH; ++c) {
+      // chunks of CH rows that the source's window misses are skipped
+      if (CH == TR || (r0 + CH * c < xhi && r0 + CH * c + CH > xlo)) {
+#pragma unroll
+        for (int i = c * CH / 4; i < (c + 1) * CH / 4; ++i) {
+          const float4 v = rr[i];
+          acc[4 * i] = fmaf(v.x * gy, wq, acc[4 * i]);
+          acc[4 * i + 1] = fmaf(v.y * gy, wq, acc[4 * i + 1]);
+          acc[4 * i + 2] = fmaf(v.z * gy, wq, acc[4 * i + 2]);
+          acc[4 * i + 3] = fmaf(v.w * gy, wq, acc[4 * i + 3]);
+        }
+      }
+    }
+  };
+  auto col_factor = [&](const TileEntry& q) {
+    const int ylo = q.yw & 0xffff, yhi = q.yw >> 16;
+    const float dy = (float)(ygh - q.yh);
+    return (col >= ylo && col < yhi) ? __builtin_amdgcn_exp2f(-(dy * dy) * cq) : 0.f;
+  };
+
+  for (int sub = 0; live && sub < K; sub += 64) {
+    const int k = sub + lane;
+    bool sel = false;
+    TileEntry en{0, 0, 0.f, 0, 0, 0};
+    if (k < K) {
+      const int4 s = src[k];
+      const int kind = s.w;
+      if (spec.enabled[kind] && spec.acc[kind] == acc_sel) {
+        const int4 wv = swin[k];
+        en.xh = s.x;
+        en.yh = s.y;
+        en.xw = (spec.win[kind] == kWinBox) ? wv.x : wv.z;
+        en.yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
+        en.w = spec.scale[kind] * (spec.use_fluct[kind] ? __int_as_float(s.z) : 1.f) * cfg.tile_w;
+        const int xlo = en.xw & 0xffff, xhi = en.xw >> 16, ylo = en.yw & 0xffff, yhi = en.yw >> 16;
+        sel = xlo < r0 + TR && xhi > r0 && ylo < c0 + 64 && yhi > c0;
+      }
+    }
+    const unsigned long long m = __ballot(sel);
+    if (sel) wl[__popcll(m & ((1ull << lane) - 1ull))] = en;
+    const int n = __popcll(m);
+    __builtin_amdgcn_wave_barrier();
+    for (int g = 0; g < n; g += (PAIR ? 2 : 1)) {
+      // row factors: lanes rl < TR of half rs for source g + rs
+      {
+        const int gi = (PAIR && g + rs < n) ? g + rs : g;
+        const TileEntry q = wl[gi];
+        if (rl < TR) {
+          const float dx = (float)(rowh - q.xh);
+          const int xlo = q.xw & 0xffff, xhi = q.xw >> 16;
+          const float gx = kk * __builtin_amdgcn_exp2f(-(dx * dx) * cq);
+          rdf[rs * TR + rl] = (myrow >= xlo && myrow < xhi) ? gx : 0.f;
+        }
+      }
+      const TileEntry q0 = wl[g];
+      const float gy0 = col_factor(q0);
+      const int xw0 = __builtin_amdgcn_readfirstlane(q0.xw);
+      __builtin_amdgcn_wave_barrier();
+      fma_phase(rd[0], gy0, q0.w, xw0);
+      if constexpr (PAIR) {
+        if (g + 1 < n) {
+          const TileEntry q1 = wl[g + 1];
+          const float gy1 = col_factor(q1);
+          const int xw1 = __builtin_amdgcn_readfirstlane(q1.xw);
+          fma_phase(rd[1], gy1, q1.w, xw1);
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (col >= ny) return;
+  const size_t N = (size_t)nx * ny;
+  const float so = cfg.tile_out;
+#pragma unroll
+  for (int r = 0; r < TR; ++r) {
+    const int ix = r0 + r;
+    if (ix < nx) out[(size_t)e * N + (size_t)ix * ny + col] = acc[r] * so;
+  }
+}
+
+template <int TR, int CH, bool PAIR>
+static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
+                      hipStream_t st) {
+  const dim3 grid(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64), nev);
+  hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s, 0, out0);
+  if (s.combine != 0) hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s, 1, out1);
+}
+
+int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
+                void* stream) {
+  if (nev <= 0) return 0;
+  if (!(cfg.tile_k > 0.f && cfg.tile_w > 0.f && cfg.tile_out > 0.f)) return (int)hipErrorInvalidValue;
+  hipStream_t st = (hipStream_t)stream;
+  launch_tr<32, 8, true>(cfg, b, s, nev, out0, out1, st);
+  if (s.combine == 1) return launch_combine_sqrt(out0, out1, (size_t)nev * cfg.nx * cfg.ny, stream);
+  return (int)hipGetLastError();
+}
+
+}  // namespace smc
