@@ -21,10 +21,13 @@
 //
 // One wavefront owns a tile of TR rows x 64 columns in TR accumulators per
 // lane (lane = column).  It selects the sources whose window meets the tile
-// in source order (ballot compaction, so the summation order is fixed),
-// and per selected source: lanes 0..TR-1 write K gx(row) into LDS, every
-// lane computes gy(column), and groups of four rows that the source's
-// window misses are skipped by a uniform branch.
+// in source order (ballot compaction, so the summation order is fixed).
+// Row factors K gx(row) go through LDS (broadcast float4 reads); with PAIR
+// the two half-waves compute them for two sources at once.  Every lane
+// computes gy(column) itself, and chunks of CH rows that a source's window
+// misses are skipped by a uniform branch.  The launch uses TR = 32, CH = 8,
+// PAIR (measured against TR 16/64, CH 4/16/32, unpaired, and a 32-column
+// layout with interleaved half-wave rows).
 #include "smc_device.h"
 
 namespace smc {
