@@ -139,6 +139,11 @@ struct smc_ctx {
   float* tb = nullptr;
   float* tmp = nullptr;
   double* evout = nullptr;
+  double* cm = nullptr;     // centre-of-mass partials [cap][max(tile units, band units)][4]
+  double* band = nullptr;   // moment partials [cap][moments_band_doubles()]
+  int cm_nu = 0;            // units per event in cm for the current grid (0: none)
+  void* mrects = nullptr;   // hot-spot mask rectangles [cap][2 kAMax] (8 bytes each)
+  int* nmrect = nullptr;    // [cap]
   double* kln_table = nullptr;
   // MC-KLN quadrature nodes (FP32 for the per-cell kernel, FP64 for points)
   void* kln_ptf = nullptr;
@@ -482,6 +487,10 @@ void alloc_buffers(smc_ctx* c) {
   b.src = dalloc<int4>(E * (size_t)b.src_cap);
   b.swin = dalloc<int4>(E * (size_t)b.src_cap);
   c->evout = dalloc<double>(E * SMC_EV_STRIDE);
+  c->cm = dalloc<double>(E * (size_t)std::max(tile_units(c->cfg), moments_cm_units_min()) * 4);
+  c->band = dalloc<double>(E * (size_t)moments_band_doubles());
+  c->mrects = dalloc<unsigned long long>(E * 2 * kAMax);
+  c->nmrect = dalloc<int>(E);
   HIPCHK(hipMemset(b.counts, 0, E * 8 * sizeof(int)));
 }
 
@@ -606,11 +615,17 @@ void run_density(smc_ctx* c) {
   if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
   if (c->p.which_mc_model == 1) {
     run_kln_density(c);
+    c->cm_nu = 0;
     c->have_density = 1;
     return;
   }
-  const SmearSpec spec = density_spec(c);
+  SmearSpec spec = density_spec(c);
+  // the tile kernel adds the centre-of-mass partials of a single-accumulator grid
+  const int sp = c->p.smear_path;
+  const bool tile = sp != 2 && !(sp == 1 && c->d.ny <= 272 && spec.combine != 1);  // run_smear's choice
+  if (tile && spec.combine == 0) spec.cm_out = c->cm;
   run_smear(c, spec, grid_buffer(c, &c->grid), spec.combine ? grid_buffer(c, &c->tmp) : nullptr);
+  c->cm_nu = spec.cm_out ? tile_units(c->cfg) : 0;
   c->have_density = 1;
 }
 
@@ -619,7 +634,9 @@ void run_moments(smc_ctx* c, int mode) {
   TimerScope ts(c, SMC_K_MOMENTS);
   DevCfg g = c->cfg;
   if (mode == 1) g.final_factor = 1.0;
-  check_launch(launch_moments(g, c->buf, c->grid, c->nev, c->evout, mode, c->stream), "k_moments");
+  check_launch(launch_moments(g, c->buf, c->grid, c->nev, c->evout, mode, c->cm_nu ? c->cm : nullptr, c->cm_nu,
+                              c->cm, c->band, c->mrects, c->nmrect, c->stream),
+               "k_moments");
   ts.done();
 }
 
@@ -771,6 +788,7 @@ void diff_forward(smc_ctx* c, const double* th) {
   float* grid = grid_buffer(c, &c->grid);
   TimerScope t1(c, SMC_K_SMEAR);
   check_launch(launch_tile(diff_cfg(c, th), c->buf, diff_spec(c, th), nev, grid, nullptr, c->stream), "k_tile");
+  c->cm_nu = 0;
   t1.done();
   TimerScope t2(c, SMC_K_MOMENTS);
   const double ffdxdy = c->p.finalFactor * c->p.dx * c->p.dy;
@@ -887,7 +905,7 @@ void smc_destroy(smc_ctx* c) {
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
-                  c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
+                  c->cm, c->band, c->mrects, c->nmrect, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (void* q : c->dif_alloc)

@@ -142,6 +142,7 @@ struct SmearSpec {
   int use_fluct[3];
   int acc[3];       // accumulator (0/1) per kind
   int combine;      // 0: out0 = acc0; 1: out0 = sqrt(acc0*acc1); 2: out0=acc0,out1=acc1
+  double* cm_out;   // combine 0, tile path: per-tile {sum v, sum x v, sum y v, 0} of out0, or null
 };
 
 // Launch wrappers (smc_kernels.hip)
@@ -149,8 +150,16 @@ int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long
                  unsigned long long event0, int mode, void* stream);
 int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,
                  float* out0, float* out1, void* stream);
-int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev,
-                   double* ev_out, int mode, void* stream);
+// mode 0: eps_n record; centre-of-mass partials from cm_units (nu per
+// event) or, if null, computed into cm_scratch (moments_cm_units_min() per
+// event); band holds moments_band_doubles() per event, rects 2 kAMax
+// 8-byte mask rectangles and nrect one int per event.  mode 1: grid sums.
+int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* ev_out, int mode,
+                   const double* cm_units, int nu, double* cm_scratch, double* band, void* rects, int* nrect,
+                   void* stream);
+int moments_band_doubles();
+int moments_cm_units_min();
+int tile_units(const DevCfg& cfg);  // smearing tiles per event (units of SmearSpec::cm_out)
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                  void* stream);
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,

@@ -202,255 +202,348 @@ __device__ void build_mask(const DevCfg& cfg, const EvBufs& buf, int e, unsigned
   }
 }
 
-// Pass 2 for any order range: <r^n> over the full grid and the masked eps_n
-// sums, rows prefetched.
-template <int NW, int NORD, int MAXC>
-__device__ void pass2_generic(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout,
-                              MomLds& L, const unsigned* mask, double total, double xc, double yc) {
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int nx = cfg.nx, ny = cfg.ny;
-  const int* cnt = buf.counts + (size_t)e * 8;
-  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
-  const double ff = cfg.final_factor;
-  const int nwy = (ny + 31) >> 5;
+// ------------------------------------------------------------------ moments
+// Three steps, all deterministic (fixed summation orders):
+//   centre of mass: per-unit partials {sum v, sum x v, sum y v} of the grid,
+//     units = the smearing kernel's tiles (written as it stores the grid) or
+//     kNB row bands of k_mom_cm;
+//   k_mom_band: one wavefront per (event, row band b: rows R.x0 + b + kNB i)
+//     builds the hot-spot mask of its rows in LDS and accumulates <r^n> and
+//     the masked eps_n sums (MakeDensity.cpp:2285-2423) about that centre;
+//   k_mom_final: one wavefront per event adds the kNB band partials and
+//     writes the event record.
+constexpr int kNB = 8;       // row bands per event
+constexpr int kNVMax = 64;   // partial sums per band (10 + 6 * 9)
+
+// Centre of mass of event e from its nu unit partials.
+__device__ __forceinline__ void event_cm(const DevCfg& cfg, const double* cmp, int nu, int e, double& total,
+                                         double& xc, double& yc) {
+  const int lane = threadIdx.x & 63;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int u = lane; u < nu; u += 64) {
+    const double* p = cmp + ((size_t)e * nu + u) * 4;
+    s0 += p[0];
+    s1 += p[1];
+    s2 += p[2];
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  total = s0 * cfg.final_factor;  // MakeDensity.cpp:2273-2282 with setSd's finalFactor
+  xc = s1 / s0;
+  yc = s2 / s0;
+}
+
+// Grid-sum units of k_mom_cm: one wavefront per (row band, event).
+template <int MAXC>
+__global__ __launch_bounds__(64) void k_mom_cm(DevCfg cfg, EvBufs buf, const float* grid, double* cmp) {
+  const int b = blockIdx.x, e = blockIdx.y, lane = threadIdx.x;
+  const float* g = grid + (size_t)e * cfg.nx * cfg.ny;
   const Rect R = event_rect(buf, e);
-  MomAcc<NORD> M;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  float cur[MAXC], nxt[MAXC];
+  load_row<MAXC>(g, R.x0 + b, R, cfg.ny, cur);
+  for (int ix = R.x0 + b; ix < R.x1; ix += kNB) {
+    load_row<MAXC>(g, ix + kNB, R, cfg.ny, nxt);
+    double rs = 0.0, ys = 0.0;
 #pragma unroll
-  for (int n = 0; n < 10; ++n) M.rn[n] = 0.0;
-#pragma unroll
-  for (int o = 0; o < NORD; ++o) M.mre[o] = M.mim[o] = M.mno[o] = M.pre[o] = M.pim[o] = M.pno[o] = 0.0;
-  const int from = cfg.from_order, to = cfg.to_order;
-  {
-    float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, R.x0 + wv, R, ny, cur);
-    for (int ix = R.x0 + wv; ix < R.x1; ix += NW) {
-      load_row<MAXC>(g, ix + NW, R, ny, nxt);
-      const double x = cfg.xmin + ix * cfg.dx - xc;
-      const unsigned* row = mask + (size_t)ix * nwy;
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j) {
-        if (cur[j] != 0.f) {
-          const int iy = R.y0 + lane + 64 * j;
-          const double y = cfg.ymin + iy * cfg.dy - yc;
-          const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
-          moment_cell<NORD>(M, from, to, (double)cur[j] * ff, x, y, masked);
-        }
-      }
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    for (int j = 0; j < MAXC; ++j) {
+      const double d = (double)cur[j];
+      rs += d;
+      ys += (cfg.ymin + (R.y0 + lane + 64 * j) * cfg.dy) * d;
     }
-  }
-  constexpr int NV = 10 + 6 * NORD;
-  double all[NV];
+    s0 += rs;
+    s1 += (cfg.xmin + ix * cfg.dx) * rs;
+    s2 += ys;
 #pragma unroll
-  for (int n = 0; n < 10; ++n) all[n] = M.rn[n];
-#pragma unroll
-  for (int o = 0; o < NORD; ++o) {
-    all[10 + 6 * o + 0] = M.mre[o];
-    all[10 + 6 * o + 1] = M.mim[o];
-    all[10 + 6 * o + 2] = M.mno[o];
-    all[10 + 6 * o + 3] = M.pre[o];
-    all[10 + 6 * o + 4] = M.pim[o];
-    all[10 + 6 * o + 5] = M.pno[o];
+    for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
   }
-  block_sums<NW, NV>(all, L);
-  if (t == 0) {
-    const double* T = L.tot;
-    double* out = evout + (size_t)e * SMC_EV_STRIDE;
-    for (int i = 0; i < 40; ++i) out[i] = 0.0;
-    for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
-    const double eps = 1e-15;
-    for (int o = 0; o < NORD; ++o) {
-      const int n = from + o;
-      if (n > to) break;
-      out[SMC_EV_EPS_RE + n] = -T[10 + 6 * o + 0] / (T[10 + 6 * o + 2] + eps);
-      out[SMC_EV_EPS_IM + n] = -T[10 + 6 * o + 1] / (T[10 + 6 * o + 2] + eps);
-      out[SMC_EV_EPSP_RE + n] = -T[10 + 6 * o + 3] / (T[10 + 6 * o + 5] + eps);
-      out[SMC_EV_EPSP_IM + n] = -T[10 + 6 * o + 4] / (T[10 + 6 * o + 5] + eps);
-    }
-    out[SMC_EV_XC] = xc;
-    out[SMC_EV_YC] = yc;
-    out[SMC_EV_TOTAL] = total;
-    out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
-    out[SMC_EV_NPART] = npa + npb;
-    out[SMC_EV_NCOLL] = ncoll;
-    out[SMC_EV_B] = buf.bval[e];
-    out[SMC_EV_STATUS] = cnt[3];
-    for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (lane == 0) {
+    double* p = cmp + ((size_t)e * kNB + b) * 4;
+    p[0] = s0;
+    p[1] = s1;
+    p[2] = s2;
+    p[3] = 0.0;
   }
 }
 
-
-// Pass 2 for the orders FROM..TO (FROM >= 2) fixed at compile time: r and
-// 1/r from one reciprocal square root, d r^n shared between <r^n> and the
-// r^n-weighted sums, one r^2 normalisation for all orders.
-template <int NW, int FROM, int TO, int MAXC>
-__device__ void pass2_fixed(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout,
-                            MomLds& L, const unsigned* mask, double total, double xc, double yc) {
-  static_assert(FROM >= 2 && TO <= 9 && FROM <= TO, "orders");
-  constexpr int NORD = TO - FROM + 1;
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+// Hot-spot mask rectangles (MCnucl.cpp:1303-1323, MakeDensity.cpp:2341-2359)
+// of every event, one 256-thread workgroup per event: the union box of the
+// participant boxes (with the zero boxes of the collision pairs at the
+// origin, CollisionPair.h:12, Box2D.cpp:14-22) fixes the offsets, then each
+// participant's box becomes grid cells [x0, x1) x [y0, y1), packed as
+// {x0 | x1 << 16, y0 | y1 << 16}; empty boxes are dropped.
+__global__ __launch_bounds__(256) void k_mask_rects(DevCfg cfg, EvBufs buf, uint2* rects, int* nrect) {
+  __shared__ int red[4][2];
+  __shared__ int wcnt[4];
+  const int e = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int nx = cfg.nx, ny = cfg.ny;
-  const int* cnt = buf.counts + (size_t)e * 8;
-  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
-  const double ff = cfg.final_factor;
-  const int nwy = (ny + 31) >> 5;
+  const int ncoll = buf.counts[(size_t)e * 8 + 2];
+  const int hbq = cfg.hb_h >> 1;
+  const int* pos = buf.pos + (size_t)e * 2 * kAMax * 2;
+  const int* wound = buf.wound + (size_t)e * 2 * kAMax;
+  int rxl = (ncoll > 0) ? 0 : 0x7fffffff, ryl = rxl;
+  for (int i = t; i < 2 * kAMax; i += 256) {
+    if ((i % kAMax) < cfg.nuc[i / kAMax].A && wound[i] >= 0) {
+      rxl = min(rxl, pos[2 * i] - hbq);
+      ryl = min(ryl, pos[2 * i + 1] - hbq);
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    rxl = min(rxl, __shfl_xor(rxl, off, 64));
+    ryl = min(ryl, __shfl_xor(ryl, off, 64));
+  }
+  if (lane == 0) {
+    red[wv][0] = rxl;
+    red[wv][1] = ryl;
+  }
+  __syncthreads();
+  rxl = min(min(red[0][0], red[1][0]), min(red[2][0], red[3][0]));
+  ryl = min(min(red[0][1], red[1][1]), min(red[2][1], red[3][1]));
+  const int offx = step_eval(cfg.tab[kOffX], rxl), offy = step_eval(cfg.tab[kOffY], ryl);
+  uint2* out = rects + (size_t)e * 2 * kAMax;
+  int base = 0;
+  for (int i0 = 0; i0 < 2 * kAMax; i0 += 256) {
+    const int i = i0 + t;
+    int x0 = 0, x1 = 0, y0 = 0, y1 = 0;
+    if (i < 2 * kAMax && (i % kAMax) < cfg.nuc[i / kAMax].A && wound[i] >= 0) {
+      const int px = pos[2 * i], py = pos[2 * i + 1];
+      x0 = max(step_eval(cfg.tab[kMaskX], px - hbq - rxl) + offx, 0);
+      x1 = min(step_eval(cfg.tab[kMaskX], px + hbq - rxl) + offx, nx);
+      y0 = max(step_eval(cfg.tab[kMaskY], py - hbq - ryl) + offy, 0);
+      y1 = min(step_eval(cfg.tab[kMaskX], py + hbq - ryl) + offy, ny);  // /dx (MakeDensity.cpp:2357)
+    }
+    const bool keep = x0 < x1 && y0 < y1;
+    const unsigned long long m = __ballot(keep);
+    __syncthreads();  // wcnt free
+    if (lane == 0) wcnt[wv] = __popcll(m);
+    __syncthreads();
+    int k = base;
+    for (int q = 0; q < wv; ++q) k += wcnt[q];
+    if (keep) out[k + __popcll(m & ((1ull << lane) - 1ull))] = make_uint2(x0 | (x1 << 16), y0 | (y1 << 16));
+    base += wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+  }
+  if (t == 0) nrect[e] = base;
+}
+
+// Hot-spot mask bits of the rows R.x0 + b + kNB lr, lr < nrow, as
+// bm[lr][nwy], from the event's rectangles (OR is order-free).
+__device__ void band_mask(const DevCfg& cfg, const uint2* rects, int np, const Rect& R, int b, int nrow,
+                          unsigned* bm) {
+  const int lane = threadIdx.x & 63;
+  const int nwy = (cfg.ny + 31) >> 5;
+  for (int i = lane; i < nrow * nwy; i += 64) bm[i] = 0u;
+  __syncthreads();  // mask cleared
+  for (int k = lane; k < np; k += 64) {
+    const uint2 rc = rects[k];
+    const int x0 = max((int)(rc.x & 0xffff), R.x0), x1 = min((int)(rc.x >> 16), R.x1);
+    const int y0 = rc.y & 0xffff, y1 = rc.y >> 16;
+    if (x0 >= x1) continue;
+    for (int lr = (x0 - R.x0 - b + kNB - 1) / kNB; R.x0 + b + kNB * lr < x1; ++lr) {
+      for (int y = y0; y < y1;) {
+        const int w = y >> 5, b0 = y & 31;
+        const int b1 = min(32, b0 + (y1 - y));
+        const unsigned bits = (b1 - b0 == 32) ? 0xffffffffu : (((1u << (b1 - b0)) - 1u) << b0);
+        atomicOr(&bm[lr * nwy + w], bits);
+        y += b1 - b0;
+      }
+    }
+  }
+  __syncthreads();  // mask complete
+}
+
+// One cell of the fixed-order (FROM..TO, FROM >= 2) sums: r and 1/r from
+// one reciprocal square root, d r^n shared between <r^n> and the
+// r^n-weighted sums, one r^2 normalisation for all orders.
+//   a[0..9] <r^n> numerators, a[10] sum_mask r^2 d, per order n:
+//   a[11 + 5 (n - FROM) + {0,1}] = sum_mask z^n r^(2-n) d, {2,3} = z^n d,
+//   {4} = r^n d (z = x + i y about the centre).
+template <int FROM, int TO>
+__device__ __forceinline__ void cell_fixed(double* a, double x, double y, double d, bool masked) {
+  const double r2 = fma(x, x, y * y);
+  const double rs = (r2 > 0.0) ? rsqrt(r2) : 0.0;
+  const double r = r2 * rs;
+  double pw[TO + 1];
+  pw[0] = d;
+#pragma unroll
+  for (int n = 1; n <= TO; ++n) pw[n] = pw[n - 1] * r;
+  double q = pw[TO];
+#pragma unroll
+  for (int n = 0; n <= TO; ++n) a[n] += pw[n];
+#pragma unroll
+  for (int n = TO + 1; n < 10; ++n) {
+    q *= r;
+    a[n] += q;
+  }
+  if (masked && r2 > 0.0) {
+    a[10] = fma(r2, d, a[10]);
+    double zr = x, zi = y, dw = d;  // z^n and d r^(2-n)
+#pragma unroll
+    for (int n = 2; n <= TO; ++n) {
+      const double nr = fma(zr, x, -zi * y), ni = fma(zr, y, zi * x);
+      zr = nr;
+      zi = ni;
+      if (n > 2) dw *= rs;
+      if (n >= FROM) {
+        double* o = a + 11 + 5 * (n - FROM);
+        o[0] = fma(zr, dw, o[0]);
+        o[1] = fma(zi, dw, o[1]);
+        o[2] = fma(zr, d, o[2]);
+        o[3] = fma(zi, d, o[3]);
+        o[4] += pw[n];
+      }
+    }
+  }
+}
+
+template <int NORD, bool FIX25>
+struct BandLayout {
+  static constexpr int NV = FIX25 ? 11 + 5 * 4 : 10 + 6 * NORD;
+};
+
+// Pass over one row band of one event.
+template <int NORD, int MAXC, bool FIX25>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 1, 8))) void k_mom_band(
+    DevCfg cfg, EvBufs buf, const float* grid, const double* cmp, int nu, const uint2* rects, const int* nrect,
+    double* band) {
+  extern __shared__ unsigned bm[];
+  constexpr int NV = BandLayout<NORD, FIX25>::NV;
+  static_assert(NV <= kNVMax, "band partials");
+  const int b = blockIdx.x, e = blockIdx.y, lane = threadIdx.x;
+  const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
+  const float* g = grid + (size_t)e * nx * ny;
   const Rect R = event_rect(buf, e);
-  constexpr int NV = 10 + 1 + 5 * NORD;  // rn[10], r^2 norm, per order re/im (r^2), re/im/norm (r^n)
+  double total, xc, yc;
+  event_cm(cfg, cmp, nu, e, total, xc, yc);
+  const int nrow = max(0, (R.x1 - R.x0 - b + kNB - 1) / kNB);
+  band_mask(cfg, rects + (size_t)e * 2 * kAMax, nrect[e], R, b, nrow, bm);
+  const double ff = cfg.final_factor;
+  const int from = cfg.from_order, to = cfg.to_order;
   double a[NV];
 #pragma unroll
   for (int i = 0; i < NV; ++i) a[i] = 0.0;
-  {
-    float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, R.x0 + wv, R, ny, cur);
-    for (int ix = R.x0 + wv; ix < R.x1; ix += NW) {
-      load_row<MAXC>(g, ix + NW, R, ny, nxt);
-      const double x = cfg.xmin + ix * cfg.dx - xc;
-      const unsigned* row = mask + (size_t)ix * nwy;
+  MomAcc<NORD> M;
+  if constexpr (!FIX25) {
 #pragma unroll
-      for (int j = 0; j < MAXC; ++j) {
-        if (cur[j] != 0.f) {
-          const int iy = R.y0 + lane + 64 * j;
-          const double y = cfg.ymin + iy * cfg.dy - yc;
-          const double d = (double)cur[j] * ff;
-          const double r2 = fma(x, x, y * y);
-          const double rs = (r2 > 0.0) ? rsqrt(r2) : 0.0;
-          const double r = r2 * rs;
-          double pw[TO + 1];
-          pw[0] = d;
+    for (int n = 0; n < 10; ++n) M.rn[n] = 0.0;
 #pragma unroll
-          for (int n = 1; n <= TO; ++n) pw[n] = pw[n - 1] * r;
-          double q = pw[TO];
+    for (int o = 0; o < NORD; ++o) M.mre[o] = M.mim[o] = M.mno[o] = M.pre[o] = M.pim[o] = M.pno[o] = 0.0;
+  }
+  float cur[MAXC], nxt[MAXC];
+  load_row<MAXC>(g, R.x0 + b, R, ny, cur);
+  for (int lr = 0; lr < nrow; ++lr) {
+    const int ix = R.x0 + b + kNB * lr;
+    load_row<MAXC>(g, ix + kNB, R, ny, nxt);
+    const double x = cfg.xmin + ix * cfg.dx - xc;
+    const unsigned* row = bm + lr * nwy;
 #pragma unroll
-          for (int n = 0; n <= TO; ++n) a[n] += pw[n];
-#pragma unroll
-          for (int n = TO + 1; n < 10; ++n) {
-            q *= r;
-            a[n] += q;
-          }
-          const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
-          if (masked && r2 > 0.0) {
-            a[10] = fma(r2, d, a[10]);
-            double zr = x, zi = y, dw = d;  // z^n and d r^(2-n)
-#pragma unroll
-            for (int n = 2; n <= TO; ++n) {
-              const double nr = fma(zr, x, -zi * y), ni = fma(zr, y, zi * x);
-              zr = nr;
-              zi = ni;
-              if (n > 2) dw *= rs;
-              if (n >= FROM) {
-                double* o = a + 11 + 5 * (n - FROM);
-                o[0] = fma(zr, dw, o[0]);
-                o[1] = fma(zi, dw, o[1]);
-                o[2] = fma(zr, d, o[2]);
-                o[3] = fma(zi, d, o[3]);
-                o[4] += pw[n];
-              }
-            }
-          }
-        }
+    for (int j = 0; j < MAXC; ++j) {
+      if (cur[j] != 0.f) {
+        const int iy = R.y0 + lane + 64 * j;
+        const double y = cfg.ymin + iy * cfg.dy - yc;
+        const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
+        if constexpr (FIX25)
+          cell_fixed<2, 5>(a, x, y, (double)cur[j] * ff, masked);
+        else
+          moment_cell<NORD>(M, from, to, (double)cur[j] * ff, x, y, masked);
       }
+    }
 #pragma unroll
-      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+  }
+  if constexpr (!FIX25) {
+#pragma unroll
+    for (int n = 0; n < 10; ++n) a[n] = M.rn[n];
+#pragma unroll
+    for (int o = 0; o < NORD; ++o) {
+      a[10 + 6 * o + 0] = M.mre[o];
+      a[10 + 6 * o + 1] = M.mim[o];
+      a[10 + 6 * o + 2] = M.mno[o];
+      a[10 + 6 * o + 3] = M.pre[o];
+      a[10 + 6 * o + 4] = M.pim[o];
+      a[10 + 6 * o + 5] = M.pno[o];
     }
   }
-  block_sums<NW, NV>(a, L);
-  if (t == 0) {
-    const double* T = L.tot;
-    double* out = evout + (size_t)e * SMC_EV_STRIDE;
-    for (int i = 0; i < 40; ++i) out[i] = 0.0;
-    for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
-    const double eps = 1e-15;
-    for (int n = FROM; n <= TO; ++n) {
-      const double* o = T + 11 + 5 * (n - FROM);
+  double* out = band + ((size_t)e * kNB + b) * kNVMax;
+#pragma unroll
+  for (int i = 0; i < NV; ++i) {
+    const double r = wave_sum(a[i]);
+    if (lane == 0) out[i] = r;
+    __builtin_amdgcn_sched_barrier(0);  // one value in flight at a time
+  }
+}
+
+// Event record from the band sums T (layout of k_mom_band).
+template <int NORD, bool FIX25>
+__device__ void write_record(const DevCfg& cfg, const EvBufs& buf, int e, const double* T, double total, double xc,
+                             double yc, double* evout) {
+  const int* cnt = buf.counts + (size_t)e * 8;
+  double* out = evout + (size_t)e * SMC_EV_STRIDE;
+  for (int i = 0; i < 40; ++i) out[i] = 0.0;
+  for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
+  const double eps = 1e-15;
+  if constexpr (FIX25) {
+    for (int n = 2; n <= 5; ++n) {
+      const double* o = T + 11 + 5 * (n - 2);
       out[SMC_EV_EPS_RE + n] = -o[0] / (T[10] + eps);
       out[SMC_EV_EPS_IM + n] = -o[1] / (T[10] + eps);
       out[SMC_EV_EPSP_RE + n] = -o[2] / (o[4] + eps);
       out[SMC_EV_EPSP_IM + n] = -o[3] / (o[4] + eps);
     }
-    out[SMC_EV_XC] = xc;
-    out[SMC_EV_YC] = yc;
-    out[SMC_EV_TOTAL] = total;
-    out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
-    out[SMC_EV_NPART] = npa + npb;
-    out[SMC_EV_NCOLL] = ncoll;
-    out[SMC_EV_B] = buf.bval[e];
-    out[SMC_EV_STATUS] = cnt[3];
-    for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
-  }
-}
-
-// Moments of one event's grid g[nx][ny] (global memory) for a workgroup of
-// NW waves.  s1 holds this thread's partial (sum rho, sum x rho, sum y rho)
-// when the caller computed them (have_s1), otherwise pass 1 reads the grid.
-// ff = finalFactor (setSd, MakeDensity.cpp:2514-2522).
-template <int NW, int NORD, int MAXC, bool FIX25>
-__device__ void grid_moments(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout,
-                             MomLds& L, unsigned* mask, bool have_s1, double (&s1)[3]) {
-  const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
-  const int nx = cfg.nx, ny = cfg.ny;
-  const int* cnt = buf.counts + (size_t)e * 8;
-  const int npa = cnt[0], npb = cnt[1], ncoll = cnt[2];
-  const double ff = cfg.final_factor;
-  const int nwy = (ny + 31) >> 5;
-  const Rect R = event_rect(buf, e);
-
-  // pass 1: centre of mass (MakeDensity.cpp:2273-2282), rows prefetched
-  if (!have_s1) {
-    float cur[MAXC], nxt[MAXC];
-    load_row<MAXC>(g, R.x0 + wv, R, ny, cur);
-    for (int ix = R.x0 + wv; ix < R.x1; ix += NW) {
-      load_row<MAXC>(g, ix + NW, R, ny, nxt);
-      const double xg = cfg.xmin + ix * cfg.dx;
-      double rs = 0.0, ys = 0.0;
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j) {
-        const double d = (double)cur[j] * ff;
-        rs += d;
-        ys += (cfg.ymin + (R.y0 + lane + 64 * j) * cfg.dy) * d;
-      }
-      s1[0] += rs;
-      s1[1] += xg * rs;
-      s1[2] += ys;
-#pragma unroll
-      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+  } else {
+    for (int o = 0; o < NORD; ++o) {
+      const int n = cfg.from_order + o;
+      if (n > cfg.to_order) break;
+      out[SMC_EV_EPS_RE + n] = -T[10 + 6 * o + 0] / (T[10 + 6 * o + 2] + eps);
+      out[SMC_EV_EPS_IM + n] = -T[10 + 6 * o + 1] / (T[10 + 6 * o + 2] + eps);
+      out[SMC_EV_EPSP_RE + n] = -T[10 + 6 * o + 3] / (T[10 + 6 * o + 5] + eps);
+      out[SMC_EV_EPSP_IM + n] = -T[10 + 6 * o + 4] / (T[10 + 6 * o + 5] + eps);
     }
   }
+  out[SMC_EV_XC] = xc;
+  out[SMC_EV_YC] = yc;
+  out[SMC_EV_TOTAL] = total;
+  out[SMC_EV_DSDY] = total * cfg.dx * cfg.dy;
+  out[SMC_EV_NPART] = cnt[0] + cnt[1];
+  out[SMC_EV_NCOLL] = cnt[2];
+  out[SMC_EV_B] = buf.bval[e];
+  out[SMC_EV_STATUS] = cnt[3];
+  for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
+}
 
-  build_mask<NW>(cfg, buf, e, mask, L);
-  block_sums<NW, 3>(s1, L);  // also orders the mask writes before the reads below
-  const double total = L.tot[0];
-  const double xc = L.tot[1] / total, yc = L.tot[2] / total;
-
-  if constexpr (FIX25)
-    pass2_fixed<NW, 2, 5, MAXC>(cfg, buf, e, g, evout, L, mask, total, xc, yc);
-  else
-    pass2_generic<NW, NORD, MAXC>(cfg, buf, e, g, evout, L, mask, total, xc, yc);
+template <int NORD, bool FIX25>
+__global__ __launch_bounds__(64) void k_mom_final(DevCfg cfg, EvBufs buf, const double* cmp, int nu,
+                                                  const double* band, double* evout) {
+  __shared__ double T[kNVMax];
+  constexpr int NV = BandLayout<NORD, FIX25>::NV;
+  const int e = blockIdx.x, lane = threadIdx.x;
+  double total, xc, yc;
+  event_cm(cfg, cmp, nu, e, total, xc, yc);
+  if (lane < NV) {
+    double s = 0.0;
+    for (int b = 0; b < kNB; ++b) s += band[((size_t)e * kNB + b) * kNVMax + lane];
+    T[lane] = s;
+  }
+  __syncthreads();
+  if (lane == 0) write_record<NORD, FIX25>(cfg, buf, e, T, total, xc, yc, evout);
 }
 
 // Sum of the grid only (dndy, MCnucl.cpp:817-820), no finalFactor.
-template <int NW>
-__device__ void grid_sum(const DevCfg& cfg, const EvBufs& buf, int e, const float* g, double* evout, MomLds& L) {
+__global__ __launch_bounds__(kGT) void k_grid_sum(DevCfg cfg, EvBufs buf, const float* grid, double* evout) {
+  __shared__ MomLds L;
+  const int e = blockIdx.x;
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const float* g = grid + (size_t)e * cfg.nx * cfg.ny;
   double s[1] = {0.0};
   const Rect R = event_rect(buf, e);
-  for (int ix = R.x0 + wv; ix < R.x1; ix += NW)
+  for (int ix = R.x0 + wv; ix < R.x1; ix += kGW)
     for (int iy = R.y0 + lane; iy < R.y1; iy += 64) s[0] += (double)g[(size_t)ix * cfg.ny + iy];
-  block_sums<NW, 1>(s, L);
+  block_sums<kGW, 1>(s, L);
   if (t == 0) evout[(size_t)e * SMC_EV_STRIDE + SMC_EV_TOTAL] = L.tot[0];
-}
-
-// Moments of grids already in memory (non-fused densities), same code path.
-template <int NORD, int MAXC, bool FIX25>
-__global__ __launch_bounds__(kGT) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 1, 8))) void k_moments2(DevCfg cfg, EvBufs buf, const float* grid, double* evout,
-                                                  int flags) {
-  __shared__ MomLds L;
-  extern __shared__ unsigned dyn_mask[];
-  const int e = blockIdx.x;
-  const float* g = grid + (size_t)e * cfg.nx * cfg.ny;
-  double s1[3] = {0.0, 0.0, 0.0};
-  if (flags & 2) grid_moments<kGW, NORD, MAXC, FIX25>(cfg, buf, e, g, evout, L, dyn_mask, false, s1);
-  else grid_sum<kGW>(cfg, buf, e, g, evout, L);
 }
 
 // Differentiable batches: per-event FP64 scalars for |eps_2|, |eps_3| and
@@ -548,23 +641,43 @@ int launch_diff_moments(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, c
 }
 
 int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* ev_out, int mode,
+                   const double* cm_units, int nu, double* cm_scratch, double* band, void* rects, int* nrect,
                    void* stream) {
   if (nev <= 0) return 0;
   if (cfg.nx > kGMaxN * 2 || cfg.ny > kGMaxN * 2) return (int)hipErrorInvalidValue;
-  const int flags = (mode == 0) ? 2 : 4;
-  const int nord = cfg.to_order - cfg.from_order + 1;
   hipStream_t st = (hipStream_t)stream;
-  const size_t shm = (size_t)cfg.nx * ((cfg.ny + 31) / 32) * sizeof(unsigned);
+  if (mode != 0) {
+    hipLaunchKernelGGL(k_grid_sum, dim3(nev), dim3(kGT), 0, st, cfg, b, grid, ev_out);
+    return (int)hipGetLastError();
+  }
   const bool narrow = cfg.ny <= 320;
-  if (cfg.from_order == 2 && cfg.to_order == 5 && narrow)
-    hipLaunchKernelGGL((k_moments2<4, 5, true>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
-  else if (nord <= 4 && narrow)
-    hipLaunchKernelGGL((k_moments2<4, 5, false>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
-  else if (nord <= 4)
-    hipLaunchKernelGGL((k_moments2<4, 9, false>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
-  else
-    hipLaunchKernelGGL((k_moments2<9, 9, false>), dim3(nev), dim3(kGT), shm, st, cfg, b, grid, ev_out, flags);
+  if (!cm_units) {  // centre-of-mass partials from the grid itself
+    if (narrow) hipLaunchKernelGGL(k_mom_cm<5>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm_scratch);
+    else hipLaunchKernelGGL(k_mom_cm<9>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm_scratch);
+    cm_units = cm_scratch;
+    nu = kNB;
+  }
+  uint2* rc = static_cast<uint2*>(rects);
+  hipLaunchKernelGGL(k_mask_rects, dim3(nev), dim3(256), 0, st, cfg, b, rc, nrect);
+  const int nord = cfg.to_order - cfg.from_order + 1;
+  const size_t shm = (size_t)((cfg.nx + kNB - 1) / kNB) * ((cfg.ny + 31) / 32) * sizeof(unsigned);
+  const dim3 gb(kNB, nev);
+  if (cfg.from_order == 2 && cfg.to_order == 5 && narrow) {
+    hipLaunchKernelGGL((k_mom_band<4, 5, true>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    hipLaunchKernelGGL((k_mom_final<4, true>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
+  } else if (nord <= 4) {
+    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    else hipLaunchKernelGGL((k_mom_band<4, 9, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    hipLaunchKernelGGL((k_mom_final<4, false>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
+  } else {
+    if (narrow) hipLaunchKernelGGL((k_mom_band<9, 5, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    else hipLaunchKernelGGL((k_mom_band<9, 9, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    hipLaunchKernelGGL((k_mom_final<9, false>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
+  }
   return (int)hipGetLastError();
 }
+
+int moments_band_doubles() { return kNB * kNVMax; }
+int moments_cm_units_min() { return kNB; }
 
 }  // namespace smc

@@ -144,9 +144,31 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
     }
     __builtin_amdgcn_wave_barrier();
   }
+  const float so = cfg.tile_out;
+  if (spec.cm_out) {  // centre-of-mass partials of the stored grid values
+    double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      if (col < ny && r0 + r < nx) {
+        const double v = (double)(acc[r] * so);
+        s0 += v;
+        s1 = fma(cfg.xmin + (r0 + r) * cfg.dx, v, s1);
+      }
+    }
+    double s2 = (col < ny) ? (cfg.ymin + col * cfg.dy) * s0 : 0.0;
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    s2 = wave_sum(s2);
+    if (lane == 0) {
+      double* p = spec.cm_out + ((size_t)e * gridDim.x + blockIdx.x) * 4;
+      p[0] = s0;
+      p[1] = s1;
+      p[2] = s2;
+      p[3] = 0.0;
+    }
+  }
   if (col >= ny) return;
   const size_t N = (size_t)nx * ny;
-  const float so = cfg.tile_out;
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
     const int ix = r0 + r;
@@ -154,12 +176,18 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
   }
 }
 
+constexpr int kTileRows = 32;
+
+int tile_units(const DevCfg& cfg) { return ((cfg.nx + kTileRows - 1) / kTileRows) * ((cfg.ny + 63) / 64); }
+
 template <int TR, int CH, bool PAIR>
 static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                       hipStream_t st) {
   const dim3 grid(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64), nev);
-  hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s, 0, out0);
-  if (s.combine != 0) hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s, 1, out1);
+  SmearSpec s1 = s;  // partials only for a single accumulator
+  if (s.combine != 0) s1.cm_out = nullptr;
+  hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 0, out0);
+  if (s.combine != 0) hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 1, out1);
 }
 
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
@@ -167,7 +195,7 @@ int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,
   if (nev <= 0) return 0;
   if (!(cfg.tile_k > 0.f && cfg.tile_w > 0.f && cfg.tile_out > 0.f)) return (int)hipErrorInvalidValue;
   hipStream_t st = (hipStream_t)stream;
-  launch_tr<32, 8, true>(cfg, b, s, nev, out0, out1, st);
+  launch_tr<kTileRows, 8, true>(cfg, b, s, nev, out0, out1, st);
   if (s.combine == 1) return launch_combine_sqrt(out0, out1, (size_t)nev * cfg.nx * cfg.ny, stream);
   return (int)hipGetLastError();
 }
