@@ -365,7 +365,10 @@ __device__ void band_mask(const DevCfg& cfg, const uint2* rects, int np, const R
 template <int FROM, int TO>
 __device__ __forceinline__ void cell_fixed(double* a, double x, double y, double d, bool masked) {
   const double r2 = fma(x, x, y * y);
-  const double rs = (r2 > 0.0) ? rsqrt(r2) : 0.0;
+  // 1/r: the hardware estimate and one Newton step (relative error ~1e-16)
+  const double y0 = __builtin_amdgcn_rsq(r2);
+  const double e0 = fma(-(r2 * y0), y0, 1.0);
+  const double rs = (r2 > 0.0) ? fma(0.5 * y0, e0, y0) : 0.0;
   const double r = r2 * rs;
   double pw[TO + 1];
   pw[0] = d;
@@ -433,6 +436,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 
 #pragma unroll
     for (int o = 0; o < NORD; ++o) M.mre[o] = M.mim[o] = M.mno[o] = M.pre[o] = M.pim[o] = M.pno[o] = 0.0;
   }
+  // cell values enter without finalFactor; the sums are scaled once below
+  double yv[MAXC];
+#pragma unroll
+  for (int j = 0; j < MAXC; ++j) yv[j] = cfg.ymin + (R.y0 + lane + 64 * j) * cfg.dy - yc;
   float cur[MAXC], nxt[MAXC];
   load_row<MAXC>(g, R.x0 + b, R, ny, cur);
   for (int lr = 0; lr < nrow; ++lr) {
@@ -444,12 +451,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 
     for (int j = 0; j < MAXC; ++j) {
       if (cur[j] != 0.f) {
         const int iy = R.y0 + lane + 64 * j;
-        const double y = cfg.ymin + iy * cfg.dy - yc;
         const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
         if constexpr (FIX25)
-          cell_fixed<2, 5>(a, x, y, (double)cur[j] * ff, masked);
+          cell_fixed<2, 5>(a, x, yv[j], (double)cur[j], masked);
         else
-          moment_cell<NORD>(M, from, to, (double)cur[j] * ff, x, y, masked);
+          moment_cell<NORD>(M, from, to, (double)cur[j], x, yv[j], masked);
       }
     }
 #pragma unroll
@@ -471,7 +477,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 
   double* out = band + ((size_t)e * kNB + b) * kNVMax;
 #pragma unroll
   for (int i = 0; i < NV; ++i) {
-    const double r = wave_sum(a[i]);
+    const double r = wave_sum(a[i] * ff);
     if (lane == 0) out[i] = r;
     __builtin_amdgcn_sched_barrier(0);  // one value in flight at a time
   }
