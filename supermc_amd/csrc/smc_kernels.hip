@@ -30,7 +30,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const unsigned long long gev = event0 + (unsigned long long)e;
 
   __shared__ __align__(16) int px[2][kAMax], py[2][kAMax];  // input/sample order (quanta)
-  __shared__ __align__(16) float pz[2][kAMax];              // local z (hard core only)
+  __shared__ __align__(16) float pz[2][kAMax];              // local z * 2^24 (hard core only)
   __shared__ __align__(16) int cqx[2][64], cqy[2][64];       // candidates of a round
   __shared__ __align__(16) float cqz[2][64];
   __shared__ int s_nc[2], s_wc[2][4];
@@ -38,6 +38,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   __shared__ int wrank[kAMax];
   __shared__ int s_bb[4];
   __shared__ unsigned long long s_okm[4];
+  __shared__ unsigned long long s_cm[4][64];  // conflict masks of a round, per wave
   __shared__ int sx[2][kAMax], sy[2][kAMax];  // sorted
   __shared__ unsigned hits[kAMax][kAMax / 32];
   __shared__ int rowcnt[kAMax], fh[kAMax], scan_tmp[256];
@@ -94,8 +95,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             float x = 0.5f * d, y = 0.f, z = 0.f;
             rotate3(ctr[wv], phr[wv], x, y, z);
             const int xq = quant(x), yq = quant(y);
-            px[wv][0] = xq + xcen; py[wv][0] = yq; pz[wv][0] = z;
-            px[wv][1] = -xq + xcen; py[wv][1] = -yq; pz[wv][1] = -z;
+            px[wv][0] = xq + xcen; py[wv][0] = yq; pz[wv][0] = z * 0x1p24f;
+            px[wv][1] = -xq + xcen; py[wv][1] = -yq; pz[wv][1] = -z * 0x1p24f;
             s_acc[wv] = 2;
           }
         }
@@ -147,7 +148,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             if (pos < 64) {
               cqx[nu][pos] = xq;
               cqy[nu][pos] = yq;
-              cqz[nu][pos] = z;
+              cqz[nu][pos] = z * 0x1p24f;
             }
           }
           nc[0] = min(64, nc[0] + wc[0] + wc[2]);
@@ -157,13 +158,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         }
         if (t < 2) s_nc[t] = nc[t];
         __syncthreads();
+        // Distances in position quanta: x, y differences exact in int32, z
+        // held as z * 2^24, so d^2 < dmin^2 reads d2q < dmin^2 * 2^48.
+        const float dmin2q = cfg.dmin2 * 0x1p48f;
         // hard core against the nucleons accepted in earlier rounds: wave
         // nu + 2*sub checks every candidate against half of them
         const int acc0 = s_acc[nu];
-        bool ok = lane < s_nc[nu];  // a full list unless the step cap was hit
+        const int ncand = s_nc[nu];  // a full list unless the step cap was hit
+        bool ok = lane < ncand;
+        const int xq = cqx[nu][lane], yq = cqy[nu][lane];
+        const float zq = cqz[nu][lane];
         if (need && ok) {
-          const int xq = cqx[nu][lane], yq = cqy[nu][lane];
-          const float z = cqz[nu][lane];
           const int half = ((acc0 + 7) >> 3) << 2;  // multiple of 4
           const int j0 = sub ? half : 0, j1 = sub ? acc0 : min(half, acc0);
           const int4* px4 = reinterpret_cast<const int4*>(px[nu]);
@@ -178,30 +183,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
             const float azs[8] = {az0.x, az0.y, az0.z, az0.w, az1.x, az1.y, az1.z, az1.w};
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
-              const float dx = (float)(xq - axs[u]) * kQf, dy = (float)(yq - ays[u]) * kQf;
-              const float dz = z - azs[u];
-              if (dx * dx + dy * dy + dz * dz < cfg.dmin2) ok = false;
+              const float fx = (float)(xq - axs[u]), fy = (float)(yq - ays[u]), fz = zq - azs[u];
+              if (fmaf(fx, fx, fmaf(fy, fy, fz * fz)) < dmin2q) ok = false;
             }
           }
           for (; j < j1 && ok; ++j) {
-            const float dx = (float)(xq - px[nu][j]) * kQf, dy = (float)(yq - py[nu][j]) * kQf;
-            const float dz = z - pz[nu][j];
-            if (dx * dx + dy * dy + dz * dz < cfg.dmin2) ok = false;
+            const float fx = (float)(xq - px[nu][j]), fy = (float)(yq - py[nu][j]), fz = zq - pz[nu][j];
+            if (fmaf(fx, fx, fmaf(fy, fy, fz * fz)) < dmin2q) ok = false;
           }
         }
-        const unsigned long long okw = __ballot(ok);
-        if (lane == 0) s_okm[wv] = okw;
-        __syncthreads();
-        if (need && sub == 0) {
-          const int xq = cqx[nu][lane], yq = cqy[nu][lane];
-          const float z = cqz[nu][lane];
-          // conflicts with the earlier candidates of this round: bit l of cm
-          unsigned long long cm = 0ull;
+        // conflicts with the earlier candidates of this round: bit l of cm
+        // for l < lane within the hard core; wave sub takes l in [32 sub, 32 sub + 32)
+        unsigned long long cm = 0ull;
+        if (need && lane < ncand) {
           const int4* cx4 = reinterpret_cast<const int4*>(cqx[nu]);
           const int4* cy4 = reinterpret_cast<const int4*>(cqy[nu]);
           const float4* cz4 = reinterpret_cast<const float4*>(cqz[nu]);
 #pragma unroll 4
-          for (int l4 = 0; l4 < 16; ++l4) {
+          for (int l4 = 8 * sub; l4 < 8 * sub + 8; ++l4) {
+            if (4 * l4 >= lane) break;
             const int4 bx = cx4[l4], by = cy4[l4];
             const float4 bz = cz4[l4];
             const int bxs[4] = {bx.x, bx.y, bx.z, bx.w}, bys[4] = {by.x, by.y, by.z, by.w};
@@ -209,42 +209,45 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
               const int l = 4 * l4 + u;
-              const float dx = (float)(xq - bxs[u]) * kQf, dy = (float)(yq - bys[u]) * kQf;
-              const float dz = z - bzs[u];
-              if (l < lane && dx * dx + dy * dy + dz * dz < cfg.dmin2) cm |= 1ull << l;
+              const float fx = (float)(xq - bxs[u]), fy = (float)(yq - bys[u]), fz = zq - bzs[u];
+              if (l < lane && fmaf(fx, fx, fmaf(fy, fy, fz * fz)) < dmin2q) cm |= 1ull << l;
             }
           }
-          // sequential acceptance in list order on wave-uniform masks; with
-          // no conflict inside the round it is the first ok candidates
+        }
+        const unsigned long long okw = __ballot(ok);
+        if (lane == 0) s_okm[wv] = okw;
+        s_cm[wv][lane] = cm;
+        __syncthreads();
+        if (need && sub == 0) {
+          cm |= s_cm[wv + 2][lane];
+          // sequential acceptance in list order: a candidate without conflicts
+          // is accepted iff it passed the hard core; the conflicted ones are
+          // decided in order against the accepted set
           const unsigned long long okm = s_okm[nu] & s_okm[nu + 2];
-          const int An = NN[nu];
-          unsigned long long accm = 0ull;
-          int nacc = acc0;
-          if (__ballot(cm != 0ull) == 0ull) {
-            accm = okm;
-            int nok = __popcll(okm);
-            while (nok > An - acc0) {  // keep the first An - acc0
-              accm &= ~(1ull << (63 - __clzll(accm)));
-              --nok;
-            }
-            nacc = acc0 + nok;
-          } else
-          for (int l = 0; l < 64 && nacc < An; ++l) {
-            if (!((okm >> l) & 1ull)) continue;
+          const unsigned long long conf = __ballot(cm != 0ull);
+          unsigned long long accm = okm & ~conf;
+          unsigned long long rest = okm & conf;
+          while (rest) {
+            const int l = __ffsll((long long)rest) - 1;
+            rest &= rest - 1;
             const unsigned lo32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)cm, l);
             const unsigned hi32 = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(cm >> 32), l);
-            const unsigned long long cl = ((unsigned long long)hi32 << 32) | lo32;
-            if (cl & accm) continue;
-            accm |= 1ull << l;
-            ++nacc;
+            if (!((((unsigned long long)hi32 << 32) | lo32) & accm)) accm |= 1ull << l;
+          }
+          // keep the first An - acc0 (later candidates never affect earlier ones)
+          const int An = NN[nu];
+          int nok = __popcll(accm);
+          while (nok > An - acc0) {
+            accm &= ~(1ull << (63 - __clzll(accm)));
+            --nok;
           }
           if ((accm >> lane) & 1ull) {
             const int slot = acc0 + __popcll(accm & ((1ull << lane) - 1ull));
             px[nu][slot] = xq;
             py[nu][slot] = yq;
-            pz[nu][slot] = z;
+            pz[nu][slot] = zq;
           }
-          if (lane == 0) s_acc[nu] = nacc;
+          if (lane == 0) s_acc[nu] = acc0 + nok;
         }
         __syncthreads();
       }
@@ -437,10 +440,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     bb[1] = cfg.nx;
     bb[3] = cfg.ny;
   }
-  atomicMin(&s_bb[0], bb[0]);
-  atomicMax(&s_bb[1], bb[1]);
-  atomicMin(&s_bb[2], bb[2]);
-  atomicMax(&s_bb[3], bb[3]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    bb[0] = min(bb[0], __shfl_xor(bb[0], off, 64));
+    bb[1] = max(bb[1], __shfl_xor(bb[1], off, 64));
+    bb[2] = min(bb[2], __shfl_xor(bb[2], off, 64));
+    bb[3] = max(bb[3], __shfl_xor(bb[3], off, 64));
+  }
+  if (lane == 0) {
+    atomicMin(&s_bb[0], bb[0]);
+    atomicMax(&s_bb[1], bb[1]);
+    atomicMin(&s_bb[2], bb[2]);
+    atomicMax(&s_bb[3], bb[3]);
+  }
   __syncthreads();
   if (t == 0) {
     int* c = buf.counts + (size_t)e * 8;
