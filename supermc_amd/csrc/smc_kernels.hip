@@ -39,7 +39,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   __shared__ int s_bb[4];
   __shared__ unsigned long long s_okm[4];
   __shared__ unsigned long long s_cm[4][64];  // conflict masks of a round, per wave
-  __shared__ int sx[2][kAMax], sy[2][kAMax];  // sorted
+  __shared__ __align__(16) int sx[2][kAMax], sy[2][kAMax];  // sorted
   __shared__ unsigned hits[kAMax][kAMax / 32];
   __shared__ int rowcnt[kAMax], fh[kAMax], scan_tmp[256];
   __shared__ int s_acc[2];
@@ -296,19 +296,33 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
           const int mid = (lo + hi) >> 1;
           if (sx[1][mid] < xa - lim) lo = mid + 1; else hi = mid;
         }
-        for (int j = lo; j < B; ++j) {
-          const int xb = sx[1][j];
-          if (xb > xa + lim) break;
-          const int dx = xa - xb, dy = ya - sy[1][j];
-          if (dy > lim || dy < -lim) continue;
-          const long long n = (long long)dx * dx + (long long)dy * dy;
-          if (n <= cfg.hit_n) {
+        // targets in x order, four per LDS load, until x exceeds xa + lim
+        const int4* sx4 = reinterpret_cast<const int4*>(sx[1]);
+        const int4* sy4 = reinterpret_cast<const int4*>(sy[1]);
+        for (int j4 = lo >> 2; 4 * j4 < B; ++j4) {
+          const int4 xb4 = sx4[j4], yb4 = sy4[j4];
+          const int xbs[4] = {xb4.x, xb4.y, xb4.z, xb4.w}, ybs[4] = {yb4.x, yb4.y, yb4.z, yb4.w};
+          bool past = false;
 #pragma unroll
-            for (int w = 0; w < kAMax / 32; ++w)
-              if (w == (j >> 5)) bits[w] |= 1u << (j & 31);
-            ++cnt;
-            atomicMin(&fh[j], t);  // first hitting projectile row
+          for (int u = 0; u < 4; ++u) {
+            const int j = 4 * j4 + u;
+            if (j < lo || j >= B || past) continue;
+            if (xbs[u] > xa + lim) {
+              past = true;
+              continue;
+            }
+            const int dx = xa - xbs[u], dy = ya - ybs[u];
+            if (dy > lim || dy < -lim) continue;
+            const long long n = (long long)dx * dx + (long long)dy * dy;
+            if (n <= cfg.hit_n) {
+#pragma unroll
+              for (int w = 0; w < kAMax / 32; ++w)
+                if (w == (j >> 5)) bits[w] |= 1u << (j & 31);
+              ++cnt;
+              atomicMin(&fh[j], t);  // first hitting projectile row
+            }
           }
+          if (past) break;
         }
 #pragma unroll
         for (int w = 0; w < kAMax / 32; ++w) hits[t][w] = bits[w];
