@@ -77,6 +77,12 @@ void MCnucl::setDensity(int iy, int ipt) {
 // MCnucl.cpp:911-960 (built on the device, kept in the context)
 void MCnucl::makeTable() { check(ctx_, smc_kln_make_table(ctx_, nullptr), "MCnucl::makeTable"); }
 
+// MCnucl.cpp:1177-1211 / 1213-1220
+void MCnucl::dumpBinaryTable(std::string filename) { smc_host::dump_binary_table(ctx_, p_, 0, filename.c_str()); }
+void MCnucl::dumpparticipantTable(std::string filename) {
+  smc_host::dump_participant_table(ctx_, p_, 0, filename.c_str());
+}
+
 void MCnucl::getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_proj, std::vector<double>& xy_targ,
                          std::vector<int>& wound_targ) {
   xy_proj.assign(2 * (size_t)p_.Aproj, 0.0);

@@ -14,6 +14,7 @@
 #define SMC_HOST_MCNUCL_H
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 #include "ParameterReader.h"
@@ -45,6 +46,11 @@ class MCnucl {
   int CentralityCut();
   void setCentralityCut(int Nmin, int Nmax) { npmax_ = Nmax; npmin_ = Nmin; }
   void makeTable();
+
+  // MCnucl.cpp:1177-1220: collision tables of the current event (formats of
+  // the reference; quarks at their nucleon's centre)
+  void dumpBinaryTable(std::string filename);
+  void dumpparticipantTable(std::string filename);
 
   // the event's nucleons in collision order (x, y in fm; wound rank or -1)
   void getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_proj, std::vector<double>& xy_targ,

@@ -153,6 +153,8 @@ void MakeDensity::generate_profile_ebe(int nevent) {
         ++dropped_;
         continue;
       }
+      // MakeDensity.cpp:597: the collision tables of every event
+      smc_host::dump_binary_table(ctx_, p_, e, "data/binary.dat");
       const float* s = &grid[(size_t)e * ncell];
       const char* kinds[2] = {"sd", "ed"};
       const bool on[2] = {use_sd, use_ed};

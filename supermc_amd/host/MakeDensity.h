@@ -7,7 +7,10 @@
 //  generateEccTable(n)      operation 9 (MakeDensity.cpp:2108-2240):
 //                           data/sn_ecc_eccp_<order>.dat (and en_, use_ed)
 //  generate_profile_ebe(n)  operation 1 (MakeDensity.cpp:502-730), pT-
-//                           integrated: data/sd_event_<i>_block.dat / _4col
+//                           integrated: data/sd_event_<i>_block.dat / _4col,
+//                           and per event the collision tables of
+//                           MCnucl::dumpBinaryTable (data/binary.dat,
+//                           wounded.data, quarks.data, nucl1/2.data)
 //
 // Random numbers: Philox stream randomSeed; event i of a run is counter i,
 // so a run is reproducible and independent of gpu_batch_events.

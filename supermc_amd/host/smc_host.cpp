@@ -2,8 +2,13 @@
 
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cstdlib>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
+#include <sstream>
+#include <vector>
 
 namespace smc_host {
 
@@ -75,6 +80,98 @@ void check(smc_ctx* ctx, smc_status st, const char* where) {
   std::cerr << where << ": " << (ctx ? smc_last_error(ctx) : "smc_create failed") << " (status " << st << ")"
             << std::endl;
   std::exit(st == SMC_ERR_MISSING_KEY ? -1 : 1);
+}
+
+namespace {
+
+struct EventNucleons {
+  std::vector<double> xy[2];
+  std::vector<int> wound[2];
+  std::vector<int> order[2];  // participant indices in wound order
+  std::vector<int> pa, pb;    // pair indices
+};
+
+EventNucleons fetch_event(smc_ctx* ctx, const smc_params& p, int ev, bool pairs) {
+  EventNucleons n;
+  for (int nu = 0; nu < 2; ++nu) {
+    n.xy[nu].assign(2 * SMC_MAX_A, 0.0);
+    n.wound[nu].assign(SMC_MAX_A, -1);
+  }
+  check(ctx, smc_get_nucleons(ctx, ev, n.xy[0].data(), n.wound[0].data(), nullptr, n.xy[1].data(),
+                              n.wound[1].data(), nullptr),
+        "dumpBinaryTable");
+  for (int nu = 0; nu < 2; ++nu) {
+    int np = 0;
+    for (int i = 0; i < SMC_MAX_A; ++i) np = std::max(np, n.wound[nu][i] + 1);
+    n.order[nu].assign(np, -1);
+    for (int i = 0; i < SMC_MAX_A; ++i)
+      if (n.wound[nu][i] >= 0) n.order[nu][n.wound[nu][i]] = i;
+  }
+  if (pairs) {
+    int np = 0;
+    n.pa.resize(p.max_pairs);
+    n.pb.resize(p.max_pairs);
+    check(ctx, smc_get_pairs(ctx, ev, p.max_pairs, n.pa.data(), n.pb.data(), nullptr, &np), "dumpBinaryTable");
+    n.pa.resize(np);
+    n.pb.resize(np);
+  }
+  return n;
+}
+
+// Nucleus::dumpParticipants (Nucleus.cpp:753-764)
+void write_participants(std::ofstream& of, const EventNucleons& n) {
+  for (int nu = 0; nu < 2; ++nu)
+    for (int i : n.order[nu])
+      of << std::setprecision(3) << std::setw(10) << n.xy[nu][2 * i] << "   " << std::setprecision(3)
+         << std::setw(10) << n.xy[nu][2 * i + 1] << "   " << nu + 1 << std::endl;
+}
+
+}  // namespace
+
+void dump_binary_table(smc_ctx* ctx, const smc_params& p, int ev, const char* binary_file) {
+  const EventNucleons n = fetch_event(ctx, p, ev, true);
+  {
+    // CollisionPair position: the midpoint of its two nucleons
+    std::ofstream of(binary_file, std::ios_base::app);
+    for (size_t k = 0; k < n.pa.size(); ++k) {
+      const double x = (n.xy[0][2 * n.pa[k]] + n.xy[1][2 * n.pb[k]]) / 2;
+      const double y = (n.xy[0][2 * n.pa[k] + 1] + n.xy[1][2 * n.pb[k] + 1]) / 2;
+      of << std::setprecision(3) << std::setw(10) << x << std::setprecision(3) << std::setw(10) << y << std::endl;
+    }
+  }
+  {
+    std::ofstream of("data/wounded.data");
+    write_participants(of, n);
+  }
+  {
+    // Nucleus::dumpQuarks (Nucleus.cpp:780-797): three quarks per
+    // participant; box = Box2D of side 8 quark_width centred on the quark
+    std::ofstream of("data/quarks.data", std::ios_base::app);
+    const double h = 8 * p.quark_width / 2;
+    for (int nu = 0; nu < 2; ++nu)
+      for (int i : n.order[nu]) {
+        const double x = n.xy[nu][2 * i], y = n.xy[nu][2 * i + 1];
+        std::ostringstream box;
+        box << (-h) + x << " " << h + x << " " << (-h) + y << " " << h + y;
+        for (int q = 0; q < 3; ++q)
+          of << std::setprecision(3) << std::setw(10) << x << std::setprecision(3) << std::setw(10) << y << " "
+             << box.str() << std::endl;
+      }
+  }
+  // Nucleus::dumpNucleons (Nucleus.cpp:766-778): every nucleon, collision order
+  const int A[2] = {p.Aproj, p.Atarg};
+  for (int nu = 0; nu < 2; ++nu) {
+    std::ofstream of(nu == 0 ? "data/nucl1.data" : "data/nucl2.data");
+    for (int i = 0; i < A[nu]; ++i)
+      of << std::setprecision(3) << std::setw(10) << n.xy[nu][2 * i] << std::setprecision(3) << std::setw(10)
+         << n.xy[nu][2 * i + 1] << std::endl;
+  }
+}
+
+void dump_participant_table(smc_ctx* ctx, const smc_params& p, int ev, const char* filename) {
+  const EventNucleons n = fetch_event(ctx, p, ev, false);
+  std::ofstream of(filename, std::ios_base::app);
+  write_participants(of, n);
 }
 
 }  // namespace smc_host

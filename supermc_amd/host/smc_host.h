@@ -23,6 +23,19 @@ uint64_t seed_from_reader(ParameterReader& r);
 // and call exit).  Returns normally when st == SMC_OK.
 void check(smc_ctx* ctx, smc_status st, const char* where);
 
+// Collision tables of event ev of the context, as MCnucl::dumpBinaryTable
+// (MCnucl.cpp:1177-1211) writes them: binary_file (appended) gets the pair
+// midpoints; data/wounded.data the participants of both nuclei in wound
+// order with the nucleus id (1, 2); data/quarks.data (appended) the three
+// valence quarks of each participant with their boxes; data/nucl1.data and
+// data/nucl2.data every nucleon.  Quarks sit at their nucleon's centre (the
+// quark table of the reference is not part of its checkout), so each quark
+// box is the nucleon position +- 4 quark_width.  p: the context's parameters.
+void dump_binary_table(smc_ctx* ctx, const smc_params& p, int ev, const char* binary_file);
+// MCnucl::dumpparticipantTable (MCnucl.cpp:1213-1220): participants of both
+// nuclei, appended to filename.
+void dump_participant_table(smc_ctx* ctx, const smc_params& p, int ev, const char* filename);
+
 }  // namespace smc_host
 
 #endif

@@ -186,6 +186,52 @@ def test_driver_profiles_match_c_abi(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_collision_tables(tmp_path):
+    """operation 1 writes MCnucl::dumpBinaryTable's files per event
+    (MCnucl.cpp:1177-1211, Nucleus.cpp:753-797): binary.dat (appended pair
+    midpoints), wounded.data (last event's participants in wound order with
+    the nucleus id), quarks.data (three quarks per participant, box +- 4
+    quark_width), nucl1/nucl2.data (last event's nucleons in collision order)."""
+    _need(EXE)
+    _write_params(tmp_path, operation=1, nev=3)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=3)
+    with Context(p) as ctx:
+        ctx.run(seed=1234, event0=0, nev=3)
+        nuc = [ctx.nucleons(e) for e in range(3)]
+        prs = [ctx.pairs(e) for e in range(3)]
+    g3 = np.vectorize(lambda v: float(f"{v:.3g}"))  # iostream setprecision(3), default float field
+    d = tmp_path / "data"
+    binary = _read_table(d / "binary.dat")
+    mids = np.concatenate([(nuc[e]["xy_proj"][pa] + nuc[e]["xy_targ"][pb]) / 2 for e, (pa, pb, _) in enumerate(prs)])
+    assert binary.shape == mids.shape
+    np.testing.assert_array_equal(binary, g3(mids))
+    last = nuc[2]
+    wounded = _read_table(d / "wounded.data")
+    want = []
+    for nu, (xy, w) in enumerate(((last["xy_proj"], last["wound_proj"]), (last["xy_targ"], last["wound_targ"]))):
+        order = np.argsort(np.where(w >= 0, w, 1 << 30))[: (w >= 0).sum()]
+        want += [[xy[i, 0], xy[i, 1], nu + 1] for i in order]
+    want = np.array(want)
+    assert wounded.shape == want.shape
+    np.testing.assert_array_equal(wounded[:, :2], g3(want[:, :2]))
+    np.testing.assert_array_equal(wounded[:, 2], want[:, 2])
+    for nu, key in ((1, "xy_proj"), (2, "xy_targ")):
+        t = _read_table(d / f"nucl{nu}.data")
+        assert t.shape == (208, 2)
+        np.testing.assert_array_equal(t, g3(last[key]))
+    quarks = _read_table(d / "quarks.data")
+    npart = sum((n["wound_proj"] >= 0).sum() + (n["wound_targ"] >= 0).sum() for n in nuc)
+    assert quarks.shape == (3 * npart, 6)
+    np.testing.assert_allclose(quarks[:, 3] - quarks[:, 2], 8 * 0.3, rtol=1e-5)  # box side 8 quark_width
+    np.testing.assert_allclose((quarks[:, 2] + quarks[:, 3]) / 2, quarks[:, 0], atol=0.051)
+    np.testing.assert_allclose((quarks[:, 4] + quarks[:, 5]) / 2, quarks[:, 1], atol=0.051)
+
+
+@pytest.mark.gpu
 def test_mcnucl_facade_matches_batched_path(tmp_path):
     _need(CHECK)
     pfile = _write_params(tmp_path)
