@@ -188,6 +188,22 @@ smc_status smc_moments(smc_ctx* ctx, double* ev_out);
 smc_status smc_run(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
 smc_status smc_fetch_events(smc_ctx* ctx, int nev, double* ev_out);
 
+/* Averaged, rotated profiles: the event loop body of
+ * MakeDensity::generate_profile_average (operation 3, MakeDensity.cpp:736-
+ * 1650, pT-integrated) for the ctx's events.  For each order n = from..to in
+ * turn (the sources stay moved from one order to the next, as in the
+ * reference): if rp_sums is non-NULL the sources are recentred on the
+ * density's centre of mass (MCnucl::recenterGrid, MCnucl.cpp:1119-1145) and
+ * finalFactor * rho is added to rp_sums[n-from]; then the sources are
+ * recentred and turned by the order-n angle -atan2(-Im, -Re)/n of
+ * sum r^n e^{i n theta} rho (GlueDensity::calcCMAngle, GlueDensity.cpp:
+ * 87-144; MCnucl::rotateGrid, MCnucl.cpp:1147-1174) and finalFactor * rho is
+ * added to sums[n-from].  sums, rp_sums: host [to-from+1][nx][ny], added
+ * to.  accept: host [nev], nonzero = event included (NULL: status 0).  The
+ * ctx's sources and density are left turned. */
+smc_status smc_average_profiles(smc_ctx* ctx, int from_order, int to_order, const unsigned char* accept,
+                                double* sums, double* rp_sums);
+
 /* MC-KLN: KLNModel::getdNdy (KLNModel.cpp:97-125) at y=0 for n (TA,TB)
  * points, by the Gauss-Legendre product rule on the device. */
 smc_status smc_kln_dndy(smc_ctx* ctx, int n, const double* ta, const double* tb, double* out);

@@ -204,6 +204,59 @@ def density_glauber(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c):
     return s, dndy
 
 
+def cm_angle(g, s, n):
+    """GlueDensity::calcCMAngle (GlueDensity.cpp:87-144): centre of mass with
+    weights rho dx dy over the full grid, then the order-n angle
+    -atan2(-Im, -Re)/n of sum r^n e^{i n theta} rho about it (atan2, cos, sin
+    and pow as the reference writes them)."""
+    x = g.xmin + np.arange(g.nx)[:, None] * g.dx + 0.0 * np.arange(g.ny)[None, :]
+    y = g.ymin + np.arange(g.ny)[None, :] * g.dy + 0.0 * np.arange(g.nx)[:, None]
+    wei = s * g.dx * g.dy
+    weight = wei.sum()
+    xc, yc = (x * wei).sum() / weight, (y * wei).sum() / weight
+    xr, yr = x - xc, y - yc
+    th = np.arctan2(yr, xr)
+    rwei = np.power(np.sqrt(xr * xr + yr * yr), n)
+    re = (rwei * np.cos(n * th) * s).sum()
+    im = (rwei * np.sin(n * th) * s).sum()
+    return xc, yc, -np.arctan2(-im, -re) / n
+
+
+def rotated_profiles(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c, from_order, to_order, ff, reaction_plane=False):
+    """One event of MakeDensity::generate_profile_average (operation 3,
+    MakeDensity.cpp:1266-1345, Glauber, pT-integrated): for each order in
+    turn, optionally recentre (MCnucl::recenterGrid, MCnucl.cpp:1119-1145)
+    and record ff * rho, then recentre and turn the sources by the order-n
+    angle (MCnucl::rotateGrid, MCnucl.cpp:1147-1174, Point3D::rotate with
+    theta = 0) and record ff * rho.  The sources stay moved between orders.
+    Returns (profiles [norder][nx][ny], recentred profiles or None)."""
+    pts = [np.array(xy_a, np.float64).reshape(-1, 2), np.array(xy_b, np.float64).reshape(-1, 2),
+           np.array(xy_c, np.float64).reshape(-1, 2)]
+
+    def dens():
+        return density_glauber(g, w, alpha, pts[0], f_a, pts[1], f_b, pts[2], f_c)[0]
+
+    out, rp = [], []
+    s = dens()
+    for n in range(from_order, to_order + 1):
+        if reaction_plane:
+            xc, yc, _ = cm_angle(g, s, n)
+            for q in pts:
+                q[:, 0] -= xc
+                q[:, 1] -= yc
+            s = dens()
+            rp.append(ff * s)
+        xc, yc, ang = cm_angle(g, s, n)
+        cp, sp = np.cos(ang), np.sin(ang)
+        for q in pts:
+            x0, y0 = q[:, 0] - xc, q[:, 1] - yc
+            q[:, 0] = 1.0 * cp * x0 - sp * y0
+            q[:, 1] = 1.0 * sp * x0 + cp * y0
+        s = dens()
+        out.append(ff * s)
+    return np.array(out), (np.array(rp) if reaction_plane else None)
+
+
 def density_trento(g, w, xy_a, f_a, xy_b, f_b):
     def cols(xy):
         xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)

@@ -90,6 +90,7 @@ SIGNATURES = {
     "smc_moments": (C.c_int, [C.c_void_p, _dp]),
     "smc_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
     "smc_fetch_events": (C.c_int, [C.c_void_p, C.c_int, _dp]),
+    "smc_average_profiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp, _dp]),
     "smc_kln_dndy": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "smc_kln_make_table": (C.c_int, [C.c_void_p, _dp]),
     "smc_diff_setup": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
@@ -260,6 +261,23 @@ class Context:
         ev = np.zeros((nev, EV_STRIDE))
         self._check(self.lib.smc_fetch_events(self.h, nev, _ptr(ev, C.c_double)))
         return ev
+
+    def average_profiles(self, from_order, to_order, accept=None, reaction_plane=False):
+        """Sums over the ctx's accepted events of finalFactor * rho after the
+        order-n recentring and turn (and the recentred-only sums if
+        reaction_plane); see smc_average_profiles."""
+        nord = to_order - from_order + 1
+        sums = np.zeros((nord, self.nx, self.ny))
+        rp = np.zeros((nord, self.nx, self.ny)) if reaction_plane else None
+        acc = None
+        if accept is not None:
+            acc = np.ascontiguousarray(np.asarray(accept, dtype=np.uint8))
+            if acc.shape != (self.nev,):
+                raise ValueError("accept must have one entry per event")
+        self._check(self.lib.smc_average_profiles(self.h, from_order, to_order,
+                                                  None if acc is None else acc.ctypes.data_as(C.c_void_p),
+                                                  _ptr(sums, C.c_double), _ptr(rp, C.c_double)))
+        return (sums, rp) if reaction_plane else sums
 
     def kln_dndy(self, ta, tb):
         ta = np.ascontiguousarray(ta, np.float64)

@@ -143,6 +143,10 @@ struct smc_ctx {
   double* band = nullptr;   // moment partials [cap][moments_band_doubles()]
   int cm_nu = 0;            // units per event in cm for the current grid (0: none)
   void* mrects = nullptr;   // hot-spot mask rectangles [cap][2 kAMax] (8 bytes each)
+  double* avg_rot = nullptr;          // averaged profiles: per event {xc, yc, angle, total}
+  unsigned char* avg_acc = nullptr;   // per event: included in the average
+  double* avg_sum = nullptr;          // [2][avg_nord][nx][ny] sums (rotated, recentred)
+  int avg_nord = 0;
   int* nmrect = nullptr;    // [cap]
   double* kln_table = nullptr;
   // MC-KLN quadrature nodes (FP32 for the per-cell kernel, FP64 for points)
@@ -649,6 +653,67 @@ void run_event(smc_ctx* c, int nev, uint64_t seed, uint64_t event0, int fed) {
   c->have_density = c->have_thickness = c->have_moments = 0;
 }
 
+// ---------------------------------------------------------------- averaged profiles
+// Operation 3 (MakeDensity.cpp:736-1650, pT-integrated): for each order
+// from..to in turn, optionally the recentred profile (reaction plane,
+// generate_reaction_plane_avg_profile), then the profile recentred and
+// turned to the order-n angle; the sources stay moved from one order to the
+// next as in the reference.  Sums of finalFactor * rho over accepted events.
+void ensure_cm(smc_ctx* c) {
+  if (c->cm_nu) return;
+  check_launch(launch_cm_partials(c->cfg, c->buf, c->grid, c->nev, c->cm, c->stream), "k_mom_cm");
+  c->cm_nu = moments_cm_units_min();
+}
+
+void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept, double* sums, double* rp_sums) {
+  if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_run first");
+  if (from < 1 || to < from || to > 9) throw SmcError(SMC_ERR_ARG, "orders must satisfy 1 <= from <= to <= 9");
+  if (!sums) throw SmcError(SMC_ERR_ARG, "sums is NULL");
+  const int nev = c->nev, nord = to - from + 1;
+  const size_t N = (size_t)c->d.nx * c->d.ny;
+  if (!c->avg_rot) {
+    c->avg_rot = dalloc<double>((size_t)c->cap * 4);
+    c->avg_acc = dalloc<unsigned char>((size_t)c->cap);
+  }
+  if (c->avg_nord < nord) {
+    if (c->avg_sum) HIPCHK(hipFree(c->avg_sum));
+    c->avg_sum = dalloc<double>(2 * (size_t)nord * N);
+    c->avg_nord = nord;
+  }
+  std::vector<unsigned char> acc(nev);
+  if (accept) {
+    for (int e = 0; e < nev; ++e) acc[e] = accept[e] ? 1 : 0;
+  } else {
+    std::vector<int> cnt((size_t)nev * 8);
+    HIPCHK(hipMemcpyAsync(cnt.data(), c->buf.counts, cnt.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    for (int e = 0; e < nev; ++e) acc[e] = cnt[(size_t)e * 8 + 3] == 0;
+  }
+  HIPCHK(hipMemcpyAsync(c->avg_acc, acc.data(), nev, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemsetAsync(c->avg_sum, 0, 2 * (size_t)nord * N * sizeof(double), c->stream));
+  if (!c->have_density) run_density(c);
+  const double ff = c->p.finalFactor;
+  for (int o = 0; o < nord; ++o) {
+    const int n = from + o;
+    for (int pass = rp_sums ? 0 : 1; pass < 2; ++pass) {  // 0: recentre only, 1: recentre and turn
+      ensure_cm(c);
+      check_launch(launch_cm_angle(c->cfg, c->buf, c->grid, c->cm, c->cm_nu, n, pass, nev, c->avg_rot, c->stream),
+                   "k_cm_angle");
+      check_launch(launch_rotate_src(c->cfg, c->buf, c->avg_rot, nev, c->stream), "k_rotate_src");
+      c->have_density = c->have_thickness = c->have_moments = 0;
+      run_density(c);
+      double* dst = c->avg_sum + ((size_t)(pass == 0 ? nord : 0) + o) * N;
+      check_launch(launch_accum_avg(c->grid, N, nev, c->avg_acc, ff, dst, c->stream), "k_accum_avg");
+    }
+  }
+  std::vector<double> h(2 * (size_t)nord * N);
+  HIPCHK(hipMemcpyAsync(h.data(), c->avg_sum, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  sync(c);
+  for (size_t i = 0; i < (size_t)nord * N; ++i) sums[i] += h[i];
+  if (rp_sums)
+    for (size_t i = 0; i < (size_t)nord * N; ++i) rp_sums[i] += h[(size_t)nord * N + i];
+}
+
 // ---------------------------------------------------------------- differentiable batches
 void diff_check_params(const smc_ctx* c) {
   const smc_params& p = c->p;
@@ -905,7 +970,7 @@ void smc_destroy(smc_ctx* c) {
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
-                  c->cm, c->band, c->mrects, c->nmrect, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
+                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (void* q : c->dif_alloc)
@@ -1102,6 +1167,15 @@ smc_status smc_moments(smc_ctx* c, double* ev_out) {
       HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)c->nev * SMC_EV_STRIDE * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
     sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_average_profiles(smc_ctx* c, int from_order, int to_order, const unsigned char* accept, double* sums,
+                                double* rp_sums) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    average_profiles(c, from_order, to_order, accept, sums, rp_sums);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }

@@ -159,6 +159,14 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
                    void* stream);
 int moments_band_doubles();
 int moments_cm_units_min();
+// centre-of-mass partials of grids from another producer: moments_cm_units_min() units per event
+int launch_cm_partials(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* cm, void* stream);
+// averaged rotated profiles (smc_avg.hip)
+int launch_cm_angle(const DevCfg& cfg, const EvBufs& b, const float* grid, const double* cm, int nu, int n,
+                    int angle_on, int nev, double* rot, void* stream);
+int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int nev, void* stream);
+int launch_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept, double ff, double* sum,
+                     void* stream);
 int tile_units(const DevCfg& cfg);  // smearing tiles per event (units of SmearSpec::cm_out)
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                  void* stream);

@@ -683,6 +683,14 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
   return (int)hipGetLastError();
 }
 
+int launch_cm_partials(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* cm, void* stream) {
+  if (nev <= 0) return 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (cfg.ny <= 320) hipLaunchKernelGGL(k_mom_cm<5>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
+  else hipLaunchKernelGGL(k_mom_cm<9>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
+  return (int)hipGetLastError();
+}
+
 int moments_band_doubles() { return kNB * kNVMax; }
 int moments_cm_units_min() { return kNB; }
 

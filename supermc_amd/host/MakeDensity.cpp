@@ -110,26 +110,45 @@ void MakeDensity::generateEccTable(int nevent) {
   }
 }
 
-void MakeDensity::dumpDensityBlock(const char* filename, const float* s, double factor) {
-  // MakeDensity.cpp:2595-2608
+namespace {
+// MakeDensity.cpp:2595-2608
+template <class T>
+void write_block(const char* filename, const T* s, double factor, int nx, int ny) {
   std::ofstream of(filename, std::ios_base::out);
-  for (int i = 0; i < nx_; ++i) {
-    for (int j = 0; j < ny_; ++j) of << std::setprecision(12) << std::setw(22) << (double)s[(size_t)i * ny_ + j] * factor;
+  for (int i = 0; i < nx; ++i) {
+    for (int j = 0; j < ny; ++j) of << std::setprecision(12) << std::setw(22) << (double)s[(size_t)i * ny + j] * factor;
     of << std::endl;
   }
 }
 
-void MakeDensity::dumpDensity4Col(const char* filename, const float* s, double factor, double npart) {
-  // MakeDensity.cpp:2573-2593 (one rapidity bin at y = 0)
+// MakeDensity.cpp:2573-2593 (one rapidity bin at y = 0)
+template <class T>
+void write_4col(const char* filename, const T* s, double factor, double npart, int nx, int ny, double xmin,
+                double ymin, double dx, double dy) {
   std::ofstream of(filename, std::ios_base::out);
-  of << "# <npart>= " << npart << " xmax= " << nx_ << " ymax= " << ny_ << std::endl;
-  for (int i = 0; i < nx_; ++i)
-    for (int j = 0; j < ny_; ++j) {
-      const double x = xmin_ + i * p_.dx, y = ymin_ + j * p_.dy, rap = 0.0;
+  of << "# <npart>= " << npart << " xmax= " << nx << " ymax= " << ny << std::endl;
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const double x = xmin + i * dx, y = ymin + j * dy, rap = 0.0;
       of << std::setprecision(3) << std::setw(10) << rap << std::setprecision(3) << std::setw(10) << x
          << std::setprecision(3) << std::setw(10) << y << std::setprecision(12) << std::setw(22)
-         << (double)s[(size_t)i * ny_ + j] * factor << std::endl;
+         << (double)s[(size_t)i * ny + j] * factor << std::endl;
     }
+}
+}  // namespace
+
+void MakeDensity::dumpDensityBlock(const char* filename, const float* s, double factor) {
+  write_block(filename, s, factor, nx_, ny_);
+}
+
+void MakeDensity::dumpDensity4Col(const char* filename, const float* s, double factor, double npart) {
+  write_4col(filename, s, factor, npart, nx_, ny_, xmin_, ymin_, p_.dx, p_.dy);
+}
+
+void MakeDensity::dumpDensityBlock(const char* filename, const double* s) { write_block(filename, s, 1.0, nx_, ny_); }
+
+void MakeDensity::dumpDensity4Col(const char* filename, const double* s, double npart) {
+  write_4col(filename, s, 1.0, npart, nx_, ny_, xmin_, ymin_, p_.dx, p_.dy);
 }
 
 void MakeDensity::generate_profile_ebe(int nevent) {
@@ -173,4 +192,67 @@ void MakeDensity::generate_profile_ebe(int nevent) {
       ++event;
     }
   }
+}
+
+void MakeDensity::generate_profile_average(int nevent) {
+  const int from = (int)rdr_->getVal("average_from_order"), to = (int)rdr_->getVal("average_to_order");
+  const bool use_sd = rdr_->getVal("use_sd") != 0, use_ed = rdr_->getVal("use_ed") != 0;
+  const bool use_block = rdr_->getVal("use_block") != 0, use_4col = rdr_->getVal("use_4col") != 0;
+  const bool rp = rdr_->getVal("generate_reaction_plane_avg_profile") != 0;
+  const char* extras[4] = {"output_TATB", "output_rho_binary", "output_TA", "output_spectator_density"};
+  for (const char* k : extras)
+    if (rdr_->getVal(k) != 0) {
+      std::cerr << "generate_profile_average: " << k << " = 1 is not offered by this build" << std::endl;
+      std::exit(1);
+    }
+  if (from < 1 || to < from || to > 9) {
+    std::cerr << "generate_profile_average: average orders must satisfy 1 <= from <= to <= 9" << std::endl;
+    std::exit(1);
+  }
+  const int nord = to - from + 1;
+  const size_t ncell = (size_t)nx_ * ny_;
+  std::vector<double> sums(nord * ncell, 0.0), rp_sums(rp ? nord * ncell : 0, 0.0);
+  std::vector<unsigned char> acc;
+  records_.clear();
+  int done = 0;
+  double npart_last = 0.0;
+  while (done < nevent) {
+    const int n = next_batch(nevent - done);
+    acc.assign(n, 0);
+    for (int e = 0; e < n && done < nevent; ++e) {
+      const double* rec = &batch_rec_[(size_t)e * SMC_EV_STRIDE];
+      if (!accept(rec)) {
+        ++dropped_;
+        continue;
+      }
+      acc[e] = 1;
+      npart_last = rec[SMC_EV_NPART];
+      records_.insert(records_.end(), rec, rec + SMC_EV_STRIDE);
+      ++done;
+    }
+    check(ctx_, smc_average_profiles(ctx_, from, to, acc.data(), sums.data(), rp ? rp_sums.data() : nullptr),
+          "MakeDensity::generate_profile_average");
+  }
+  // the reference's running average over accepted events
+  for (double& v : sums) v /= nevent;
+  for (double& v : rp_sums) v /= nevent;
+  char buffer[512];
+  const char* kinds[2] = {"sd", "ed"};
+  const bool on[2] = {use_sd, use_ed};
+  for (int o = 0; o < nord; ++o)
+    for (int k = 0; k < 2; ++k) {
+      if (!on[k]) continue;
+      for (int r = 0; r < (rp ? 2 : 1); ++r) {
+        const double* g = (r ? rp_sums.data() : sums.data()) + o * ncell;
+        const char* tag = r ? "_RP" : "";
+        if (use_4col) {
+          std::snprintf(buffer, sizeof(buffer), "data/%sAvg%s_order_%d_4col.dat", kinds[k], tag, from + o);
+          dumpDensity4Col(buffer, g, npart_last);
+        }
+        if (use_block) {
+          std::snprintf(buffer, sizeof(buffer), "data/%sAvg%s_order_%d_block.dat", kinds[k], tag, from + o);
+          dumpDensityBlock(buffer, g);
+        }
+      }
+    }
 }

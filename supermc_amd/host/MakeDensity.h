@@ -11,6 +11,16 @@
 //                           and per event the collision tables of
 //                           MCnucl::dumpBinaryTable (data/binary.dat,
 //                           wounded.data, quarks.data, nucl1/2.data)
+//  generate_profile_average(n)  operation 3 (MakeDensity.cpp:736-1650),
+//                           pT-integrated: averages over n accepted events
+//                           of the profiles recentred and turned to each
+//                           order's angle, data/sdAvg_order_<n>_block.dat /
+//                           _4col (edAvg_ with use_ed; sdAvg_RP_ with
+//                           generate_reaction_plane_avg_profile).  The
+//                           optional TA*TB, binary-density, TA and
+//                           spectator averages are not offered (exit 1).
+//                           The dS/dy cut is applied once per event, on
+//                           the unturned profile.
 //
 // Random numbers: Philox stream randomSeed; event i of a run is counter i,
 // so a run is reproducible and independent of gpu_batch_events.
@@ -33,6 +43,7 @@ class MakeDensity {
 
   void generateEccTable(int nevent);
   void generate_profile_ebe(int nevent);
+  void generate_profile_average(int nevent);
 
   // Writes the eccentricity lines of one event record (the SMC_EV_* layout
   // of smc.h) in the format of MakeDensity.cpp:2432-2500.
@@ -40,6 +51,8 @@ class MakeDensity {
                           double Npart_current, double Nbin_current, double b);
   void dumpDensityBlock(const char* filename, const float* s, double factor);
   void dumpDensity4Col(const char* filename, const float* s, double factor, double npart);
+  void dumpDensityBlock(const char* filename, const double* s);
+  void dumpDensity4Col(const char* filename, const double* s, double npart);
 
   // events accepted by the last generate call, and their records
   const std::vector<double>& records() const { return records_; }

@@ -1,6 +1,7 @@
 // superMC.e -- the reference's driver (main.cpp) on the MI355X path:
 // parameters.dat, then "key=value" arguments, then operation 9
-// (eccentricity table) or 1 (event-by-event profiles).
+// (eccentricity table), 1 (event-by-event profiles) or 3 (averaged
+// rotated profiles).
 #include <cstdlib>
 #include <ctime>
 #include <iostream>
@@ -23,11 +24,14 @@ int main(int argc, char* argv[]) {
       case 1:
         dens.generate_profile_ebe(nevent);
         break;
+      case 3:
+        dens.generate_profile_average(nevent);
+        break;
       case 9:
         dens.generateEccTable(nevent);
         break;
       default:
-        std::cout << "Error: operation choice " << operation << " not recognized (this build runs 1 and 9)."
+        std::cout << "Error: operation choice " << operation << " not recognized (this build runs 1, 3 and 9)."
                   << std::endl;
     }
   }

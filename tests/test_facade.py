@@ -232,6 +232,40 @@ def test_driver_collision_tables(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_average_profiles_match_c_abi(tmp_path):
+    """operation 3: data/sdAvg_order_<n>_block.dat and sdAvg_RP_... are the
+    averages over the first nev events of smc_average_profiles' sums, and
+    the batching (gpu_batch_events = 7) does not change them."""
+    _need(EXE)
+    _write_params(tmp_path, operation=3, nev=10, average_from_order=2, average_to_order=3,
+                  generate_reaction_plane_avg_profile=1, output_TATB=0, output_rho_binary=0, output_TA=0,
+                  output_spectator_density=0, use_4col=1, finalFactor=1.3)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=10, finalFactor=1.3)
+    with Context(p) as ctx:
+        rec = ctx.run(seed=1234, event0=0, nev=10)
+        assert (rec[:, 57] == 0).all()
+        sums, rp = ctx.average_profiles(2, 3, reaction_plane=True)
+    for o, n in enumerate((2, 3)):
+        blk = _read_table(tmp_path / "data" / f"sdAvg_order_{n}_block.dat")
+        np.testing.assert_allclose(blk, sums[o] / 10, rtol=1e-10, atol=1e-300)
+        blk = _read_table(tmp_path / "data" / f"sdAvg_RP_order_{n}_block.dat")
+        np.testing.assert_allclose(blk, rp[o] / 10, rtol=1e-10, atol=1e-300)
+        col = _read_table(tmp_path / "data" / f"sdAvg_order_{n}_4col.dat")
+        assert col.shape == (261 * 261, 4)
+        np.testing.assert_allclose(col[:, 3], sums[o].ravel() / 10, rtol=1e-10)
+    # an optional average this build does not offer stops the run
+    _write_params(tmp_path, operation=3, nev=2, average_from_order=2, average_to_order=2,
+                  generate_reaction_plane_avg_profile=0, output_TATB=1, output_rho_binary=0, output_TA=0,
+                  output_spectator_density=0)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "output_TATB" in r.stderr
+
+
+@pytest.mark.gpu
 def test_mcnucl_facade_matches_batched_path(tmp_path):
     _need(CHECK)
     pfile = _write_params(tmp_path)

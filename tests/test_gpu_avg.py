@@ -1,0 +1,85 @@
+"""Averaged, rotated profiles (operation 3, MakeDensity.cpp:736-1650,
+pT-integrated): smc_average_profiles against the FP64 oracle's
+rotated_profiles on the same nucleons and Gamma factors.
+
+Bars: the sum over events of finalFactor * rho after the order-n recentring
+and turn, and of the recentred-only (reaction-plane) profile: relative L-inf
+(max|d| / max|ref|) <= 1e-5, as for single grids.  The device turns the
+sources from its FP32 grid and keeps positions in half-quanta (2^-25 fm),
+the oracle from its FP64 grid and FP64 positions; both are far below the bar.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import parity_helpers as H
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _oracle_sum(ctx, p, events, from_order, to_order, rp):
+    g = oracle.grid(p.maxx, p.dx, p.maxy, p.dy)
+    tot = np.zeros((to_order - from_order + 1, g.nx, g.ny))
+    tot_rp = np.zeros_like(tot)
+    for e in events:
+        o = H.oracle_event(ctx, e)
+        nuc = o["nuc"]
+        pa, pb, pf = o["gpu_pairs"]
+        xa, xb = nuc["xy_proj"], nuc["xy_targ"]
+        ia, ib = o["ia"], o["ib"]
+        mid = (xa[pa] + xb[pb]) / 2.0
+        prof, rprof = oracle.rotated_profiles(g, ctx.derived.width, p.alpha, xa[ia], nuc["fluct_proj"][ia].astype(float),
+                                              xb[ib], nuc["fluct_targ"][ib].astype(float), mid, pf.astype(float),
+                                              from_order, to_order, p.finalFactor, reaction_plane=rp)
+        tot += prof
+        if rp:
+            tot_rp += rprof
+    return tot, tot_rp
+
+
+def _rel(a, b):
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.mark.parametrize("rp", [False, True])
+def test_rotated_average_matches_oracle(rp):
+    from supermc_amd import Context
+    p = H.glauber_params("PbPb", max_events=6, finalFactor=1.7)
+    bvals = [0.0, 2.0, 4.5, 7.0, 9.5, 12.0]
+    xa, xb, bs = H.cpu_configs(p, len(bvals), 41, bvals)
+    with Context(p) as ctx:
+        ctx.set_positions(xa, xb, bs, fluct_seed=5)
+        npa, npb, nc, _ = ctx.collide(allow=(4,))
+        accept = (nc > 0).astype(np.uint8)
+        accept[3] = 0  # an excluded event stays out of the sums
+        events = [e for e in range(len(bvals)) if accept[e]]
+        # the oracle works from the unturned nucleons: take them first
+        ref, ref_rp = _oracle_sum(ctx, p, events, 2, 4, rp)
+        out = ctx.average_profiles(2, 4, accept=accept, reaction_plane=rp)
+    sums, sums_rp = out if rp else (out, None)
+    for o in range(3):
+        assert _rel(sums[o], ref[o]) <= TOL, (o, _rel(sums[o], ref[o]))
+        if rp:
+            assert _rel(sums_rp[o], ref_rp[o]) <= TOL, (o, _rel(sums_rp[o], ref_rp[o]))
+    # turned profiles: the order-2 axis lies on y (the reference's convention)
+    g = oracle.grid(p.maxx, p.dx, p.maxy, p.dy)
+    x = g.xmin + np.arange(g.nx)[:, None] * g.dx
+    y = g.ymin + np.arange(g.ny)[None, :] * g.dy
+    w = sums[0] / sums[0].sum()
+    assert ((y ** 2) * w).sum() > ((x ** 2) * w).sum()
+
+
+def test_rotated_average_sampled_events_status_default():
+    """Sampled events, accept = status 0 (NULL): finite, non-negative sums
+    whose total equals the turned events' dS/dy up to the footprints that
+    leave the grid."""
+    from supermc_amd import Context
+    p = H.glauber_params("PbPb", max_events=64, bmin=0.0, bmax=10.0)
+    with Context(p) as ctx:
+        rec = ctx.run(seed=77, event0=0, nev=64)
+        sums = ctx.average_profiles(2, 2)
+    assert np.isfinite(sums).all() and (sums >= 0).all()
+    tot = sums[0].sum() * p.dx * p.dy
+    assert tot == pytest.approx(rec[rec[:, 57] == 0, 53].sum(), rel=1e-3)

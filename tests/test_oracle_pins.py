@@ -263,3 +263,58 @@ def test_gamma_sampler_law(orc):
     for k, th in ((0.344, 2.3), (0.112, 8.9), (2.5, 0.4)):
         v = [rng.gamma(k, th) for _ in range(20000)]
         assert stats.kstest(v, stats.gamma(k, scale=th).cdf).pvalue > 1e-3
+
+
+def _blob_pair(d, phi):
+    c, s = math.cos(phi), math.sin(phi)
+    return np.array([[d * c, d * s], [-d * c, -d * s]])
+
+
+def _second_moments(g, prof):
+    x = g.xmin + np.arange(g.nx)[:, None] * g.dx
+    y = g.ymin + np.arange(g.ny)[None, :] * g.dy
+    w = prof / prof.sum()
+    xc, yc = (x * w).sum(), (y * w).sum()
+    return xc, yc, ((x - xc) ** 2 * w).sum(), ((y - yc) ** 2 * w).sum(), ((x - xc) * (y - yc) * w).sum()
+
+
+def test_rotated_profiles_turn_the_order_n_axis_onto_y(orc):
+    """calcCMAngle + rotateGrid: -atan2(-Im, -Re)/n turns the order-2 axis of
+    two blobs onto the y axis from any starting angle (a sign or factor
+    error in the angle turns it elsewhere), and the result does not depend
+    on where the pair sat (rotation covariance)."""
+    g = orc.grid(13.0, 0.1)
+    no = np.zeros((0, 2))
+    profs = []
+    for phi in (0.0, 0.3, 1.1, 2.0, -0.7):
+        pts = _blob_pair(2.0, phi) + np.array([0.8, -0.5])  # off centre: recentring is tested too
+        prof, _ = orc.rotated_profiles(g, 0.5, 0.0, pts, np.ones(2), no, np.zeros(0), no, np.zeros(0), 2, 2, 1.0)
+        xc, yc, sxx, syy, sxy = _second_moments(g, prof[0])
+        assert abs(xc) < 0.02 and abs(yc) < 0.02
+        assert syy > sxx + 3.0 and abs(sxy) < 0.02
+        profs.append(prof[0])
+    for p in profs[1:]:
+        assert np.abs(p - profs[0]).max() < 0.02 * profs[0].max()
+
+
+def test_rotated_profiles_order3_and_reaction_plane(orc):
+    """Order 3 on a triangle: the angle -atan2(-Im, -Re)/3 brings a vertex
+    to a fixed direction whatever the starting angle; the reaction-plane
+    profile is the recentred, unturned one."""
+    g = orc.grid(13.0, 0.1)
+    no = np.zeros((0, 2))
+    ang = np.array([0, 2 * np.pi / 3, 4 * np.pi / 3])
+    outs = []
+    for phi in (0.0, 0.4, 1.3):
+        pts = 2.0 * np.stack([np.cos(ang + phi), np.sin(ang + phi)], axis=1) + np.array([0.3, 0.6])
+        prof, rp = orc.rotated_profiles(g, 0.5, 0.0, pts, np.ones(3), no, np.zeros(0), no, np.zeros(0), 3, 3, 2.0,
+                                        reaction_plane=True)
+        outs.append(prof[0])
+        # reaction plane: recentred only; its order-3 axis still points along phi
+        xc, yc, *_ = _second_moments(g, rp[0])
+        assert abs(xc) < 0.02 and abs(yc) < 0.02
+        ref, _ = orc.density_glauber(g, 0.5, 0.0, pts - np.array([0.3, 0.6]), np.ones(3), no, np.zeros(0), no,
+                                     np.zeros(0))
+        assert np.abs(rp[0] - 2.0 * ref).max() < 0.02 * rp[0].max()
+    for p in outs[1:]:
+        assert np.abs(p - outs[0]).max() < 0.02 * outs[0].max()
