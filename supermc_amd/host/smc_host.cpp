@@ -109,9 +109,10 @@ EventNucleons fetch_event(smc_ctx* ctx, const smc_params& p, int ev, bool pairs)
   }
   if (pairs) {
     int np = 0;
-    n.pa.resize(p.max_pairs);
-    n.pb.resize(p.max_pairs);
-    check(ctx, smc_get_pairs(ctx, ev, p.max_pairs, n.pa.data(), n.pb.data(), nullptr, &np), "dumpBinaryTable");
+    const int cap = p.Aproj * p.Atarg;  // every possible pair
+    n.pa.resize(cap);
+    n.pb.resize(cap);
+    check(ctx, smc_get_pairs(ctx, ev, cap, n.pa.data(), n.pb.data(), nullptr, &np), "dumpBinaryTable");
     n.pa.resize(np);
     n.pb.resize(np);
   }
