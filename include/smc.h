@@ -188,6 +188,16 @@ smc_status smc_moments(smc_ctx* ctx, double* ev_out);
 smc_status smc_run(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
 smc_status smc_fetch_events(smc_ctx* ctx, int nev, double* ev_out);
 
+/* Operation 2 grids (MakeDensity::generate_profile_ebe_Jet).
+ * smc_rho_binary: MCnucl::calculate_rho_binary (MCnucl.cpp:481-527), the sum
+ * over collision pairs of the normalised nucleon Gaussian (+-5w window,
+ * d^2 <= 25 w^2).  smc_spectator_density: MCnucl::calculate_spectator_density
+ * (MCnucl.cpp:530-614), the same Gaussians on the nucleons without a
+ * collision, projectile and target apart.  Host grids [nev][nx][ny] (NULL:
+ * device only). */
+smc_status smc_rho_binary(smc_ctx* ctx, float* rho);
+smc_status smc_spectator_density(smc_ctx* ctx, float* spec_proj, float* spec_targ);
+
 /* Averaged, rotated profiles: the event loop body of
  * MakeDensity::generate_profile_average (operation 3, MakeDensity.cpp:736-
  * 1650, pT-integrated) for the ctx's events.  For each order n = from..to in

@@ -91,6 +91,8 @@ SIGNATURES = {
     "smc_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
     "smc_fetch_events": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "smc_average_profiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp, _dp]),
+    "smc_rho_binary": (C.c_int, [C.c_void_p, _fp]),
+    "smc_spectator_density": (C.c_int, [C.c_void_p, _fp, _fp]),
     "smc_kln_dndy": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
     "smc_kln_make_table": (C.c_int, [C.c_void_p, _dp]),
     "smc_diff_setup": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
@@ -261,6 +263,17 @@ class Context:
         ev = np.zeros((nev, EV_STRIDE))
         self._check(self.lib.smc_fetch_events(self.h, nev, _ptr(ev, C.c_double)))
         return ev
+
+    def rho_binary(self):
+        rho = np.zeros((self.nev, self.nx, self.ny), np.float32)
+        self._check(self.lib.smc_rho_binary(self.h, _ptr(rho, C.c_float)))
+        return rho
+
+    def spectator_density(self):
+        a = np.zeros((self.nev, self.nx, self.ny), np.float32)
+        b = np.zeros((self.nev, self.nx, self.ny), np.float32)
+        self._check(self.lib.smc_spectator_density(self.h, _ptr(a, C.c_float), _ptr(b, C.c_float)))
+        return a, b
 
     def average_profiles(self, from_order, to_order, accept=None, reaction_plane=False):
         """Sums over the ctx's accepted events of finalFactor * rho after the

@@ -143,6 +143,11 @@ struct smc_ctx {
   double* band = nullptr;   // moment partials [cap][moments_band_doubles()]
   int cm_nu = 0;            // units per event in cm for the current grid (0: none)
   void* mrects = nullptr;   // hot-spot mask rectangles [cap][2 kAMax] (8 bytes each)
+  float* aux0 = nullptr;              // rho_binary / spectator A grids [cap][nx][ny]
+  float* aux1 = nullptr;              // spectator B grids
+  int4* spec_src = nullptr;           // spectator source lists [cap][2 kAMax]
+  int4* spec_swin = nullptr;
+  int* spec_counts = nullptr;         // [cap][8] (counts of the spectator lists)
   double* avg_rot = nullptr;          // averaged profiles: per event {xc, yc, angle, total}
   unsigned char* avg_acc = nullptr;   // per event: included in the average
   double* avg_sum = nullptr;          // [2][avg_nord][nx][ny] sums (rotated, recentred)
@@ -653,6 +658,46 @@ void run_event(smc_ctx* c, int nev, uint64_t seed, uint64_t event0, int fed) {
   c->have_density = c->have_thickness = c->have_moments = 0;
 }
 
+// ---------------------------------------------------------------- operation 2 grids
+// MCnucl::calculate_rho_binary (MCnucl.cpp:481-527): the normalised Gaussian
+// of every collision pair, +-5w window, d^2 <= 25 w^2, no fluctuation factor.
+void run_rho_binary(smc_ctx* c, float* out) {
+  SmearSpec s{};
+  s.enabled[kSrcPair] = 1;
+  s.win[kSrcPair] = kWin5;
+  s.scale[kSrcPair] = c->cfg.norm;
+  run_smear(c, s, out, nullptr);
+}
+
+// MCnucl::calculate_spectator_density (MCnucl.cpp:530-614): the same
+// Gaussians on the nucleons without a collision, projectile (rapidity > 0)
+// and target separately.
+void run_spectator_density(smc_ctx* c, float* out_a, float* out_b) {
+  const size_t E = (size_t)c->cap, cap = 2 * kAMax;
+  if (!c->spec_src) {
+    c->spec_src = dalloc<int4>(E * cap);
+    c->spec_swin = dalloc<int4>(E * cap);
+    c->spec_counts = dalloc<int>(E * 8);
+  }
+  EvBufs sb = c->buf;
+  sb.src = c->spec_src;
+  sb.swin = c->spec_swin;
+  sb.counts = c->spec_counts;
+  sb.src_cap = (int)cap;
+  TimerScope ts(c, SMC_K_SMEAR);
+  check_launch(launch_spectator_src(c->cfg, c->buf, sb, c->nev, c->stream), "k_spectator_src");
+  SmearSpec s{};
+  for (int k : {kSrcPartA, kSrcPartB}) {
+    s.enabled[k] = 1;
+    s.win[k] = kWin5;
+    s.scale[k] = c->cfg.norm;
+  }
+  s.acc[kSrcPartB] = 1;
+  s.combine = 2;
+  check_launch(launch_tile(c->cfg, sb, s, c->nev, out_a, out_b, c->stream), "k_tile");
+  ts.done();
+}
+
 // ---------------------------------------------------------------- averaged profiles
 // Operation 3 (MakeDensity.cpp:736-1650, pT-integrated): for each order
 // from..to in turn, optionally the recentred profile (reaction plane,
@@ -970,7 +1015,8 @@ void smc_destroy(smc_ctx* c) {
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
-                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
+                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->aux0, c->aux1, c->spec_src, c->spec_swin,
+                  c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (void* q : c->dif_alloc)
@@ -1166,6 +1212,35 @@ smc_status smc_moments(smc_ctx* c, double* ev_out) {
     if (ev_out)
       HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)c->nev * SMC_EV_STRIDE * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_rho_binary(smc_ctx* c, float* rho) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events");
+    float* g = grid_buffer(c, &c->aux0);
+    run_rho_binary(c, g);
+    if (rho)
+      HIPCHK(hipMemcpyAsync(rho, g, (size_t)c->nev * c->d.nx * c->d.ny * sizeof(float), hipMemcpyDeviceToHost,
+                            c->stream));
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_spectator_density(smc_ctx* c, float* spec_proj, float* spec_targ) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events");
+    float* a = grid_buffer(c, &c->aux0);
+    float* b = grid_buffer(c, &c->aux1);
+    run_spectator_density(c, a, b);
+    const size_t n = (size_t)c->nev * c->d.nx * c->d.ny * sizeof(float);
+    if (spec_proj) HIPCHK(hipMemcpyAsync(spec_proj, a, n, hipMemcpyDeviceToHost, c->stream));
+    if (spec_targ) HIPCHK(hipMemcpyAsync(spec_targ, b, n, hipMemcpyDeviceToHost, c->stream));
     sync(c);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }

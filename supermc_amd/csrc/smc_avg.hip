@@ -1,5 +1,6 @@
 // smc_avg.hip -- averaged, rotated profiles (operation 3 of the reference,
-// MakeDensity.cpp:736-1650 pT-integrated branch).
+// MakeDensity.cpp:736-1650 pT-integrated branch), and the spectator source
+// lists of operation 2 (MakeDensity.cpp:148-500).
 //
 // Per accepted event and order n (in the reference's order, rotations
 // accumulating from one order to the next):
@@ -144,6 +145,78 @@ __global__ __launch_bounds__(256) void k_accum_avg(const float* grid, size_t nce
   for (int e = 0; e < nev; ++e)
     if (accept[e]) s += ff * (double)grid[(size_t)e * ncell + c];
   sum[c] += s;
+}
+
+// Spectators (MCnucl::getSpectators, MCnucl.cpp:1222-1245): the nucleons
+// without a collision, projectile ones then target ones in collision order,
+// as sources of kind A / B with the +-5w window of MCnucl.cpp:546-560, into
+// their own source list (sb: the counts / src / swin of that list).
+__global__ __launch_bounds__(256) void k_spectator_src(DevCfg cfg, EvBufs buf, EvBufs sb) {
+  __shared__ int s_bb[4], s_cnt[4];
+  const int e = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+  const int* pos = buf.pos + (size_t)e * 2 * kAMax * 2;
+  const int* wound = buf.wound + (size_t)e * 2 * kAMax;
+  int4* src = sb.src + (size_t)e * sb.src_cap;
+  int4* swin = sb.swin + (size_t)e * sb.src_cap;
+  if (t < 4) s_bb[t] = (t & 1) ? -1 : 0x7FFFFFFF;
+  int bb[4] = {0x7FFFFFFF, -1, 0x7FFFFFFF, -1};
+  int base = 0;
+  for (int nu = 0; nu < 2; ++nu) {
+    const int A = cfg.nuc[nu].A;
+    for (int i0 = 0; i0 < A; i0 += 256) {
+      const int i = i0 + t;
+      const bool spec = i < A && wound[nu * kAMax + i] < 0;
+      const unsigned long long m = __ballot(spec);
+      __syncthreads();
+      if (lane == 0) s_cnt[wv] = __popcll(m);
+      __syncthreads();
+      int k = base;
+      for (int q = 0; q < wv; ++q) k += s_cnt[q];
+      k += __popcll(m & ((1ull << lane) - 1ull));
+      if (spec) {
+        const int xh = 2 * pos[(nu * kAMax + i) * 2], yh = 2 * pos[(nu * kAMax + i) * 2 + 1];
+        src[k] = make_int4(xh, yh, __float_as_int(1.f), nu == 0 ? kSrcPartA : kSrcPartB);
+        const int4 w = src_windows(cfg, xh, yh, false);
+        swin[k] = w;
+        const int x0 = w.z & 0xffff, x1 = w.z >> 16, y0 = w.w & 0xffff, y1 = w.w >> 16;
+        if (x0 < x1 && y0 < y1) {
+          bb[0] = min(bb[0], x0);
+          bb[1] = max(bb[1], x1);
+          bb[2] = min(bb[2], y0);
+          bb[3] = max(bb[3], y1);
+        }
+      }
+      base += s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    bb[0] = min(bb[0], __shfl_xor(bb[0], off, 64));
+    bb[1] = max(bb[1], __shfl_xor(bb[1], off, 64));
+    bb[2] = min(bb[2], __shfl_xor(bb[2], off, 64));
+    bb[3] = max(bb[3], __shfl_xor(bb[3], off, 64));
+  }
+  if (lane == 0) {
+    atomicMin(&s_bb[0], bb[0]);
+    atomicMax(&s_bb[1], bb[1]);
+    atomicMin(&s_bb[2], bb[2]);
+    atomicMax(&s_bb[3], bb[3]);
+  }
+  __syncthreads();
+  if (t == 0) {
+    int* c = sb.counts + (size_t)e * 8;
+    for (int q = 0; q < 5; ++q) c[q] = buf.counts[(size_t)e * 8 + q];
+    c[5] = base;
+    const bool none = s_bb[0] >= s_bb[1];
+    c[6] = none ? 0 : (s_bb[0] | (s_bb[1] << 16));
+    c[7] = none ? 0 : (s_bb[2] | (s_bb[3] << 16));
+  }
+}
+
+int launch_spectator_src(const DevCfg& cfg, const EvBufs& b, const EvBufs& sb, int nev, void* stream) {
+  if (nev <= 0) return 0;
+  hipLaunchKernelGGL(k_spectator_src, dim3(nev), dim3(256), 0, (hipStream_t)stream, cfg, b, sb);
+  return (int)hipGetLastError();
 }
 
 int launch_cm_angle(const DevCfg& cfg, const EvBufs& b, const float* grid, const double* cm, int nu, int n,

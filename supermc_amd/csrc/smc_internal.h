@@ -165,6 +165,8 @@ int launch_cm_partials(const DevCfg& cfg, const EvBufs& b, const float* grid, in
 int launch_cm_angle(const DevCfg& cfg, const EvBufs& b, const float* grid, const double* cm, int nu, int n,
                     int angle_on, int nev, double* rot, void* stream);
 int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int nev, void* stream);
+// spectators of every event as a source list of their own (sb.counts / src / swin)
+int launch_spectator_src(const DevCfg& cfg, const EvBufs& b, const EvBufs& sb, int nev, void* stream);
 int launch_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept, double ff, double* sum,
                      void* stream);
 int tile_units(const DevCfg& cfg);  // smearing tiles per event (units of SmearSpec::cm_out)

@@ -83,6 +83,31 @@ void MCnucl::dumpparticipantTable(std::string filename) {
   smc_host::dump_participant_table(ctx_, p_, 0, filename.c_str());
 }
 
+int MCnucl::getSpectators() {
+  std::vector<double> xa, xb;
+  std::vector<int> wa, wb;
+  getNucleons(xa, wa, xb, wb);
+  int n = 0;
+  for (int w : wa) n += w < 0;
+  for (int w : wb) n += w < 0;
+  return n;
+}
+
+void MCnucl::dumpSpectatorsTable(int event) { smc_host::dump_spectators_table(ctx_, p_, 0, event); }
+
+void MCnucl::calculate_rho_binary() {
+  std::vector<float> r(rho_.size());
+  check(ctx_, smc_rho_binary(ctx_, r.data()), "MCnucl::calculate_rho_binary");
+  rho_binary_.assign(r.begin(), r.end());
+}
+
+void MCnucl::calculate_spectator_density() {
+  std::vector<float> a(rho_.size()), b(rho_.size());
+  check(ctx_, smc_spectator_density(ctx_, a.data(), b.data()), "MCnucl::calculate_spectator_density");
+  spec1_.assign(a.begin(), a.end());
+  spec2_.assign(b.begin(), b.end());
+}
+
 void MCnucl::getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_proj, std::vector<double>& xy_targ,
                          std::vector<int>& wound_targ) {
   xy_proj.assign(2 * (size_t)p_.Aproj, 0.0);

@@ -51,6 +51,16 @@ class MCnucl {
   // the reference; quarks at their nucleon's centre)
   void dumpBinaryTable(std::string filename);
   void dumpparticipantTable(std::string filename);
+  // MCnucl.cpp:1222-1260: spectators of the current event (count, table)
+  int getSpectators();
+  void dumpSpectatorsTable(int event);
+  // MCnucl.cpp:481-614: binary-collision and spectator densities
+  void calculate_rho_binary();
+  double get_rho_binary(int x, int y) { return rho_binary_[(size_t)x * ny_ + y]; }
+  void calculate_spectator_density();
+  double get_spectator_density(int nucleus_id, int x, int y) {
+    return (nucleus_id == 1 ? spec1_ : spec2_)[(size_t)x * ny_ + y];
+  }
 
   // the event's nucleons in collision order (x, y in fm; wound rank or -1)
   void getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_proj, std::vector<double>& xy_targ,
@@ -66,7 +76,7 @@ class MCnucl {
   int npart1_ = 0, npart2_ = 0, ncoll_ = 0, npmin_ = 0, npmax_ = 0;
   double dndy_ = 0;
   bool have_event_ = false;
-  std::vector<double> ta_, tb_, rho_;
+  std::vector<double> ta_, tb_, rho_, rho_binary_, spec1_, spec2_;
 };
 
 #endif

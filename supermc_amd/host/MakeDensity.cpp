@@ -256,3 +256,93 @@ void MakeDensity::generate_profile_average(int nevent) {
       }
     }
 }
+
+void MakeDensity::generate_profile_ebe_Jet(int nevent) {
+  // MakeDensity.cpp:148-500 (pT-integrated, one rapidity bin)
+  const int from = p_.ecc_from_order, to = p_.ecc_to_order;
+  const bool use_sd = rdr_->getVal("use_sd") != 0, use_ed = rdr_->getVal("use_ed") != 0;
+  const bool use_block = rdr_->getVal("use_block") != 0, use_4col = rdr_->getVal("use_4col") != 0;
+  const bool out_rho_binary = rdr_->getVal("output_rho_binary") != 0, out_ta = rdr_->getVal("output_TA") != 0;
+  const bool out_rhob = rdr_->getVal("output_rhob") != 0;
+  const bool out_spec = rdr_->getVal("output_spectator_density") != 0;
+  const size_t ncell = (size_t)nx_ * ny_;
+  const int chunk = std::min(batch_, 512);  // per-event grids come back to the host
+  std::vector<float> grid, ta, tb, rho, spa, spb;
+  std::vector<double> dndy;
+  records_.clear();
+  int event = 1;
+  char buffer[512];
+  auto grids = [&](const char* stem, const float* g, double factor, double npart) {
+    if (use_4col) {
+      std::snprintf(buffer, sizeof(buffer), "data/%s_event_%d_4col.dat", stem, event);
+      dumpDensity4Col(buffer, g, factor, npart);
+    }
+    if (use_block) {
+      std::snprintf(buffer, sizeof(buffer), "data/%s_event_%d_block.dat", stem, event);
+      dumpDensityBlock(buffer, g, factor);
+    }
+  };
+  while (event <= nevent) {
+    const int n = next_batch(std::min(chunk, nevent - event + 1));
+    grid.resize((size_t)n * ncell);
+    dndy.resize(n);
+    const smc_status st = smc_density(ctx_, grid.data(), dndy.data());
+    if (st != SMC_ERR_DNDY_ZERO) check(ctx_, st, "MakeDensity::generate_profile_ebe_Jet");
+    if (out_ta || out_rhob) {
+      ta.resize((size_t)n * ncell);
+      tb.resize((size_t)n * ncell);
+      check(ctx_, smc_thickness(ctx_, ta.data(), tb.data()), "MakeDensity::generate_profile_ebe_Jet");
+    }
+    if (out_rho_binary) {
+      rho.resize((size_t)n * ncell);
+      check(ctx_, smc_rho_binary(ctx_, rho.data()), "MakeDensity::generate_profile_ebe_Jet");
+    }
+    if (out_spec) {
+      spa.resize((size_t)n * ncell);
+      spb.resize((size_t)n * ncell);
+      check(ctx_, smc_spectator_density(ctx_, spa.data(), spb.data()), "MakeDensity::generate_profile_ebe_Jet");
+    }
+    for (int e = 0; e < n && event <= nevent; ++e) {
+      const double* rec = &batch_rec_[(size_t)e * SMC_EV_STRIDE];
+      if (!accept(rec)) {
+        ++dropped_;
+        continue;
+      }
+      const double npart = rec[SMC_EV_NPART];
+      std::snprintf(buffer, sizeof(buffer), "data/ParticipantTable_event_%d.dat", event);
+      smc_host::dump_participant_table(ctx_, p_, e, buffer);
+      smc_host::dump_spectators_table(ctx_, p_, e, event);
+      std::snprintf(buffer, sizeof(buffer), "data/BinaryCollisionTable_event_%d.dat", event);
+      smc_host::dump_binary_table(ctx_, p_, e, buffer);
+      const size_t off = (size_t)e * ncell;
+      if (use_sd) {
+        std::snprintf(buffer, sizeof(buffer), "data/sn_ecc_eccp_%%d_event_%d.dat", event);
+        const std::string base = buffer;
+        dumpEccentricities(base.c_str(), rec, from, to, npart, rec[SMC_EV_NCOLL], rec[SMC_EV_B]);
+        grids("sd", &grid[off], p_.finalFactor, npart);
+      }
+      if (use_ed) {
+        std::snprintf(buffer, sizeof(buffer), "data/en_ecc_eccp_%%d_event_%d.dat", event);
+        const std::string base = buffer;
+        dumpEccentricities(base.c_str(), rec, from, to, npart, rec[SMC_EV_NCOLL], rec[SMC_EV_B]);
+        grids("ed", &grid[off], p_.finalFactor, npart);
+      }
+      if (out_rho_binary) grids("rho_binary", &rho[off], 1.0, npart);
+      if (out_ta) {
+        grids("nuclear_thickness_TA", &ta[off], 1.0, npart);
+        grids("nuclear_thickness_TB", &tb[off], 1.0, npart);
+      }
+      if (out_rhob) {
+        std::vector<float> sum(ncell);
+        for (size_t i = 0; i < ncell; ++i) sum[i] = ta[off + i] + tb[off + i];
+        grids("rhob", sum.data(), 1.0, npart);
+      }
+      if (out_spec) {
+        grids("spectator_density_A", &spa[off], 1.0, npart);
+        grids("spectator_density_B", &spb[off], 1.0, npart);
+      }
+      records_.insert(records_.end(), rec, rec + SMC_EV_STRIDE);
+      ++event;
+    }
+  }
+}

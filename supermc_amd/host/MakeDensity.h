@@ -11,6 +11,16 @@
 //                           and per event the collision tables of
 //                           MCnucl::dumpBinaryTable (data/binary.dat,
 //                           wounded.data, quarks.data, nucl1/2.data)
+//  generate_profile_ebe_Jet(n)  operation 2 (MakeDensity.cpp:148-500),
+//                           pT-integrated: per event the participant,
+//                           spectator and binary-collision tables, the
+//                           eccentricity lines (sn_ecc_eccp_<n>_event_<i>),
+//                           the profiles, and with output_rho_binary /
+//                           output_TA / output_rhob / output_spectator_density
+//                           the binary density, T_A and T_B, T_A + T_B and the
+//                           spectator densities.  Events failing the dS/dy
+//                           cut write nothing (the reference writes their
+//                           tables before the cut).
 //  generate_profile_average(n)  operation 3 (MakeDensity.cpp:736-1650),
 //                           pT-integrated: averages over n accepted events
 //                           of the profiles recentred and turned to each
@@ -44,6 +54,7 @@ class MakeDensity {
   void generateEccTable(int nevent);
   void generate_profile_ebe(int nevent);
   void generate_profile_average(int nevent);
+  void generate_profile_ebe_Jet(int nevent);
 
   // Writes the eccentricity lines of one event record (the SMC_EV_* layout
   // of smc.h) in the format of MakeDensity.cpp:2432-2500.

@@ -3,6 +3,7 @@
 #include <sys/time.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdlib>
 #include <fstream>
 #include <iomanip>
@@ -167,6 +168,29 @@ void dump_binary_table(smc_ctx* ctx, const smc_params& p, int ev, const char* bi
       of << std::setprecision(3) << std::setw(10) << n.xy[nu][2 * i] << std::setprecision(3) << std::setw(10)
          << n.xy[nu][2 * i + 1] << std::endl;
   }
+}
+
+double spectator_rapidity(double ecm) {
+  const double v_z = std::sqrt(1. - 1. / ((ecm / 2.) * (ecm / 2.)));
+  return 0.5 * std::log((1. + v_z) / (1. - v_z + 1e-100));
+}
+
+int dump_spectators_table(smc_ctx* ctx, const smc_params& p, int ev, int event) {
+  const EventNucleons n = fetch_event(ctx, p, ev, false);
+  const double Y = spectator_rapidity(p.ecm);
+  std::ostringstream name;
+  name << "data/Spectators_event_" << event << ".dat";
+  std::ofstream of(name.str());
+  const int A[2] = {p.Aproj, p.Atarg};
+  int count = 0;
+  for (int nu = 0; nu < 2; ++nu)
+    for (int i = 0; i < A[nu]; ++i) {
+      if (n.wound[nu][i] >= 0) continue;
+      of << std::scientific << std::setprecision(4) << std::setw(10) << n.xy[nu][2 * i] << "  " << n.xy[nu][2 * i + 1]
+         << "  " << (nu == 0 ? Y : -Y) << std::endl;
+      ++count;
+    }
+  return count;
 }
 
 void dump_participant_table(smc_ctx* ctx, const smc_params& p, int ev, const char* filename) {

@@ -35,6 +35,13 @@ void dump_binary_table(smc_ctx* ctx, const smc_params& p, int ev, const char* bi
 // MCnucl::dumpparticipantTable (MCnucl.cpp:1213-1220): participants of both
 // nuclei, appended to filename.
 void dump_participant_table(smc_ctx* ctx, const smc_params& p, int ev, const char* filename);
+// MCnucl::getSpectators + dumpSpectatorsTable (MCnucl.cpp:1222-1260): the
+// nucleons without a collision (projectile then target, collision order)
+// with rapidity +-Y of the beams, to data/Spectators_event_<event>.dat.
+// Returns the number of spectators.
+int dump_spectators_table(smc_ctx* ctx, const smc_params& p, int ev, int event);
+// beam rapidity of the spectators (MCnucl.cpp:1225-1228)
+double spectator_rapidity(double ecm);
 
 }  // namespace smc_host
 

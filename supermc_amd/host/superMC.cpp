@@ -1,7 +1,8 @@
 // superMC.e -- the reference's driver (main.cpp) on the MI355X path:
 // parameters.dat, then "key=value" arguments, then operation 9
-// (eccentricity table), 1 (event-by-event profiles) or 3 (averaged
-// rotated profiles).
+// (eccentricity table), 1 (event-by-event profiles), 2 (event-by-event
+// profiles with the jet-calculation tables and grids) or 3 (averaged rotated
+// profiles).
 #include <cstdlib>
 #include <ctime>
 #include <iostream>
@@ -24,6 +25,9 @@ int main(int argc, char* argv[]) {
       case 1:
         dens.generate_profile_ebe(nevent);
         break;
+      case 2:
+        dens.generate_profile_ebe_Jet(nevent);
+        break;
       case 3:
         dens.generate_profile_average(nevent);
         break;
@@ -31,7 +35,7 @@ int main(int argc, char* argv[]) {
         dens.generateEccTable(nevent);
         break;
       default:
-        std::cout << "Error: operation choice " << operation << " not recognized (this build runs 1, 3 and 9)."
+        std::cout << "Error: operation choice " << operation << " not recognized (this build runs 1, 2, 3 and 9)."
                   << std::endl;
     }
   }

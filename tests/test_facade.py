@@ -266,6 +266,52 @@ def test_driver_average_profiles_match_c_abi(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_jet_operation(tmp_path):
+    """operation 2 (MakeDensity.cpp:148-500): per-event tables, eccentricity
+    lines in sn_ecc_eccp_<n>_event_<i>.dat, and the rho_binary, T_A / T_B,
+    T_A + T_B and spectator grids against the C-ABI on the same events."""
+    _need(EXE)
+    _write_params(tmp_path, operation=2, nev=3, output_rho_binary=1, output_TA=1, output_rhob=1,
+                  output_spectator_density=1, use_block=1, use_4col=0)
+    r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=3)
+    with Context(p) as ctx:
+        ev = ctx.run(seed=1234, event0=0, nev=3)
+        rho = ctx.rho_binary()
+        ta, tb = ctx.thickness()
+        spa, spb = ctx.spectator_density()
+        nuc = [ctx.nucleons(e) for e in range(3)]
+        prs = [ctx.pairs(e) for e in range(3)]
+    d = tmp_path / "data"
+    Y = np.arctanh(np.sqrt(1 - 1 / 1380.0 ** 2))
+    for e in range(3):
+        i = e + 1
+        for n in range(2, 6):
+            t = _read_table(d / f"sn_ecc_eccp_{n}_event_{i}.dat")
+            assert t.shape == (1, 9)
+            np.testing.assert_allclose(t[0, 0], ev[e, n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_allclose(_read_table(d / f"rho_binary_event_{i}_block.dat"), rho[e], rtol=1e-7, atol=1e-300)
+        np.testing.assert_allclose(_read_table(d / f"nuclear_thickness_TA_event_{i}_block.dat"), ta[e], rtol=1e-7,
+                                   atol=1e-300)
+        np.testing.assert_allclose(_read_table(d / f"rhob_event_{i}_block.dat"),
+                                   ta[e].astype(np.float32) + tb[e].astype(np.float32), rtol=1e-7, atol=1e-300)
+        np.testing.assert_allclose(_read_table(d / f"spectator_density_B_event_{i}_block.dat"), spb[e], rtol=1e-7,
+                                   atol=1e-300)
+        part = _read_table(d / f"ParticipantTable_event_{i}.dat")
+        assert len(part) == ev[e, 54]
+        binary = _read_table(d / f"BinaryCollisionTable_event_{i}.dat")
+        assert len(binary) == ev[e, 55] == len(prs[e][0])
+        spec = _read_table(d / f"Spectators_event_{i}.dat")
+        n_spec = (nuc[e]["wound_proj"] < 0).sum() + (nuc[e]["wound_targ"] < 0).sum()
+        assert spec.shape == (n_spec, 3)
+        assert np.allclose(np.abs(spec[:, 2]), Y, rtol=1e-4)
+        assert (spec[: (nuc[e]["wound_proj"] < 0).sum(), 2] > 0).all()
+
+
+@pytest.mark.gpu
 def test_mcnucl_facade_matches_batched_path(tmp_path):
     _need(CHECK)
     pfile = _write_params(tmp_path)
