@@ -254,12 +254,16 @@ def main():
         achieved = work / (kernels[dom]["avg_ms"] * 1e-3)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK)
     roof["kernel"] = dom
+    # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
+    # WRITE_SIZE summary (calibrated against a plain copy), scaled to this
+    # batch size; null for a smearing path it was not measured on
     traffic = None
     tpath = os.path.join(ROOT, "profiles", f"{dom}_pbpb261_traffic.json")
-    if os.path.exists(tpath):
+    if os.path.exists(tpath) and args.smear_path == 0:
         try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
-        except Exception:
+            t = json.load(open(tpath))
+            traffic = t["bytes_per_launch"] * nev / t["events_per_launch"]
+        except (OSError, ValueError, KeyError):
             traffic = None
     roof["traffic"] = traffic
 

@@ -40,15 +40,21 @@ struct TileEntry {
 };
 
 template <int TR, int CH, bool PAIR>
-__global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, float* out) {
+__global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, int nev, float* out) {
   static_assert(TR % CH == 0 && CH % 4 == 0 && TR <= 64 && (!PAIR || TR <= 32), "tile rows");
   __shared__ TileEntry wl[64];
   __shared__ float4 rd[2][TR / 4];  // K gx(row) of the current source(s), 0 outside the window
   const int lane = threadIdx.x;
   const int nx = cfg.nx, ny = cfg.ny;
   const int ntx = (nx + TR - 1) / TR;
-  const int r0 = TR * (blockIdx.x % ntx), c0 = 64 * (blockIdx.x / ntx);
-  const int e = blockIdx.y;
+  // Workgroups go to the 8 XCDs round-robin by linear id; this mapping
+  // gives all tiles of an event one XCD (one L2 holds its source list, and
+  // the partial cache lines at tile edges meet in that L2).
+  const int T = ntx * ((ny + 63) >> 6);
+  const unsigned L = blockIdx.x, rg = L % (8u * T);
+  const int e = (int)(8u * (L / (8u * T)) + rg % 8u), tile = (int)(rg / 8u);
+  if (e >= nev) return;
+  const int r0 = TR * (tile % ntx), c0 = 64 * (tile / ntx);
   // exp2 argument -(d_h)^2 kH^2 inv2w2 log2 e with d_h in half-quanta (kH^2 is a power of two)
   const float cq = cfg.inv2w2 * 1.4426950408889634f * (kHf * kHf);
   const float kk = cfg.tile_k;
@@ -160,7 +166,7 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
     s1 = wave_sum(s1);
     s2 = wave_sum(s2);
     if (lane == 0) {
-      double* p = spec.cm_out + ((size_t)e * gridDim.x + blockIdx.x) * 4;
+      double* p = spec.cm_out + ((size_t)e * T + tile) * 4;
       p[0] = s0;
       p[1] = s1;
       p[2] = s2;
@@ -183,11 +189,12 @@ int tile_units(const DevCfg& cfg) { return ((cfg.nx + kTileRows - 1) / kTileRows
 template <int TR, int CH, bool PAIR>
 static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                       hipStream_t st) {
-  const dim3 grid(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64), nev);
+  // one dimension: tiles x events padded to a multiple of the 8 XCDs
+  const dim3 grid((unsigned)(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64)) * (unsigned)((nev + 7) / 8 * 8));
   SmearSpec s1 = s;  // partials only for a single accumulator
   if (s.combine != 0) s1.cm_out = nullptr;
-  hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 0, out0);
-  if (s.combine != 0) hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 1, out1);
+  hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 0, nev, out0);
+  if (s.combine != 0) hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 1, nev, out1);
 }
 
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
