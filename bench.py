@@ -179,7 +179,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--smear-path", type=int, default=0,
-                    help="0 exact footprint (separable VALU tiles), 1 MFMA Gx.diag(w).Gy^T, 2 exact (per-cell exp)")
+                    help="0 exact footprint (separable VALU tiles), 1 MFMA Gx.diag(w).Gy^T, 2 exact (per-cell exp), "
+                         "3 BF16x3 MFMA Gx.diag(w).Gy^T")
     ap.add_argument("--parity-events", type=int, default=12,
                     help="events of the last batch re-evaluated by the FP64 oracle for the eps_n error")
     ap.add_argument("--kln-events", type=int, default=20000, help="MC-KLN events per step (0: skip)")
@@ -292,7 +293,8 @@ def main():
         "config": {"workload": "batched MC-Glauber Pb+Pb min-bias, 261x261 grid, Gamma fluct k=0.8",
                    "events_per_step": nev, "grid": [ctx.nx, ctx.ny], "sigma_nn_mb": 64.0, "moments_dtype": "f64",
                    "mean_sources_per_event": sum_k / nev, "failed_events": bad,
-                   "smear_path": ["exact separable VALU tiles", "MFMA Gx.diag(w).Gy^T", "exact per-cell exp tiles"][args.smear_path],
+                   "smear_path": ["exact separable VALU tiles", "MFMA Gx.diag(w).Gy^T", "exact per-cell exp tiles",
+                                  "BF16x3 MFMA Gx.diag(w).Gy^T"][args.smear_path],
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
                    "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj},

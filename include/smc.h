@@ -86,7 +86,9 @@ typedef struct smc_params {
   int smear_path;              /* 0 exact reference footprint, separable VALU tiles [default]
                                   1 MFMA fp32 strips: separable Gx.diag(w).Gy^T over the
                                     square window (no r <= 5w corner cut)
-                                  2 exact reference footprint, per-cell exponentials */
+                                  2 exact reference footprint, per-cell exponentials
+                                  3 as 1 with BF16x3 split-precision MFMA (bf16 hi + lo
+                                    factors, three products, FP32 accumulation) */
 } smc_params;
 
 /* Derived quantities (for the facade and tests) */

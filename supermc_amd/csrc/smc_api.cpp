@@ -235,6 +235,7 @@ void validate_and_derive(smc_ctx* c) {
   if (p.ecc_from_order < 1 || p.ecc_to_order > 9 || p.ecc_from_order > p.ecc_to_order)
     throw SmcError(SMC_ERR_ARG, "ecc orders must satisfy 1 <= from <= to <= 9");
   if (p.bmax < p.bmin || p.bmin < 0) throw SmcError(SMC_ERR_ARG, "bad impact parameter range");
+  if (p.smear_path < 0 || p.smear_path > 3) throw SmcError(SMC_ERR_ARG, "smear_path must be 0, 1, 2 or 3");
 
   // sigma_inel and width (MCnucl.cpp:58-64, 93; GaussianNucleonsCal.cpp:34-54)
   d.siginNN = (p.sigma_nn_mb > 0) ? p.sigma_nn_mb : regge_sigma_inel(p.ecm);
@@ -551,12 +552,18 @@ SmearSpec thickness_spec(const smc_ctx* c) {
 // by flushing denormal products); 1 MFMA strips Gx.diag(w).Gy^T (square
 // window, no r <= 5w corner cut; grids up to 272 wide, not for
 // sqrt(T_A T_B)); 2 exact footprint, per-cell exponentials (256-thread
-// tiles).
+// tiles); 3 as 1 with BF16x3 split-precision MFMA.
+static bool strip_path(const smc_ctx* c, const SmearSpec& spec) {
+  const int sp = c->p.smear_path;
+  return (sp == 1 || sp == 3) && c->d.ny <= 272 && spec.combine != 1;
+}
+
 void run_smear(smc_ctx* c, const SmearSpec& spec, float* out0, float* out1) {
   TimerScope ts(c, SMC_K_SMEAR);
   const int sp = c->p.smear_path;
-  if (sp == 1 && c->d.ny <= 272 && spec.combine != 1)
-    check_launch(launch_strip(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_strip");
+  if (strip_path(c, spec))
+    check_launch(launch_strip(c->cfg, c->buf, spec, c->nev, out0, out1, sp == 3, c->stream),
+                 sp == 3 ? "k_strip3" : "k_strip");
   else if (sp != 2)
     check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
   else
@@ -630,8 +637,7 @@ void run_density(smc_ctx* c) {
   }
   SmearSpec spec = density_spec(c);
   // the tile kernel adds the centre-of-mass partials of a single-accumulator grid
-  const int sp = c->p.smear_path;
-  const bool tile = sp != 2 && !(sp == 1 && c->d.ny <= 272 && spec.combine != 1);  // run_smear's choice
+  const bool tile = c->p.smear_path != 2 && !strip_path(c, spec);  // run_smear's choice
   if (tile && spec.combine == 0) spec.cm_out = c->cm;
   run_smear(c, spec, grid_buffer(c, &c->grid), spec.combine ? grid_buffer(c, &c->tmp) : nullptr);
   c->cm_nu = spec.cm_out ? tile_units(c->cfg) : 0;

@@ -170,8 +170,9 @@ int launch_spectator_src(const DevCfg& cfg, const EvBufs& b, const EvBufs& sb, i
 int launch_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept, double ff, double* sum,
                      void* stream);
 int tile_units(const DevCfg& cfg);  // smearing tiles per event (units of SmearSpec::cm_out)
+// bf16x3: the BF16x3 split-precision kernel instead of the FP32 one (not for combine 1)
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
-                 void* stream);
+                 bool bf16x3, void* stream);
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                 void* stream);
 int launch_combine_sqrt(float* a, const float* b, size_t n, void* stream);

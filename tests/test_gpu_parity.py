@@ -7,7 +7,8 @@ Bars (written here, checked below):
   exact-footprint paths (smear_path 0 and 2, the parity paths):
     eps_n (r^2 weight, n = 2..5) and eps'_n (r^n weight): absolute <= 1e-6
     <r^n>: relative <= 1e-5; dS/dy, grid sum: relative <= 1e-6
-  MFMA path (smear_path 1, opt-in, not a parity path): it smears over the
+  MFMA paths (smear_path 1 FP32 and 3 BF16x3, opt-in, not parity paths):
+  they smear over the
   square +-5w window, whose corners (r > 5w, each below e^-12.5 of the
   source's peak) the reference leaves out, so its moment bars are wider:
     eps_n: absolute <= 1e-5; eps'_n: absolute <= 5e-5 (the r^n weight
@@ -16,6 +17,9 @@ Bars (written here, checked below):
     peripheral event weights the corners most);
     dS/dy, grid sum: relative <= 1e-5; centre of mass: absolute <= 1e-5 fm
   centre of mass: absolute <= 1e-6 fm on the exact paths
+  BF16x3 (smear_path 3): a factor split as bf16 hi + lo keeps 16 bits
+  (|f - hi - lo| <= 2^-16 |f|), so its grids get relative L-inf <= 3e-5,
+  against the oracle and against the FP32 MFMA strips on the same windows
 """
 import numpy as np
 import pytest
@@ -25,7 +29,12 @@ import parity_helpers as H
 pytestmark = pytest.mark.gpu
 
 GRID_TOL = 1e-5
+GRID_TOL_BF16X3 = 3e-5
 EPS_TOL = 1e-6
+
+
+def _grid_tol(p):
+    return GRID_TOL_BF16X3 if p.smear_path == 3 else GRID_TOL
 
 
 def _ctx(p):
@@ -76,8 +85,9 @@ def _check_event_record(rec, o, p):
 # smear_path 0: exact reference footprint, separable VALU tiles (default)
 #            1: MFMA strips Gx.diag(w).Gy^T over the square window
 #            2: exact reference footprint, per-cell exponentials
-PATHS = [0, 1, 2]
-MFMA_PATHS = (1,)
+#            3: as 1 with BF16x3 split-precision MFMA
+PATHS = [0, 1, 2, 3]
+MFMA_PATHS = (1, 3)
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -101,9 +111,9 @@ def test_fed_positions_full_chain(system, path):
             if nc[e] == 0:
                 continue
             _check_collision(ctx, e, o)
-            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL, H.rel_linf(s[e], o["s"])
-            assert H.rel_linf(ta[e], o["TA"]) <= GRID_TOL
-            assert H.rel_linf(tb[e], o["TB"]) <= GRID_TOL
+            assert H.rel_linf(s[e], o["s"]) <= _grid_tol(p), H.rel_linf(s[e], o["s"])
+            assert H.rel_linf(ta[e], o["TA"]) <= _grid_tol(p), H.rel_linf(ta[e], o["TA"])
+            assert H.rel_linf(tb[e], o["TB"]) <= _grid_tol(p), H.rel_linf(tb[e], o["TB"])
             assert dndy[e] == pytest.approx(o["dndy"], rel=_sum_tol(p))
             _check_event_record(rec[e], o, p)
 
@@ -125,7 +135,7 @@ def test_sampled_events_match_oracle(path):
         s, _, _ = ctx.density()
         for e in range(0, 48, 7):
             o = H.oracle_event(ctx, e)
-            assert H.rel_linf(s[e], o["s"]) <= GRID_TOL
+            assert H.rel_linf(s[e], o["s"]) <= _grid_tol(p)
 
 
 @pytest.mark.parametrize("path", PATHS)
@@ -211,3 +221,23 @@ def test_sampled_nuclei_follow_reference_sampler():
         npc.append(wa.sum() + wb.sum())
     assert stats.ks_2samp(rg, rc).pvalue > 1e-3
     assert abs(np.mean(npa + npb) - np.mean(npc)) < 4 * np.std(npc) / np.sqrt(200) * 1.5
+
+
+@pytest.mark.parametrize("system", ["PbPb", "AuAu"])
+def test_bf16x3_strips_match_fp32_strips(system):
+    """The split-precision contraction against the FP32 one on the same
+    sources and square windows: entropy grid (one accumulator) and T_A / T_B
+    (two accumulators)."""
+    grids = {}
+    for path in (1, 3):
+        p = H.glauber_params(system, max_events=16, smear_path=path)
+        with _ctx(p) as ctx:
+            ctx.run(seed=31, event0=0, nev=16)
+            s, _, _ = ctx.density()
+            ta, tb = ctx.thickness()
+            grids[path] = (s, ta, tb)
+    for g1, g3 in zip(grids[1], grids[3]):
+        for e in range(16):
+            assert H.rel_linf(g3[e], g1[e]) <= GRID_TOL_BF16X3, (e, H.rel_linf(g3[e], g1[e]))
+        # and not the FP32 result by accident: split products differ in the last bits
+        assert not np.array_equal(g1, g3)
