@@ -605,8 +605,24 @@ void ensure_kln_table(smc_ctx* c) {
 
 // MCnucl::setDensity for which_mc_model 1 (MCnucl.cpp:654-687): T_A, T_B
 // (MCnucl.cpp:398-478), then per-cell dN/dy from the table (reference) or
-// from the quadrature in every cell (kln_direct).
+// from the quadrature in every cell (kln_direct).  With the table on the
+// default smearing path both steps run in one tile launch, which also
+// writes the centre-of-mass partials of the dN/dy grid.
 void run_kln_density(smc_ctx* c) {
+  c->cm_nu = 0;
+  if (!c->p.kln_direct && c->p.smear_path == 0) {
+    ensure_kln_table(c);
+    float* s = grid_buffer(c, &c->grid);
+    SmearSpec spec = thickness_spec(c);
+    spec.cm_out = c->cm;
+    TimerScope ts(c, SMC_K_SMEAR);
+    check_launch(launch_tile_kln(c->cfg, c->buf, spec, c->nev, s, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
+                                 c->stream),
+                 "k_tile (KLN)");
+    ts.done();
+    c->cm_nu = tile_units(c->cfg);
+    return;
+  }
   float* a = grid_buffer(c, &c->ta);
   float* b = grid_buffer(c, &c->tb);
   run_smear(c, thickness_spec(c), a, b);
@@ -631,7 +647,6 @@ void run_density(smc_ctx* c) {
   if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
   if (c->p.which_mc_model == 1) {
     run_kln_density(c);
-    c->cm_nu = 0;
     c->have_density = 1;
     return;
   }

@@ -175,6 +175,11 @@ int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  bool bf16x3, void* stream);
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                 void* stream);
+// MC-KLN density in one pass: T_A (acc 0) and T_B (acc 1) of s, then dN/dy
+// from the tmax x tmax table at (T_A/dT, T_B/dT); s.cm_out gets the output's
+// centre-of-mass partials (tile_units per event) when set
+int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
+                    const double* table, int tmax, double dT, void* stream);
 int launch_combine_sqrt(float* a, const float* b, size_t n, void* stream);
 void set_tile_scales(DevCfg& g);
 int launch_diff_topo(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,

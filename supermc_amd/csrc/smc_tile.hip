@@ -39,8 +39,46 @@ struct TileEntry {
   int pad;
 };
 
-template <int TR, int CH, bool PAIR>
-__global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, int nev, float* out) {
+// The MC-KLN density of MCnucl::setDensity (which_mc_model 1,
+// MCnucl.cpp:654-687) fused into the smearing: T_A and T_B accumulate in two
+// register sets of one wave, and each cell's dN/dy is the sixPoint2dInterp
+// (arsenal.cpp:33-54) of the table at (T_A/dT, T_B/dT) in FP64, from the same
+// float T values the separate k_tile + k_kln_table pair reads back.
+struct KlnTab {
+  const double* table;  // tmax x tmax, row = T_A index
+  int tmax;
+  double dT;
+};
+
+__device__ __forceinline__ float kln_cell(float A, float B, const KlnTab& kt, bool& out_of_table) {
+  const double di = (double)A / kt.dT, dj = (double)B / kt.dT;
+  if (!(di >= 0 && di < kt.tmax - 2) || !(dj >= 0 && dj < kt.tmax - 2)) {  // also catches NaN
+    out_of_table = true;
+    return 0.f;
+  }
+  // row and column 0 of the table are 0, so T_A = 0 or T_B = 0 gives 0
+  if (A == 0.f || B == 0.f) return 0.f;
+  const int a = (int)floor(di), b = (int)floor(dj);
+  const double x = di - a, y = dj - b;
+  const double* t = kt.table;
+  const int m = kt.tmax;
+  const double v00 = t[a * m + b], v01 = t[a * m + b + 1], v02 = t[a * m + b + 2];
+  const double v10 = t[(a + 1) * m + b], v11 = t[(a + 1) * m + b + 1];
+  const double v20 = t[(a + 2) * m + b];
+  const double axx = 1.0 / 2.0 * (v00 - 2 * v10 + v20);
+  const double axy = v00 - v01 - v10 + v11;
+  const double ayy = 1.0 / 2.0 * (v00 - 2 * v01 + v02);
+  const double bx = 1.0 / 2.0 * (-3.0 * v00 + 4 * v10 - v20);
+  const double by = 1.0 / 2.0 * (-3.0 * v00 + 4 * v01 - v02);
+  return (float)(axx * x * x + axy * x * y + ayy * y * y + bx * x + by * y + v00);
+}
+
+// KLN: the sources of accumulator 0 (T_A), then those of accumulator 1 (T_B),
+// each set in source order, and the output is dN/dy.  Otherwise the sources
+// of accumulator acc_sel only.
+template <int TR, int CH, bool PAIR, bool KLN>
+__global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, int nev, float* out,
+                                             KlnTab kt) {
   static_assert(TR % CH == 0 && CH % 4 == 0 && TR <= 64 && (!PAIR || TR <= 32), "tile rows");
   __shared__ TileEntry wl[64];
   __shared__ float4 rd[2][TR / 4];  // K gx(row) of the current source(s), 0 outside the window
@@ -69,7 +107,7 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
   const int4* swin = buf.swin + (size_t)e * buf.src_cap;
   float* rdf = reinterpret_cast<float*>(&rd[0][0]);
 
-  float acc[TR];
+  float acc[TR], accA[KLN ? TR : 1];  // accA: T_A while the second pass fills acc with T_B
 #pragma unroll
   for (int r = 0; r < TR; ++r) acc[r] = 0.f;
   // tiles outside the event's source rectangle stay zero
@@ -99,14 +137,23 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
     return (col >= ylo && col < yhi) ? __builtin_amdgcn_exp2f(-(dy * dy) * cq) : 0.f;
   };
 
-  for (int sub = 0; live && sub < K; sub += 64) {
+  for (int pass = 0; pass < (KLN ? 2 : 1); ++pass) {
+   if (KLN && pass == 1) {
+#pragma unroll
+    for (int r = 0; r < TR; ++r) {
+      accA[KLN ? r : 0] = acc[r];
+      acc[r] = 0.f;
+    }
+   }
+   const int want = KLN ? pass : acc_sel;
+   for (int sub = 0; live && sub < K; sub += 64) {
     const int k = sub + lane;
     bool sel = false;
     TileEntry en{0, 0, 0.f, 0, 0, 0};
     if (k < K) {
       const int4 s = src[k];
       const int kind = s.w;
-      if (spec.enabled[kind] && spec.acc[kind] == acc_sel) {
+      if (spec.enabled[kind] && spec.acc[kind] == want) {
         const int4 wv = swin[k];
         en.xh = s.x;
         en.yh = s.y;
@@ -149,14 +196,23 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
       __builtin_amdgcn_wave_barrier();
     }
     __builtin_amdgcn_wave_barrier();
+   }
   }
   const float so = cfg.tile_out;
+  if constexpr (KLN) {  // dN/dy from (T_A, T_B) = (accA, acc) as stored floats
+    bool off = false;
+#pragma unroll
+    for (int r = 0; r < TR; ++r)
+      acc[r] = (live && col < ny && r0 + r < nx) ? kln_cell(accA[r] * so, acc[r] * so, kt, off) : 0.f;
+    if (off) atomicOr(&buf.counts[(size_t)e * 8 + 3], kStKlnTable);
+  }
+  const float os = KLN ? 1.f : so;  // stored value = acc * os
   if (spec.cm_out) {  // centre-of-mass partials of the stored grid values
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
       if (col < ny && r0 + r < nx) {
-        const double v = (double)(acc[r] * so);
+        const double v = (double)(acc[r] * os);
         s0 += v;
         s1 = fma(cfg.xmin + (r0 + r) * cfg.dx, v, s1);
       }
@@ -178,7 +234,7 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
 #pragma unroll
   for (int r = 0; r < TR; ++r) {
     const int ix = r0 + r;
-    if (ix < nx) out[(size_t)e * N + (size_t)ix * ny + col] = acc[r] * so;
+    if (ix < nx) out[(size_t)e * N + (size_t)ix * ny + col] = acc[r] * os;
   }
 }
 
@@ -193,8 +249,22 @@ static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, in
   const dim3 grid((unsigned)(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64)) * (unsigned)((nev + 7) / 8 * 8));
   SmearSpec s1 = s;  // partials only for a single accumulator
   if (s.combine != 0) s1.cm_out = nullptr;
-  hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 0, nev, out0);
-  if (s.combine != 0) hipLaunchKernelGGL((k_tile<TR, CH, PAIR>), grid, dim3(64), 0, st, cfg, b, s1, 1, nev, out1);
+  const KlnTab none{nullptr, 0, 1.0};
+  hipLaunchKernelGGL((k_tile<TR, CH, PAIR, false>), grid, dim3(64), 0, st, cfg, b, s1, 0, nev, out0, none);
+  if (s.combine != 0)
+    hipLaunchKernelGGL((k_tile<TR, CH, PAIR, false>), grid, dim3(64), 0, st, cfg, b, s1, 1, nev, out1, none);
+}
+
+int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
+                    const double* table, int tmax, double dT, void* stream) {
+  if (nev <= 0) return 0;
+  if (!(cfg.tile_k > 0.f && cfg.tile_w > 0.f && cfg.tile_out > 0.f) || !table || tmax < 3 || !(dT > 0))
+    return (int)hipErrorInvalidValue;
+  const dim3 grid((unsigned)tile_units(cfg) * (unsigned)((nev + 7) / 8 * 8));
+  const KlnTab kt{table, tmax, dT};
+  hipLaunchKernelGGL((k_tile<kTileRows, 8, true, true>), grid, dim3(64), 0, (hipStream_t)stream, cfg, b, s, 0, nev,
+                     out, kt);
+  return (int)hipGetLastError();
 }
 
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,

@@ -99,6 +99,30 @@ def test_kln_table_density_full_chain():
             assert rec[e, 57] == 0
 
 
+def test_kln_fused_density_is_the_table_of_its_own_thickness():
+    """The one-launch smearing + table path against the oracle's
+    MCnucl::setDensity table step fed the float T_A, T_B grids that the
+    thickness call returns for the same events: the same FP64 arithmetic on
+    the same inputs, so only rounding of the FP64 expression can differ
+    (relative L-inf <= 1e-6)."""
+    p = _kln_params()
+    xa, xb, bs = H.cpu_configs(p, len(BVALS), seed=17, bvals=BVALS)
+    with _ctx(p) as ctx:
+        ctx.set_positions(xa, xb, bs)
+        ctx.collide(allow=(4,))
+        s, dndy, _ = ctx.density()
+        ta, tb = ctx.thickness()
+        table = ctx.kln_make_table()
+        g = oracle.grid(p.maxx, p.dx, p.maxy, p.dy)
+        for e in range(len(BVALS)):
+            ref, ref_sum = oracle.density_kln_table(g, ta[e].astype(np.float64), tb[e].astype(np.float64), table,
+                                                    ctx.derived.kln_dT)
+            if ref_sum <= 0:
+                continue
+            assert H.rel_linf(s[e], ref) <= 1e-6, (e, H.rel_linf(s[e], ref))
+            assert dndy[e] == pytest.approx(ref_sum, rel=1e-6)
+
+
 def test_kln_direct_density_matches_fp64_quadrature():
     p = _kln_params(kln_direct=1, max_events=3)
     k = _orc_kln(p)
