@@ -80,6 +80,10 @@ def lib():
             "orc_six_point": (C.c_double, [C.c_double] * 8),
             "orc_hotspot_mask": (C.c_int, [C.POINTER(OrcGrid), C.c_int, _dp, C.c_int, C.POINTER(C.c_ubyte)]),
             "orc_density_kln_table": (C.c_double, [C.POINTER(OrcGrid), _dp, _dp, _dp, C.c_int, C.c_double, _dp]),
+            "orc_p3d_setup": (None, [C.c_double, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp]),
+            "orc_p3d_search": (C.c_long, [_dp, C.c_int, C.c_double]),
+            "orc_p3d_smear": (None, [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, _dp,
+                                     _dp]),
             "orc_rng_seed": (None, [C.POINTER(OrcRng), C.c_uint64]),
             "orc_rng_uniform": (C.c_double, [C.POINTER(OrcRng)]),
             "orc_sample_ws": (None, [C.POINTER(OrcRng), C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
@@ -324,6 +328,30 @@ def density_kln_table(g, ta, tb, table, dT):
     s = np.zeros_like(ta)
     dndy = lib().orc_density_kln_table(C.byref(g), _p(ta), _p(tb), _p(table), table.shape[0], dT, _p(s))
     return s, dndy
+
+
+def p3d_setup(ecm, n=1000):
+    """profile_3d.cpp:24-27, 133-155: dict(y_beam, sigma_inel (fm^2), eta,
+    proj, targ, proj_max, targ_max)."""
+    yb, sig = C.c_double(), C.c_double()
+    eta, proj, targ, mx = np.zeros(n), np.zeros(n), np.zeros(n), np.zeros(2)
+    lib().orc_p3d_setup(ecm, n, C.byref(yb), C.byref(sig), _p(eta), _p(proj), _p(targ), _p(mx))
+    return dict(y_beam=yb.value, sigma_inel=sig.value, eta=eta, proj=proj, targ=targ, proj_max=mx[0],
+                targ_max=mx[1])
+
+
+def p3d_search(A, v):
+    A = np.ascontiguousarray(A, np.float64)
+    return lib().orc_p3d_search(_p(A), len(A), float(v))
+
+
+def p3d_smear(nx, ny, neta, dx, dy, deta, src):
+    """profile_3d.cpp:302-400 in FP64: rho [neta][nx][ny] from source rows
+    (x, y, eta, sigma_x, sigma_y, sigma_eta)."""
+    src = np.ascontiguousarray(np.asarray(src, np.float64).reshape(-1, 6))
+    rho = np.zeros((neta, nx, ny))
+    lib().orc_p3d_smear(nx, ny, neta, dx, dy, deta, len(src), _p(src), _p(rho))
+    return rho
 
 
 class Rng:

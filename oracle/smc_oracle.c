@@ -852,3 +852,78 @@ int orc_run_events(const orc_event_cfg* c, uint64_t seed, int nev, double* eps2_
   free(cx); free(cy); free(cf); free(boxes);
   return done;
 }
+
+/* ------------------------------------------------------------------ 3D profiles */
+void orc_p3d_setup(double ecm, int n, double* y_beam, double* sigma_inel_fm2, double* eta, double* proj,
+                   double* targ, double* maxes)
+{
+  /* profile_3d.cpp:24-27 */
+  const double yb = atanh(sqrt(1. - 1. / pow(ecm / 2., 2)));
+  *y_beam = yb;
+  *sigma_inel_fm2 = orc_sigma_inel(ecm) * 0.1;
+  /* profile_3d.cpp:133-155 */
+  const double eta_peak = 2.5, sigma_eta_out = 0.5;
+  const double deta = 2 * yb / (n - 1);
+  maxes[0] = maxes[1] = 0.0;
+  for (int i = 0; i < n; i++) {
+    const double e = -yb + i * deta;
+    double f = 1.0;
+    if (fabs(e) > eta_peak) f = exp(-pow((fabs(e) - eta_peak), 2) / (2. * sigma_eta_out * sigma_eta_out));
+    const double l = (1. - e / yb) * f, r = (1. + e / yb) * f;
+    if (l > maxes[0]) maxes[0] = l;
+    if (r > maxes[1]) maxes[1] = r;
+    eta[i] = e;
+    proj[i] = l;
+    targ[i] = r;
+  }
+}
+
+long orc_p3d_search(const double* A, int n, double v)
+{
+  /* profile_3d.cpp:402-434 with skip_out_of_range */
+  int i = 0, f = n - 1, m;
+  if (v > A[f]) return -1;
+  if (v < A[i]) return -1;
+  m = (int)floor((f + i) / 2.);
+  while ((f - i) > 1) {
+    if (A[m] < v) i = m;
+    else f = m;
+    m = (int)floor((f + i) / 2.);
+  }
+  return i;
+}
+
+void orc_p3d_smear(int nx, int ny, int neta, double dx, double dy, double deta, int n, const double* src,
+                   double* rho)
+{
+  /* profile_3d.cpp:75-91: cell centres */
+  for (int i = 0; i < n; i++) {
+    const double* s = src + 6 * (size_t)i;
+    const double px = s[0], py = s[1], pe = s[2], sx = s[3], sy = s[4], se = s[5];
+    /* profile_3d.cpp:310-323: integer (n - 1)/2 in the index, clamped windows */
+    const int ix0 = (int)(px / dx + (nx - 1) / 2);
+    const int iy0 = (int)(py / dy + (ny - 1) / 2);
+    const int ie0 = (int)(pe / deta + (neta - 1) / 2);
+    const int rx = (int)(6 * sx / dx), ry = (int)(6 * sy / dy), re = (int)(6 * se / deta);
+    const int xl = ix0 - rx > 0 ? ix0 - rx : 0, xr = ix0 + rx < nx ? ix0 + rx : nx;
+    const int yl = iy0 - ry > 0 ? iy0 - ry : 0, yr = iy0 + ry < ny ? iy0 + ry : ny;
+    const int el = ie0 - re > 0 ? ie0 - re : 0, er = ie0 + re < neta ? ie0 + re : neta;
+    /* profile_3d.cpp:325-343 */
+    for (int j = el; j < er; j++) {
+      const double ej = (j - (neta - 1) / 2.) * deta;
+      const double dis_eta = (ej - pe) * (ej - pe) / (2. * se * se);
+      const double norm_eta = 1. / sqrt(2. * M_PI * se * se);
+      for (int k = xl; k < xr; k++) {
+        const double xk = (k - (nx - 1) / 2.) * dx;
+        const double dis_x = (xk - px) * (xk - px) / (2. * sx * sx);
+        const double norm_x = 1. / sqrt(2. * M_PI * sx * sx);
+        for (int l = yl; l < yr; l++) {
+          const double yl_ = (l - (ny - 1) / 2.) * dy;
+          const double dis_y = (yl_ - py) * (yl_ - py) / (2. * sy * sy);
+          const double norm_y = 1. / sqrt(2. * M_PI * sy * sy);
+          rho[((size_t)j * nx + k) * ny + l] += exp(-dis_eta - dis_x - dis_y) * norm_eta * norm_x * norm_y;
+        }
+      }
+    }
+  }
+}

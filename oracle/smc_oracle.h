@@ -160,6 +160,20 @@ typedef struct {
 int orc_run_events(const orc_event_cfg* c, uint64_t seed, int nev,
                    double* eps2_out /* nev x 2, may be NULL */);
 
+/* ---- 3D profiles, scripts/generate_3d_profiles/profile_3d.cpp ---- */
+/* profile_3d.cpp:24-27 and 133-155: y_beam, sigma_inelastic [fm^2] and the
+ * n-point eta table (eta, projectile and target weights); maxes = {proj, targ}. */
+void orc_p3d_setup(double ecm, int n, double* y_beam, double* sigma_inel_fm2, double* eta, double* proj,
+                   double* targ, double* maxes);
+/* profile_3d.cpp:402-434: index of the largest entry below v (-1 out of range
+ * when skip_out_of_range, which is the only mode the sampler uses). */
+long orc_p3d_search(const double* A, int n, double v);
+/* profile_3d.cpp:302-344 (participants) = :346-400 (binary collisions):
+ * rho[neta][nx][ny] += norm exp(-dis_eta - dis_x - dis_y) for n sources of 6
+ * doubles {x, y, eta, sigma_x, sigma_y, sigma_eta}; rho is not cleared. */
+void orc_p3d_smear(int nx, int ny, int neta, double dx, double dy, double deta, int n, const double* src,
+                   double* rho);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,21 @@ class SmcDerived(C.Structure):
     ]
 
 
+class P3dParams(C.Structure):
+    """smc_p3d_params (include/smc_p3d.h): profile_3d's constructor arguments."""
+    _fields_ = [("grid_nx", C.c_int), ("grid_ny", C.c_int), ("grid_neta", C.c_int),
+                ("grid_dx", C.c_double), ("grid_dy", C.c_double), ("grid_deta", C.c_double),
+                ("ecm", C.c_double), ("random_flag", C.c_int)]
+
+
+class P3dDerived(C.Structure):
+    _fields_ = [("y_beam", C.c_double), ("sigma_inelastic", C.c_double), ("eta0", C.c_double),
+                ("sigma_x0", C.c_double), ("sigma_eta0", C.c_double), ("proj_max", C.c_double),
+                ("targ_max", C.c_double)]
+
+
+P3D_TABLE = 1000
+
 _P = C.POINTER
 _dp = _P(C.c_double)
 _ip = _P(C.c_int)
@@ -102,6 +117,15 @@ SIGNATURES = {
     "smc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "smc_get_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
     "smc_synchronize": (C.c_int, [C.c_void_p]),
+    # include/smc_p3d.h
+    "smc_p3d_create": (C.c_int, [_P(C.c_void_p), _P(P3dParams)]),
+    "smc_p3d_destroy": (None, [C.c_void_p]),
+    "smc_p3d_last_error": (C.c_char_p, [C.c_void_p]),
+    "smc_p3d_get_derived": (C.c_int, [C.c_void_p, _P(P3dDerived), _dp, _dp, _dp]),
+    "smc_p3d_sample": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _ip, _dp, _ip, C.c_uint64, C.c_uint64, _dp, _dp]),
+    "smc_p3d_smear": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _fp]),
+    "smc_p3d_generate": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _ip, _dp, _ip, C.c_uint64, C.c_uint64, _fp, _fp,
+                                   _dp, _dp]),
 }
 
 _lib = None
@@ -350,3 +374,89 @@ class Context:
 
     def synchronize(self):
         self._check(self.lib.smc_synchronize(self.h))
+
+
+def _events(lists, width, dtype):
+    """Concatenates per-event arrays of `width` columns; returns (flat, offsets)."""
+    arrs = [np.asarray(a, dtype).reshape(-1, width) if len(a) else np.zeros((0, width), dtype) for a in lists]
+    off = np.zeros(len(arrs) + 1, np.int32)
+    off[1:] = np.cumsum([len(a) for a in arrs])
+    flat = np.ascontiguousarray(np.concatenate(arrs) if arrs else np.zeros((0, width), dtype))
+    return flat, off
+
+
+class Profile3D:
+    """The 3D (x, y, eta) profiles of scripts/generate_3d_profiles
+    (include/smc_p3d.h): one handle per grid / ecm / random_flag."""
+
+    def __init__(self, grid_nx=261, grid_ny=261, grid_neta=101, grid_dx=0.1, grid_dy=0.1, grid_deta=0.1,
+                 ecm=19.6, random_flag=1):
+        self.lib = load()
+        self.params = P3dParams(grid_nx, grid_ny, grid_neta, grid_dx, grid_dy, grid_deta, ecm, random_flag)
+        h = C.c_void_p()
+        rc = self.lib.smc_p3d_create(C.byref(h), C.byref(self.params))
+        if rc != SMC_OK:
+            raise SmcError(rc, "smc_p3d_create failed")
+        self.h = h
+        self.shape = (grid_neta, grid_nx, grid_ny)
+
+    def close(self):
+        if self.h:
+            self.lib.smc_p3d_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _check(self, rc):
+        if rc != SMC_OK:
+            raise SmcError(rc, self.lib.smc_p3d_last_error(self.h).decode())
+
+    def derived(self):
+        d = P3dDerived()
+        eta, proj, targ = (np.zeros(P3D_TABLE) for _ in range(3))
+        self._check(self.lib.smc_p3d_get_derived(self.h, C.byref(d), _ptr(eta, C.c_double), _ptr(proj, C.c_double),
+                                                 _ptr(targ, C.c_double)))
+        return d, eta, proj, targ
+
+    def _lists(self, parts, bins):
+        pxy, poff = _events([np.asarray(p, np.float64)[:, :2] if len(p) else [] for p in parts], 2, np.float64)
+        pid, _ = _events([np.asarray(p, np.float64)[:, 2].astype(np.int32) if len(p) else [] for p in parts], 1,
+                         np.int32)
+        bxy, boff = _events([np.asarray(b, np.float64)[:, :2] if len(b) else [] for b in bins], 2, np.float64)
+        return pxy, np.ascontiguousarray(pid.ravel()), poff, bxy, boff
+
+    def sample(self, parts, bins, seed, event0=0):
+        """parts[e]: rows (x, y, id); bins[e]: rows (x, y).  Returns per-event
+        source records (x, y, eta, sigma_x, sigma_y, sigma_eta)."""
+        pxy, pid, poff, bxy, boff = self._lists(parts, bins)
+        ps, bs = np.zeros((poff[-1], 6)), np.zeros((boff[-1], 6))
+        self._check(self.lib.smc_p3d_sample(self.h, len(parts), _ptr(pxy, C.c_double), _ptr(pid, C.c_int),
+                                            _ptr(poff, C.c_int), _ptr(bxy, C.c_double), _ptr(boff, C.c_int), seed,
+                                            event0, _ptr(ps, C.c_double), _ptr(bs, C.c_double)))
+        return ([ps[poff[e]:poff[e + 1]] for e in range(len(parts))],
+                [bs[boff[e]:boff[e + 1]] for e in range(len(bins))])
+
+    def smear(self, srcs):
+        """srcs[e]: source records; returns float32 [nev, neta, nx, ny]."""
+        flat, off = _events(srcs, 6, np.float64)
+        rho = np.zeros((len(srcs),) + self.shape, np.float32)
+        self._check(self.lib.smc_p3d_smear(self.h, len(srcs), _ptr(flat, C.c_double), _ptr(off, C.c_int),
+                                           _ptr(rho, C.c_float)))
+        return rho
+
+    def generate(self, parts, bins, seed, event0=0, binary=True):
+        """Sampling and both grids: (rho_part, rho_binary or None, part_src, bin_src)."""
+        pxy, pid, poff, bxy, boff = self._lists(parts, bins)
+        nev = len(parts)
+        rp = np.zeros((nev,) + self.shape, np.float32)
+        rb = np.zeros((nev,) + self.shape, np.float32) if binary else None
+        ps, bs = np.zeros((poff[-1], 6)), np.zeros((boff[-1], 6))
+        self._check(self.lib.smc_p3d_generate(self.h, nev, _ptr(pxy, C.c_double), _ptr(pid, C.c_int),
+                                              _ptr(poff, C.c_int), _ptr(bxy, C.c_double), _ptr(boff, C.c_int), seed,
+                                              event0, _ptr(rp, C.c_float), _ptr(rb, C.c_float),
+                                              _ptr(ps, C.c_double), _ptr(bs, C.c_double)))
+        return rp, rb, ps, bs

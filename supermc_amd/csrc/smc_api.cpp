@@ -38,14 +38,18 @@ T* dalloc(size_t n) {
   return (T*)p;
 }
 
+}  // namespace
+
 // Regge96.cpp:27-50 (pp total, baryon-baryon elastic); MCnucl.cpp:58-64
-double regge_sigma_inel(double ecm) {
+double smc::regge_sigma_inel(double ecm) {
   const double s = ecm * ecm;
   const double sig = 22.0 * std::pow(s, 0.079) + 56.1 * std::pow(s, -0.46);
   const double bel = 2.0 * 2.3 + 2.0 * 2.3 + 4.0 * std::pow(s, 0.0808) - 4.2;
   const double sigel = 0.0511 * sig * sig / bel;
   return sig - sigel;
 }
+
+namespace {
 
 // Woods-Saxon radius / diffuseness per mass number (Nucleus.cpp:65-111)
 void ws_table(int A, double& R, double& a) {

@@ -175,6 +175,25 @@ int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  bool bf16x3, void* stream);
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                 void* stream);
+// 3D profiles (smc_p3d.hip), scripts/generate_3d_profiles/profile_3d.cpp
+struct P3dCfg {
+  int nx, ny, neta;
+  double dx, dy, deta;
+  int random_flag;
+  double y_beam, eta0, sigma_x0, sigma_eta0, dsigma_x, dsigma_eta;
+  const double* tab_eta;  // tab_n entries each (device)
+  const double* tab_proj;
+  const double* tab_targ;
+  int tab_n;
+  double proj_max, targ_max;
+};
+int launch_p3d_sample(const P3dCfg& c, unsigned long long seed, unsigned long long event0, int nev,
+                      const double* part_xy, const int* part_id, const int* part_off, const double* bin_xy,
+                      const int* bin_off, double* part_src, double* bin_src, int* status, void* stream);
+int launch_p3d_smear(const P3dCfg& c, int nev, const double* src, const int* src_off, float* rho, void* stream);
+// Regge96.cpp:27-50 sigma_tot - sigma_el [mb] (smc_api.cpp)
+double regge_sigma_inel(double ecm);
+
 // MC-KLN density in one pass: T_A (acc 0) and T_B (acc 1) of s, then dN/dy
 // from the tmax x tmax table at (T_A/dT, T_B/dT); s.cm_out gets the output's
 // centre-of-mass partials (tile_units per event) when set

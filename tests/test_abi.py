@@ -1,5 +1,5 @@
 """CPU checks of the C-ABI library: it loads, exports every symbol that
-include/smc.h declares, its struct layouts match the binding, its defaults
+include/*.h declares, its struct layouts match the binding, its defaults
 are the reference's parameters.dat, and parameter validation maps onto the
 documented status codes.  No device compute is issued here."""
 import ctypes as C
@@ -15,9 +15,10 @@ GOLD = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_vectors.j
 
 
 def _declared_symbols():
-    txt = open(os.path.join(ROOT, "include", "smc.h")).read()
+    inc = os.path.join(ROOT, "include")
+    txt = "".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc)) if f.endswith(".h"))
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    return sorted(set(re.findall(r"\b(smc_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(smc_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -66,4 +67,19 @@ def test_validation_status_codes(override, code):
     h = C.c_void_p()
     rc = lib.smc_create(C.byref(h), C.byref(p))
     assert rc == code
+    assert not h.value
+
+
+@pytest.mark.parametrize("override", [dict(random_flag=4), dict(random_flag=-1), dict(grid_nx=0),
+                                      dict(grid_deta=0.0), dict(ecm=1.5)])
+def test_p3d_parameter_validation(override):
+    # smc_p3d_create validates before touching the device
+    from supermc_amd import _native
+    lib = _native.load()
+    args = dict(grid_nx=261, grid_ny=261, grid_neta=101, grid_dx=0.1, grid_dy=0.1, grid_deta=0.1, ecm=19.6,
+                random_flag=1)
+    args.update(override)
+    p = _native.P3dParams(**args)
+    h = C.c_void_p()
+    assert lib.smc_p3d_create(C.byref(h), C.byref(p)) == 1
     assert not h.value

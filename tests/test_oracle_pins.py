@@ -318,3 +318,78 @@ def test_rotated_profiles_order3_and_reaction_plane(orc):
         assert np.abs(rp[0] - 2.0 * ref).max() < 0.02 * rp[0].max()
     for p in outs[1:]:
         assert np.abs(p - outs[0]).max() < 0.02 * outs[0].max()
+
+
+# ---- 3D profiles (scripts/generate_3d_profiles/profile_3d.cpp) -------------
+# The reference class needs GSL and cannot be built here: these pin the
+# oracle's restatement by identities a slip in it would break.
+
+def test_p3d_beam_rapidity_and_eta_table(orc):
+    for ecm in (19.6, 62.4, 200.0):
+        t = orc.p3d_setup(ecm)
+        yb, eta = t["y_beam"], t["eta"]
+        # y_beam is the rapidity of a nucleon of energy ecm/2: cosh(y) = gamma
+        assert math.cosh(yb) == pytest.approx(ecm / 2.0, rel=1e-12)
+        assert eta[0] == -yb and eta[-1] == pytest.approx(yb, abs=1e-12)
+        assert np.allclose(np.diff(eta), 2 * yb / 999, rtol=0, atol=1e-12)
+        # the two tables are mirror images and add up to twice the fall-off
+        assert np.allclose(t["proj"], t["targ"][::-1], rtol=0, atol=1e-12)
+        fall = np.where(np.abs(eta) > 2.5, np.exp(-(np.abs(eta) - 2.5) ** 2 / 0.5), 1.0)
+        assert np.allclose(t["proj"] + t["targ"], 2 * fall, rtol=0, atol=1e-12)
+        # on the plateau the projectile weight falls linearly from 1 + 2.5/y_beam
+        plateau = np.abs(eta) <= 2.5
+        assert np.allclose(t["proj"][plateau], 1 - eta[plateau] / yb, rtol=0, atol=1e-12)
+        assert t["proj_max"] == t["proj"].max() and t["targ_max"] == t["targ"].max()
+        assert t["proj_max"] >= 1 + min(2.5, yb) / yb - 1e-9
+        # sigma_inelastic is the Regge96 value in fm^2
+        assert t["sigma_inel"] == pytest.approx(orc.lib().orc_sigma_inel(ecm) * 0.1, rel=1e-15)
+
+
+def test_p3d_search_is_largest_entry_below(orc):
+    rng = np.random.default_rng(5)
+    A = orc.p3d_setup(19.6)["eta"]
+    for v in rng.uniform(A[0] - 0.5, A[-1] + 0.5, 400):
+        k = orc.p3d_search(A, v)
+        if v > A[-1] or v < A[0]:
+            assert k == -1
+        else:
+            below = np.nonzero(A < v)[0]
+            assert k == (below[-1] if len(below) else 0)
+
+
+def test_p3d_smear_normalisation_and_windows(orc):
+    # one source well inside the grid integrates to 1 (tails beyond 6 sigma
+    # and the half-open windows drop < 1e-7), and exactly its window's cells
+    # are non-zero: [idx0 - range, idx0 + range) with idx0 = (int)(p/d + 130)
+    nx = ny = 261
+    neta = 101
+    for (x, y, eta, sx, sy, se) in [(0.33, -1.27, 0.41, 0.3575, 0.3575, 0.5),
+                                    (2.05, 0.01, -1.93, 0.21, 0.55, 0.37),
+                                    (-4.44, 3.3, 1.1, 0.66, 0.4, 0.6)]:
+        rho = orc.p3d_smear(nx, ny, neta, 0.1, 0.1, 0.1, [[x, y, eta, sx, sy, se]])
+        assert rho.sum() * 0.1 ** 3 == pytest.approx(1.0, abs=1e-6)
+        ix0, iy0, ie0 = int(x / 0.1 + 130), int(y / 0.1 + 130), int(eta / 0.1 + 50)
+        rx, ry, re = int(6 * sx / 0.1), int(6 * sy / 0.1), int(6 * se / 0.1)
+        nz = np.argwhere(rho > 0)
+        assert nz[:, 0].min() == ie0 - re and nz[:, 0].max() == ie0 + re - 1
+        assert nz[:, 1].min() == ix0 - rx and nz[:, 1].max() == ix0 + rx - 1
+        assert nz[:, 2].min() == iy0 - ry and nz[:, 2].max() == iy0 + ry - 1
+        assert len(nz) == (2 * rx) * (2 * ry) * (2 * re)
+    # the concrete window of a documented case: x = 0.33 fm -> cells 112..153
+    rho = orc.p3d_smear(nx, ny, neta, 0.1, 0.1, 0.1, [[0.33, 0.0, 0.0, 0.3575, 0.3575, 0.5]])
+    xs = np.nonzero(rho.sum(axis=(0, 2)))[0]
+    assert xs[0] == 112 and xs[-1] == 153
+
+
+def test_p3d_smear_is_separable_and_clamped(orc):
+    # a cell's value is the product of three normalised 1D Gaussians, and a
+    # source at the grid edge loses the clamped part of its window
+    src = [0.12, -0.47, 0.33, 0.3, 0.45, 0.6]
+    rho = orc.p3d_smear(61, 61, 21, 0.1, 0.1, 0.1, [src])
+    j, k, l = 12, 31, 26
+    ej, xk, yl = (j - 10.0) * 0.1, (k - 30.0) * 0.1, (l - 30.0) * 0.1
+    g = lambda d, s: math.exp(-d * d / (2 * s * s)) / math.sqrt(2 * math.pi * s * s)
+    assert rho[j, k, l] == pytest.approx(g(ej - src[2], src[5]) * g(xk - src[0], src[3]) * g(yl - src[1], src[4]),
+                                         rel=1e-13)
+    edge = orc.p3d_smear(61, 61, 21, 0.1, 0.1, 0.1, [[2.95, 0.0, 0.0, 0.3, 0.3, 0.5]])
+    assert edge[:, 60, :].max() > 0 and 0.3 < edge.sum() * 1e-3 < 0.7
