@@ -33,7 +33,8 @@ def _run(cmd):
 
 
 HOST_DIR = os.path.join(HERE, "host")
-FACADE_SOURCES = ["ParameterReader.cpp", "smc_host.cpp", "MCnucl.cpp", "MakeDensity.cpp", "KLNModel.cpp"]
+FACADE_SOURCES = ["ParameterReader.cpp", "smc_host.cpp", "MCnucl.cpp", "MakeDensity.cpp", "KLNModel.cpp",
+                  "profile_3d.cpp"]
 CXX = os.environ.get("CXX", "g++")
 
 
@@ -54,6 +55,10 @@ def build_facade():
     _run([CXX, "-O2", "-std=c++17", "-Wall", *inc, os.path.join(HOST_DIR, "superMC.cpp"), "-o", exe + ".tmp",
           "-L", LIBDIR, "-lsmc_facade", "-lsmc", rpath])
     os.replace(exe + ".tmp", exe)
+    exe3 = os.path.join(bindir, "generate_profile_3d.e")  # scripts/generate_3d_profiles' driver
+    _run([CXX, "-O2", "-std=c++17", "-Wall", *inc, os.path.join(HOST_DIR, "generate_profile_3d.cpp"), "-o",
+          exe3 + ".tmp", "-L", LIBDIR, "-lsmc_facade", "-lsmc", rpath])
+    os.replace(exe3 + ".tmp", exe3)
     check_src = os.path.join(ROOT, "tests", "facade", "facade_check.cpp")
     if os.path.exists(check_src):  # the facade's test driver
         chk = os.path.join(ROOT, "tests", "facade", "facade_check.bin")

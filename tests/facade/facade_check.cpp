@@ -2,14 +2,19 @@
 //   facade_check reader <params-file> [key=value ...]   parse + print names/values (no GPU)
 //   facade_check mcnucl <params-file> b0 b1 ...         one event per b through MCnucl
 //   facade_check kln <ecm> <lambda> ta0 tb0 ta1 tb1 ... KLNModel::getdNdy per point
+//   facade_check p3d <parts> <bins> nx ny neta dx deta ecm random_flag alpha <out>
+//                                                       profile_3d: both grids, entropy file
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
 
+#include <fstream>
+
 #include "KLNModel.h"
 #include "MCnucl.h"
 #include "ParameterReader.h"
+#include "profile_3d.h"
 
 int main(int argc, char* argv[]) {
   if (argc < 3) return 2;
@@ -48,6 +53,27 @@ int main(int argc, char* argv[]) {
                   nc > 0 ? mc.getdNdy() : 0.0, ta, tb, rho);
       mc.deleteNucleus();
     }
+    return 0;
+  }
+  if (!std::strcmp(mode, "p3d") && argc == 13) {
+    std::vector<participant_info> parts, bins;
+    std::ifstream pf(argv[2]), bf(argv[3]);
+    participant_info t{};
+    while (pf >> t.x >> t.y >> t.id) parts.push_back(t);
+    while (bf >> t.x >> t.y) bins.push_back(t);
+    const int nx = std::atoi(argv[4]), ny = std::atoi(argv[5]), neta = std::atoi(argv[6]);
+    const double dx = std::atof(argv[7]), deta = std::atof(argv[8]), ecm = std::atof(argv[9]);
+    profile_3d p(parts, bins, nx, ny, neta, dx, dx, deta, ecm, std::atoi(argv[10]));
+    p.generate_3d_profile();
+    p.generate_3d_profile_binary();
+    p.output_3d_entropy_profile(argv[12], std::atof(argv[11]));
+    std::printf("seed %llu\n", (unsigned long long)p.random_seed());
+    for (const participant_info& q : p.participants())
+      std::printf("part %.17g %.17g %.17g %.17g\n", q.eta, q.sigma_x, q.sigma_y, q.sigma_eta);
+    std::printf("draw %.17g\n", p.sample_eta_distribution_from_array(1));
+    std::vector<double> tab{-1.0, 0.0, 0.5, 2.0};
+    std::printf("search %ld %ld %ld\n", p.binarySearch(&tab, 0.25, true), p.binarySearch(&tab, 3.0, true),
+                p.binarySearch(&tab, -1.0, true));
     return 0;
   }
   if (!std::strcmp(mode, "kln")) {

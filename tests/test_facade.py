@@ -354,3 +354,91 @@ def test_klnmodel_facade_matches_oracle():
     for (ta, tb), g in zip(pts, got):
         ref = oracle.kln_dndy_gl(k, ta, tb, 32, 32, 48)
         assert g == pytest.approx(ref, rel=1e-11, abs=0.0)
+
+
+# ---- scripts/generate_3d_profiles: profile_3d and generate_profile_3d.e ----
+EXE3D = os.path.join(ROOT, "supermc_amd", "bin", "generate_profile_3d.e")
+
+
+def _write_tables(tmp_path, parts, bins):
+    """Operation-2 table formats: x, y (setprecision(3), setw(10)) and the
+    nucleus id (Nucleus.cpp:753-764, MCnucl.cpp:1177-1190)."""
+    pf, bf = tmp_path / "participants.dat", tmp_path / "binary.dat"
+    pf.write_text("".join("%10.3g   %10.3g   %d\n" % (x, y, i) for x, y, i in parts))
+    bf.write_text("".join("%10.3g%10.3g\n" % (x, y) for x, y in bins))
+    return pf, bf
+
+
+def test_3d_driver_usage():
+    r = subprocess.run([EXE3D], capture_output=True, text=True)
+    assert r.returncode == 0
+    assert r.stdout.strip() == "Usage: generate_profile_3d.e participant_list binary_collision_list"
+
+
+def _event_tables(rng, npart, nbin):
+    parts = [(x, y, 1 + (k % 2)) for k, (x, y) in enumerate(rng.uniform(-5, 5, (npart, 2)))]
+    bins = [tuple(v) for v in rng.uniform(-4, 4, (nbin, 2))]
+    return parts, bins
+
+
+def _read_back(pf, bf):
+    p = np.loadtxt(pf, ndmin=2)
+    b = np.loadtxt(bf, ndmin=2)
+    return p, b
+
+
+@pytest.mark.gpu
+def test_profile_3d_facade_matches_c_abi(tmp_path):
+    from supermc_amd import Profile3D
+    rng = np.random.default_rng(21)
+    parts, bins = _event_tables(rng, 24, 37)
+    pf, bf = _write_tables(tmp_path, parts, bins)
+    out = tmp_path / "sd.dat"
+    nx, ny, neta, alpha = 41, 41, 21, 0.14
+    r = subprocess.run([CHECK, "p3d", str(pf), str(bf), str(nx), str(ny), str(neta), "0.25", "0.3", "19.6", "3",
+                        str(alpha), str(out)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = r.stdout.splitlines()
+    assert lines[0].startswith("ecm = 19.6 A GeV, y_beam = ")
+    seed = int(next(l for l in lines if l.startswith("seed ")).split()[1])
+    p, b = _read_back(pf, bf)
+    with Profile3D(nx, ny, neta, 0.25, 0.25, 0.3, 19.6, 3) as P:
+        rp, rb, ps, bs = P.generate([p], [b[:, :2]], seed=seed, event0=0)
+    part_rows = np.array([[float(v) for v in l.split()[1:]] for l in lines if l.startswith("part ")])
+    np.testing.assert_array_equal(part_rows, ps[:, 2:])
+    sd = np.loadtxt(out).reshape(neta, nx, ny)
+    ref = rp[0].astype(np.float64) * (1 - alpha) / 2 + rb[0].astype(np.float64) * alpha
+    assert np.abs(sd - ref).max() <= 1e-7 * np.abs(ref).max()
+    # one line per (eta, x) with ny values in the reference's 18-wide columns
+    first = out.read_text().split("\n", 1)[0]
+    assert len(first) == ny * (18 + 3)
+    eta_draw = float(next(l for l in lines if l.startswith("draw ")).split()[1])
+    assert -3.0 < eta_draw < 3.0
+    assert next(l for l in lines if l.startswith("search ")).split()[1:] == ["1", "-1", "0"]
+
+
+@pytest.mark.gpu
+def test_generate_profile_3d_driver_end_to_end(tmp_path):
+    """generate_profile_3d.e on operation-2 tables: 261 x 261 x 101, ecm 19.6,
+    random_flag 1, participant density in test.dat."""
+    from supermc_amd import Profile3D
+    rng = np.random.default_rng(22)
+    parts, bins = _event_tables(rng, 60, 90)
+    pf, bf = _write_tables(tmp_path, parts, bins)
+    r = subprocess.run([EXE3D, str(pf), str(bf)], capture_output=True, text=True, cwd=tmp_path, timeout=110)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "Npart = 60" in r.stdout and "Ncoll = 90" in r.stdout
+    seed = int(next(l for l in r.stdout.splitlines() if l.startswith("random seed: ")).split()[2])
+    p, b = _read_back(pf, bf)
+    with Profile3D(random_flag=1) as P:
+        rp, _, _, _ = P.generate([p], [b[:, :2]], seed=seed, event0=0, binary=False)
+    n_lines = 0
+    with open(tmp_path / "test.dat") as f:
+        for i, line in enumerate(f):
+            n_lines += 1
+            if 50 * 261 <= i < 51 * 261 and i % 13 == 0:  # rows of the central eta slice
+                vals = np.array(line.split(), np.float64)
+                ref = rp[0][50, i - 50 * 261].astype(np.float64)
+                assert len(vals) == 261
+                assert np.abs(vals - ref).max() <= 1e-7 * max(rp[0].max(), 1e-30)
+    assert n_lines == 101 * 261
