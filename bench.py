@@ -103,6 +103,58 @@ def time_kln(nev, steps, seed):
                 failed_events=int((ev[:, 57] != 0).sum()), mean_eps2=float(np.hypot(ev[:, 2], ev[:, 12]).mean()))
 
 
+def p3d_lists(nev, seed):
+    """Operation-2 participant (x, y, nucleus id) and binary-collision (x, y)
+    lists of nev minimum-bias Pb+Pb events of the main workload."""
+    from supermc_amd import Context
+    parts, bins = [], []
+    with Context(bench_params(nev)) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)
+        for e in range(nev):
+            nuc = ctx.nucleons(e)
+            pa, pb, _ = ctx.pairs(e)
+            wa, wb = nuc["wound_proj"] >= 0, nuc["wound_targ"] >= 0
+            parts.append(np.concatenate([np.column_stack([nuc["xy_proj"][wa], np.ones(wa.sum())]),
+                                         np.column_stack([nuc["xy_targ"][wb], np.full(wb.sum(), 2.0)])]))
+            bins.append((nuc["xy_proj"][pa] + nuc["xy_targ"][pb]) / 2)
+    return parts, bins
+
+
+def time_p3d(nev, steps, seed):
+    """3D (x, y, eta) profiles of scripts/generate_3d_profiles (261 x 261 x 101,
+    ecm 19.6 GeV, random_flag 1) of nev events per call: eta sampling and both
+    grids on the device, the grids returned to the host (2 x 27.5 MB per
+    event, included in the time)."""
+    from supermc_amd import Profile3D
+    parts, bins = p3d_lists(nev, seed)
+    with Profile3D(random_flag=1) as P:
+        P.generate(parts, bins, seed, 0)  # warm-up
+        t0 = time.perf_counter()
+        for k in range(steps):
+            P.generate(parts, bins, seed, (k + 1) * nev)
+        dt = time.perf_counter() - t0
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, grid=[101, 261, 261],
+                mean_npart=float(np.mean([len(p) for p in parts])), mean_ncoll=float(np.mean([len(b) for b in bins])),
+                note="wall time of smc_p3d_generate incl. the device-to-host copy of both grids")
+
+
+def cpu_p3d_baseline(seed, nev=4):
+    """The oracle's FP64 restatement of profile_3d::generate_3d_profile +
+    generate_3d_profile_binary on the sampled sources of the first nev events
+    of the GPU leg, one core."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    from supermc_amd import Profile3D
+    parts, bins = p3d_lists(nev, seed)
+    with Profile3D(random_flag=1) as P:
+        ps, bs = P.sample(parts, bins, seed, 0)
+    t = time.perf_counter()
+    for e in range(nev):
+        oracle.p3d_smear(261, 261, 101, 0.1, 0.1, 0.1, ps[e])
+        oracle.p3d_smear(261, 261, 101, 0.1, 0.1, 0.1, bs[e])
+    return nev / (time.perf_counter() - t)
+
+
 def time_adjoint(nev, steps, seed):
     """Differentiable Pb+Pb batch: forward (|eps_2|, |eps_3|, dS/dy) plus
     the adjoint d/d(w, R, a, alpha, k), per event."""
@@ -187,7 +239,8 @@ def main():
                     help="events of the last batch re-evaluated by the FP64 oracle for the eps_n error")
     ap.add_argument("--kln-events", type=int, default=20000, help="MC-KLN events per step (0: skip)")
     ap.add_argument("--adjoint-events", type=int, default=4096, help="differentiable batch size (0: skip)")
-    ap.add_argument("--only", choices=("kln", "adjoint"), default=None,
+    ap.add_argument("--p3d-events", type=int, default=16, help="3D profile events per call (0: skip)")
+    ap.add_argument("--only", choices=("kln", "adjoint", "p3d"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
@@ -205,9 +258,12 @@ def main():
         if args.only == "kln":
             r = time_kln(args.kln_events, args.steps, args.seed)
             metric, wl = "events/s MC-KLN Pb+Pb 2.76 TeV 261^2 grid", "MC-KLN Pb+Pb, dN/dy table, 261x261"
-        else:
+        elif args.only == "adjoint":
             r = time_adjoint(args.adjoint_events, args.steps, args.seed)
             metric, wl = "events/s differentiable Pb+Pb 261^2 (forward + adjoint)", "frozen-topology Pb+Pb batch"
+        else:
+            r = time_p3d(args.p3d_events, args.steps, args.seed)
+            metric, wl = "events/s 3D profiles 101x261x261", "eta-sampled 3D Gaussians of Pb+Pb operation-2 lists"
         print(json.dumps({"metric": metric, "value": r["events_per_s"], "unit": "events/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": 1, "higher_is_better": True, "dtype": "f32",
                           "data": "synthetic", "config": dict(workload=wl, **r)}), flush=True)
@@ -274,10 +330,13 @@ def main():
 
     kln = time_kln(args.kln_events, 2, args.seed) if args.kln_events > 0 else None
     adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
+    p3d = time_p3d(args.p3d_events, 2, args.seed) if args.p3d_events > 0 else None
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
+        if p3d is not None:
+            p3d["cpu_oracle_events_per_s_1core"] = cpu_p3d_baseline(args.seed)
 
     line = {
         "metric": "events/s Pb+Pb 261^2 grid (Glauber, KLN); fwd+adjoint; eps_n err vs FP64",
@@ -299,7 +358,7 @@ def main():
                                   "BF16x3 MFMA Gx.diag(w).Gy^T"][args.smear_path],
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
-                   "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj},
+                   "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
