@@ -155,6 +155,25 @@ def cpu_p3d_baseline(seed, nev=4):
     return nev / (time.perf_counter() - t)
 
 
+def time_graph(seed):
+    """Batches launched eagerly (smc_run) and replayed from a captured
+    hipGraph (smc_run_graph), at 10^3 and 10^5 events per batch."""
+    from supermc_amd import Context
+    out = {}
+    for nev, steps in ((1000, 50), (100000, 3)):
+        with Context(bench_params(nev)) as ctx:
+            for mode, fn in (("eager", ctx.run), ("graph", ctx.run_graph)):
+                fn(seed, 0, nev, fetch=False)  # for the graph: the eager call, then the capture
+                fn(seed, nev, nev, fetch=False)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                for k in range(steps):
+                    fn(seed, (k + 2) * nev, nev, fetch=False)
+                ctx.synchronize()
+                out[f"{mode}_events_per_s_{nev}"] = nev * steps / (time.perf_counter() - t0)
+    return out
+
+
 def time_adjoint(nev, steps, seed):
     """Differentiable Pb+Pb batch: forward (|eps_2|, |eps_3|, dS/dy) plus
     the adjoint d/d(w, R, a, alpha, k), per event."""
@@ -240,6 +259,7 @@ def main():
     ap.add_argument("--kln-events", type=int, default=20000, help="MC-KLN events per step (0: skip)")
     ap.add_argument("--adjoint-events", type=int, default=4096, help="differentiable batch size (0: skip)")
     ap.add_argument("--p3d-events", type=int, default=16, help="3D profile events per call (0: skip)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the eager vs hipGraph-replay comparison")
     ap.add_argument("--only", choices=("kln", "adjoint", "p3d"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
@@ -331,6 +351,7 @@ def main():
     kln = time_kln(args.kln_events, 2, args.seed) if args.kln_events > 0 else None
     adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
     p3d = time_p3d(args.p3d_events, 2, args.seed) if args.p3d_events > 0 else None
+    graph = time_graph(args.seed) if not args.no_graph else None
 
     cpu = None
     if not args.no_cpu:
@@ -358,7 +379,8 @@ def main():
                                   "BF16x3 MFMA Gx.diag(w).Gy^T"][args.smear_path],
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
-                   "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d},
+                   "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
+                   "graph_replay": graph},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

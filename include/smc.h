@@ -188,6 +188,12 @@ smc_status smc_moments(smc_ctx* ctx, double* ev_out);
  * moments, inputs and outputs resident on the device.  ev_out as above
  * (NULL: keep on device, fetch with smc_fetch_events). */
 smc_status smc_run(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
+/* smc_run replayed as a hipGraph (no counterpart in the reference: for
+ * callers that launch many same-sized batches).  The first call for a batch
+ * size runs eagerly, the second captures the launch sequence, later calls
+ * set the event kernel's seed / event0 and launch the graph.  Results are
+ * smc_run's; per-kernel timing covers the eager calls only. */
+smc_status smc_run_graph(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
 smc_status smc_fetch_events(smc_ctx* ctx, int nev, double* ev_out);
 
 /* Operation 2 grids (MakeDensity::generate_profile_ebe_Jet).

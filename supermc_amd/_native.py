@@ -104,6 +104,7 @@ SIGNATURES = {
     "smc_density": (C.c_int, [C.c_void_p, _fp, _dp]),
     "smc_moments": (C.c_int, [C.c_void_p, _dp]),
     "smc_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
+    "smc_run_graph": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
     "smc_fetch_events": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "smc_average_profiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp, _dp]),
     "smc_rho_binary": (C.c_int, [C.c_void_p, _fp]),
@@ -280,6 +281,13 @@ class Context:
     def run(self, seed, event0, nev, fetch=True):
         ev = np.zeros((nev, EV_STRIDE)) if fetch else None
         self._check(self.lib.smc_run(self.h, seed, event0, nev, _ptr(ev, C.c_double)))
+        self.nev = nev
+        return ev
+
+    def run_graph(self, seed, event0, nev, fetch=True):
+        """run() through a replayed hipGraph (smc_run_graph)."""
+        ev = np.zeros((nev, EV_STRIDE)) if fetch else None
+        self._check(self.lib.smc_run_graph(self.h, seed, event0, nev, _ptr(ev, C.c_double)))
         self.nev = nev
         return ev
 

@@ -134,6 +134,16 @@ struct smc_ctx {
   EvBufs buf{};
   hipStream_t stream = nullptr;
   std::string err;
+  // smc_run_graph: the captured batch of graph_nev events (k_event's node
+  // takes new key / event0 values per replay)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  hipGraphNode_t graph_event_node = nullptr;
+  hipKernelNodeParams graph_event_params{};
+  std::vector<void*> graph_args;
+  uint2 graph_key{0, 0};
+  unsigned long long graph_event0 = 0;
+  int graph_nev = 0, graph_warm = 0, graph_cm_nu = 0;
   int cap = 0;        // max events
   int nev = 0;        // events currently held
   int have_events = 0, have_density = 0, have_thickness = 0, have_moments = 0;
@@ -1037,6 +1047,8 @@ smc_status smc_create(smc_ctx** out, const smc_params* p) {
 
 void smc_destroy(smc_ctx* c) {
   if (!c) return;
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
@@ -1286,6 +1298,102 @@ smc_status smc_run(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* 
     if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
     run_event(c, nev, seed, event0, 0);
     run_moments(c, 0);
+    c->have_moments = 1;
+    if (ev_out) {
+      HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double),
+                            hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+namespace {
+
+void graph_release(smc_ctx* c) {
+  if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->graph_exec = nullptr;
+  c->graph = nullptr;
+  c->graph_event_node = nullptr;
+  c->graph_args.clear();
+}
+
+// Captures smc_run's launches for nev events on the ctx's stream.  Every
+// buffer they touch exists already (the eager first call made them), and
+// the sequence has no memset node.
+void graph_capture(smc_ctx* c, int nev, uint64_t seed, uint64_t event0) {
+  const int timing = c->timing;
+  c->timing = 0;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  try {
+    c->have_density = 0;
+    run_event(c, nev, seed, event0, 0);
+    run_moments(c, 0);
+  } catch (...) {
+    hipGraph_t g = nullptr;
+    (void)hipStreamEndCapture(c->stream, &g);
+    if (g) (void)hipGraphDestroy(g);
+    c->timing = timing;
+    throw;
+  }
+  c->timing = timing;
+  HIPCHK(hipStreamEndCapture(c->stream, &c->graph));
+  c->graph_cm_nu = c->cm_nu;
+  size_t n = 0;
+  HIPCHK(hipGraphGetNodes(c->graph, nullptr, &n));
+  std::vector<hipGraphNode_t> nodes(n);
+  HIPCHK(hipGraphGetNodes(c->graph, nodes.data(), &n));
+  for (hipGraphNode_t node : nodes) {
+    hipGraphNodeType t;
+    HIPCHK(hipGraphNodeGetType(node, &t));
+    if (t != hipGraphNodeTypeKernel) continue;
+    hipKernelNodeParams kp{};
+    HIPCHK(hipGraphKernelNodeGetParams(node, &kp));
+    if (kp.func == event_kernel_symbol() && kp.kernelParams) {
+      c->graph_event_node = node;
+      c->graph_event_params = kp;
+      c->graph_args.assign(kp.kernelParams, kp.kernelParams + kEventKernelArgs);
+      break;
+    }
+  }
+  if (!c->graph_event_node) throw SmcError(SMC_ERR_HIP, "captured batch has no event kernel node");
+  c->graph_args[kEventArgKey] = &c->graph_key;
+  c->graph_args[kEventArgEvent0] = &c->graph_event0;
+  c->graph_event_params.kernelParams = c->graph_args.data();
+  HIPCHK(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+}
+
+}  // namespace
+
+// smc_run as a replayed hipGraph.  The first call for a batch size runs
+// eagerly (it allocates every buffer the batch uses); the second captures
+// the launch sequence; later calls only set the event kernel's seed and
+// event0 and launch the graph.  Results equal smc_run's.
+smc_status smc_run_graph(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* ev_out) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    if (c->graph_nev != nev) {
+      graph_release(c);
+      c->graph_nev = nev;
+      c->graph_warm = 0;
+    }
+    if (!c->graph_warm) {
+      run_event(c, nev, seed, event0, 0);
+      run_moments(c, 0);
+      c->graph_warm = 1;
+    } else {
+      if (!c->graph_exec) graph_capture(c, nev, seed, event0);
+      c->graph_key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
+      c->graph_event0 = event0;
+      HIPCHK(hipGraphExecKernelNodeSetParams(c->graph_exec, c->graph_event_node, &c->graph_event_params));
+      HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+      c->nev = nev;
+      c->have_events = c->have_density = 1;
+      c->have_thickness = 0;
+      c->cm_nu = c->graph_cm_nu;
+    }
     c->have_moments = 1;
     if (ev_out) {
       HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double),

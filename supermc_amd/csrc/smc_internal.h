@@ -146,6 +146,10 @@ struct SmearSpec {
 };
 
 // Launch wrappers (smc_kernels.hip)
+// k_event's host handle (matches the func of its captured graph node); its
+// kernel parameters are (cfg, buf, key, event0, mode)
+const void* event_kernel_symbol();
+constexpr int kEventKernelArgs = 5, kEventArgKey = 2, kEventArgEvent0 = 3;
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
                  unsigned long long event0, int mode, void* stream);
 int launch_smear(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,

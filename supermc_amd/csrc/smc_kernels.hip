@@ -590,6 +590,8 @@ __global__ __launch_bounds__(256) void k_smear(DevCfg cfg, EvBufs buf, SmearSpec
 // ------------------------------------------------------------ launchers
 static hipStream_t S(void* s) { return (hipStream_t)s; }
 
+const void* event_kernel_symbol() { return reinterpret_cast<const void*>(&k_event); }
+
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
                  unsigned long long event0, int mode, void* stream) {
   if (nev <= 0) return 0;
