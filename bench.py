@@ -260,6 +260,7 @@ def main():
     ap.add_argument("--adjoint-events", type=int, default=4096, help="differentiable batch size (0: skip)")
     ap.add_argument("--p3d-events", type=int, default=16, help="3D profile events per call (0: skip)")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager vs hipGraph-replay comparison")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
     ap.add_argument("--only", choices=("kln", "adjoint", "p3d"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
@@ -295,6 +296,12 @@ def main():
     for w in range(args.warmup):
         ctx.run(args.seed, w * nev, nev, fetch=False)
     ctx.synchronize()
+    tune = None
+    if args.smear_path == 0 and not args.no_autotune:
+        # tile shape chosen on the warm-up batch (smc_autotune; grids do not depend on it)
+        best, ms = ctx.autotune(reps=2)
+        tune = dict(variant=best, ms_per_variant=[round(float(m), 3) for m in ms],
+                    variants=["32x64 chunk 8", "16x64 chunk 8", "32x64 chunk 4", "64x64 chunk 8"])
     ctx.set_timing(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
@@ -380,7 +387,7 @@ def main():
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
                    "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
-                   "graph_replay": graph},
+                   "graph_replay": graph, "tile_autotune": tune},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -255,6 +255,16 @@ smc_status smc_diff_backward(smc_ctx* ctx, const double* cot, double* grad);
 smc_status smc_diff_topology(smc_ctx* ctx, int ev, double* uvw_proj, double* uvw_targ, double* b,
                              double* gamma_v, int cap, int* nsrc);
 
+/* Tile shapes of the exact-footprint smearing kernel (no counterpart in
+ * the reference): 0 = 32 x 64 tiles, 8-row chunks (default); 1 = 16 x 64;
+ * 2 = 32 x 64, 4-row chunks; 3 = 64 x 64.  Every shape gives the same grids.
+ * smc_autotune times each on the ctx's current events (reps launches after
+ * a warm-up), keeps the fastest, and returns it with the per-shape times in
+ * ms[SMC_TILE_VARIANTS] (either pointer may be NULL). */
+#define SMC_TILE_VARIANTS 4
+smc_status smc_set_tile_variant(smc_ctx* ctx, int variant);
+smc_status smc_autotune(smc_ctx* ctx, int reps, int* best, double* ms);
+
 /* Timing of the ctx's kernels with HIP events on its stream. */
 #define SMC_K_EVENT 0
 #define SMC_K_SMEAR 1

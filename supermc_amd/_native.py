@@ -115,6 +115,8 @@ SIGNATURES = {
     "smc_diff_forward": (C.c_int, [C.c_void_p, _dp, _dp]),
     "smc_diff_backward": (C.c_int, [C.c_void_p, _dp, _dp]),
     "smc_diff_topology": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip]),
+    "smc_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "smc_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
     "smc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
     "smc_get_timing": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip]),
     "smc_synchronize": (C.c_int, [C.c_void_p]),
@@ -283,6 +285,17 @@ class Context:
         self._check(self.lib.smc_run(self.h, seed, event0, nev, _ptr(ev, C.c_double)))
         self.nev = nev
         return ev
+
+    def set_tile_variant(self, variant):
+        self._check(self.lib.smc_set_tile_variant(self.h, variant))
+
+    def autotune(self, reps=3):
+        """Times every tile shape on the current events and keeps the
+        fastest: (best, ms per shape)."""
+        best = C.c_int()
+        ms = np.zeros(4)
+        self._check(self.lib.smc_autotune(self.h, reps, C.byref(best), _ptr(ms, C.c_double)))
+        return best.value, ms
 
     def run_graph(self, seed, event0, nev, fetch=True):
         """run() through a replayed hipGraph (smc_run_graph)."""

@@ -511,7 +511,7 @@ void alloc_buffers(smc_ctx* c) {
   b.src = dalloc<int4>(E * (size_t)b.src_cap);
   b.swin = dalloc<int4>(E * (size_t)b.src_cap);
   c->evout = dalloc<double>(E * SMC_EV_STRIDE);
-  c->cm = dalloc<double>(E * (size_t)std::max(tile_units(c->cfg), moments_cm_units_min()) * 4);
+  c->cm = dalloc<double>(E * (size_t)std::max(tile_units_max(c->cfg), moments_cm_units_min()) * 4);
   c->band = dalloc<double>(E * (size_t)moments_band_doubles());
   c->mrects = dalloc<unsigned long long>(E * 2 * kAMax);
   c->nmrect = dalloc<int>(E);
@@ -634,7 +634,7 @@ void run_kln_density(smc_ctx* c) {
                                  c->stream),
                  "k_tile (KLN)");
     ts.done();
-    c->cm_nu = tile_units(c->cfg);
+    c->cm_nu = tile_units_kln(c->cfg);
     return;
   }
   float* a = grid_buffer(c, &c->ta);
@@ -1400,6 +1400,59 @@ smc_status smc_run_graph(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, do
                             hipMemcpyDeviceToHost, c->stream));
       sync(c);
     }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_set_tile_variant(smc_ctx* c, int variant) {
+  if (!c) return SMC_ERR_ARG;
+  if (variant < 0 || variant >= kTileVariants) {
+    c->err = "tile variant out of range";
+    return SMC_ERR_ARG;
+  }
+  c->cfg.tile_variant = variant;
+  c->have_density = c->have_moments = 0;  // the centre-of-mass partials follow the tile shape
+  return SMC_OK;
+}
+
+// Times the exact-footprint smearing (k_tile) of the ctx's current events
+// in every tile shape, reps launches each after one untimed launch, and
+// keeps the fastest.  The grids do not depend on the shape.
+smc_status smc_autotune(smc_ctx* c, int reps, int* best, double* ms) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_run first");
+    if (reps < 1) throw SmcError(SMC_ERR_ARG, "reps must be positive");
+    const bool kln = c->p.which_mc_model == 1;
+    SmearSpec spec = kln ? thickness_spec(c) : density_spec(c);
+    float* out0 = kln ? grid_buffer(c, &c->ta) : grid_buffer(c, &c->grid);
+    float* out1 = spec.combine ? (kln ? grid_buffer(c, &c->tb) : grid_buffer(c, &c->tmp)) : nullptr;
+    if (spec.combine == 0) spec.cm_out = c->cm;
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    double t[kTileVariants];
+    int pick = 0;
+    for (int v = 0; v < kTileVariants; ++v) {
+      c->cfg.tile_variant = v;
+      check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
+      HIPCHK(hipEventRecord(a, c->stream));
+      for (int r = 0; r < reps; ++r)
+        check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
+      HIPCHK(hipEventRecord(b, c->stream));
+      HIPCHK(hipEventSynchronize(b));
+      float f = 0.f;
+      HIPCHK(hipEventElapsedTime(&f, a, b));
+      t[v] = f / reps;
+      if (t[v] < t[pick]) pick = v;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->cfg.tile_variant = pick;
+    c->have_density = c->have_thickness = c->have_moments = 0;
+    if (best) *best = pick;
+    if (ms)
+      for (int v = 0; v < kTileVariants; ++v) ms[v] = t[v];
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }

@@ -70,6 +70,7 @@ struct DevCfg {
   // scale K = 2^-126 / exp2(-cut2 * inv2w2 * log2 e), weight scale, and the
   // output scale 1 / (K tile_w)
   float tile_k, tile_w, tile_out;
+  int tile_variant;      // k_tile shape, 0 .. kTileVariants - 1 (smc_tile.hip)
   int hb_h;              // box half-width 4*max(w,qw) in half-quanta (rounded)
   float alpha, pref_part;
   int use_binary;        // alpha > 1e-8 (MCnucl.cpp:724)
@@ -173,7 +174,10 @@ int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int
 int launch_spectator_src(const DevCfg& cfg, const EvBufs& b, const EvBufs& sb, int nev, void* stream);
 int launch_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept, double ff, double* sum,
                      void* stream);
-int tile_units(const DevCfg& cfg);  // smearing tiles per event (units of SmearSpec::cm_out)
+constexpr int kTileVariants = 4;
+int tile_units(const DevCfg& cfg);  // smearing tiles per event of cfg's variant (units of SmearSpec::cm_out)
+int tile_units_max(const DevCfg& cfg);  // the most over all variants (buffer sizing)
+int tile_units_kln(const DevCfg& cfg);  // of the fused MC-KLN launch (one shape)
 // bf16x3: the BF16x3 split-precision kernel instead of the FP32 one (not for combine 1)
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                  bool bf16x3, void* stream);

@@ -240,7 +240,26 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
 
 constexpr int kTileRows = 32;
 
-int tile_units(const DevCfg& cfg) { return ((cfg.nx + kTileRows - 1) / kTileRows) * ((cfg.ny + 63) / 64); }
+// Tile shapes the autotuner chooses from (rows x 64 columns, row chunk,
+// paired row factors).  Every shape adds the same products in the same
+// source order per cell, so the grids do not depend on the choice.
+//   0: 32 rows, chunks of 8, paired   (the default)
+//   1: 16 rows, chunks of 8, paired
+//   2: 32 rows, chunks of 4, paired
+//   3: 64 rows, chunks of 8, unpaired
+constexpr int kVariantRows[kTileVariants] = {32, 16, 32, 64};
+
+static int units_for_rows(const DevCfg& cfg, int rows) { return ((cfg.nx + rows - 1) / rows) * ((cfg.ny + 63) / 64); }
+static int variant_of(const DevCfg& cfg) {
+  return (cfg.tile_variant >= 0 && cfg.tile_variant < kTileVariants) ? cfg.tile_variant : 0;
+}
+int tile_units(const DevCfg& cfg) { return units_for_rows(cfg, kVariantRows[variant_of(cfg)]); }
+int tile_units_max(const DevCfg& cfg) {
+  int m = 0;
+  for (int v = 0; v < kTileVariants; ++v) m = max(m, units_for_rows(cfg, kVariantRows[v]));
+  return m;
+}
+int tile_units_kln(const DevCfg& cfg) { return units_for_rows(cfg, kTileRows); }
 
 template <int TR, int CH, bool PAIR>
 static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
@@ -260,7 +279,7 @@ int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int 
   if (nev <= 0) return 0;
   if (!(cfg.tile_k > 0.f && cfg.tile_w > 0.f && cfg.tile_out > 0.f) || !table || tmax < 3 || !(dT > 0))
     return (int)hipErrorInvalidValue;
-  const dim3 grid((unsigned)tile_units(cfg) * (unsigned)((nev + 7) / 8 * 8));
+  const dim3 grid((unsigned)tile_units_kln(cfg) * (unsigned)((nev + 7) / 8 * 8));
   const KlnTab kt{table, tmax, dT};
   hipLaunchKernelGGL((k_tile<kTileRows, 8, true, true>), grid, dim3(64), 0, (hipStream_t)stream, cfg, b, s, 0, nev,
                      out, kt);
@@ -272,7 +291,12 @@ int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,
   if (nev <= 0) return 0;
   if (!(cfg.tile_k > 0.f && cfg.tile_w > 0.f && cfg.tile_out > 0.f)) return (int)hipErrorInvalidValue;
   hipStream_t st = (hipStream_t)stream;
-  launch_tr<kTileRows, 8, true>(cfg, b, s, nev, out0, out1, st);
+  switch (variant_of(cfg)) {
+    case 1: launch_tr<16, 8, true>(cfg, b, s, nev, out0, out1, st); break;
+    case 2: launch_tr<32, 4, true>(cfg, b, s, nev, out0, out1, st); break;
+    case 3: launch_tr<64, 8, false>(cfg, b, s, nev, out0, out1, st); break;
+    default: launch_tr<kTileRows, 8, true>(cfg, b, s, nev, out0, out1, st); break;
+  }
   if (s.combine == 1) return launch_combine_sqrt(out0, out1, (size_t)nev * cfg.nx * cfg.ny, stream);
   return (int)hipGetLastError();
 }
