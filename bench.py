@@ -155,6 +155,52 @@ def cpu_p3d_baseline(seed, nev=4):
     return nev / (time.perf_counter() - t)
 
 
+MIXED = [  # (name, Aproj, Atarg, nucleus keys): the mixed-system workload
+    ("p+Pb", 1, 208, dict(targ_ws_R=6.62, targ_ws_a=0.546)),
+    ("d+Au", 2, 197, dict(targ_ws_R=6.38, targ_ws_a=0.535)),
+    ("Cu+Cu", 63, 63, dict(proj_ws_R=4.2, proj_ws_a=0.596, targ_ws_R=4.2, targ_ws_a=0.596)),
+    ("Au+Au", 197, 197, dict(proj_ws_R=6.38, proj_ws_a=0.535, targ_ws_R=6.38, targ_ws_a=0.535)),
+    ("U+U", 238, 238, dict(proj_ws_R=6.81, proj_ws_a=0.6, targ_ws_R=6.81, targ_ws_a=0.6, proj_deformed=1,
+                           targ_deformed=1, proj_beta2=0.28, proj_beta4=0.093, targ_beta2=0.28, targ_beta4=0.093)),
+]
+
+
+def mixed_params(nev, A, B, nuclei):
+    from supermc_amd import default_params
+    return default_params(
+        which_mc_model=5, sub_model=1, shape_of_nucleons=4, gauss_nucl_width=0.5, shape_of_entropy=2,
+        collision_criterion=1, Aproj=A, Atarg=B, nucleon_min_dist=0.4, sigma_nn_mb=42.0, alpha=0.14,
+        cc_fluctuation_model=6, cc_fluctuation_Gamma_theta=1.25, bmin=0.0, bmax=20.0, Npmin=2, Npmax=500,
+        maxx=13.0, maxy=13.0, dx=0.05, dy=0.05, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev,
+        max_pairs=8192, **nuclei)
+
+
+def time_mixed(nev, steps, seed):
+    """The mixed-system workload: p+Pb, d+Au, Cu+Cu, Au+Au and deformed U+U
+    cycled event by event inside every batch (smc_set_systems), 521^2 cells
+    at 0.05 fm, exact-footprint smearing with the tile shape autotuned."""
+    from supermc_amd import Context
+    systems = [mixed_params(nev, A, B, k) for _, A, B, k in MIXED]
+    with Context(mixed_params(nev, 238, 238, MIXED[-1][3])) as ctx:
+        ctx.set_systems(systems, list(range(len(systems))))
+        ctx.run(seed, 0, nev, fetch=False)
+        ctx.synchronize()
+        best, ms = ctx.autotune(reps=1)
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ev = ctx.fetch_events(nev)
+        ker = {name: ctx.timing(kid)[0] / max(ctx.timing(kid)[1], 1)
+               for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments"))}
+    npart = {MIXED[s][0]: float(ev[ev[:, 58] == s, 54].mean()) for s in range(len(MIXED))}
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, grid=[521, 521],
+                systems=[m[0] for m in MIXED], mean_npart=npart, failed_events=int((ev[:, 57] != 0).sum()),
+                tile_variant=best, tile_ms=[round(float(m), 3) for m in ms], kernels_avg_ms=ker)
+
+
 def time_graph(seed):
     """Batches launched eagerly (smc_run) and replayed from a captured
     hipGraph (smc_run_graph), at 10^3 and 10^5 events per batch."""
@@ -261,7 +307,9 @@ def main():
     ap.add_argument("--p3d-events", type=int, default=16, help="3D profile events per call (0: skip)")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager vs hipGraph-replay comparison")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
-    ap.add_argument("--only", choices=("kln", "adjoint", "p3d"), default=None,
+    ap.add_argument("--mixed-events", type=int, default=50000,
+                    help="mixed-system 521^2 events per batch (0: skip)")
+    ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
@@ -282,9 +330,12 @@ def main():
         elif args.only == "adjoint":
             r = time_adjoint(args.adjoint_events, args.steps, args.seed)
             metric, wl = "events/s differentiable Pb+Pb 261^2 (forward + adjoint)", "frozen-topology Pb+Pb batch"
-        else:
+        elif args.only == "p3d":
             r = time_p3d(args.p3d_events, args.steps, args.seed)
             metric, wl = "events/s 3D profiles 101x261x261", "eta-sampled 3D Gaussians of Pb+Pb operation-2 lists"
+        else:
+            r = time_mixed(args.mixed_events, args.steps, args.seed)
+            metric, wl = "events/s mixed systems 521^2", "p+Pb, d+Au, Cu+Cu, Au+Au, U+U in one batch"
         print(json.dumps({"metric": metric, "value": r["events_per_s"], "unit": "events/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": 1, "higher_is_better": True, "dtype": "f32",
                           "data": "synthetic", "config": dict(workload=wl, **r)}), flush=True)
@@ -359,6 +410,7 @@ def main():
     adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
     p3d = time_p3d(args.p3d_events, 2, args.seed) if args.p3d_events > 0 else None
     graph = time_graph(args.seed) if not args.no_graph else None
+    mixed = time_mixed(args.mixed_events, 2, args.seed) if args.mixed_events > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -387,7 +439,7 @@ def main():
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
                    "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
-                   "graph_replay": graph, "tile_autotune": tune},
+                   "graph_replay": graph, "tile_autotune": tune, "mixed_systems_521": mixed},
         "roofline": roof,
         "cpu_baseline": cpu,
     }

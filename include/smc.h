@@ -120,6 +120,7 @@ typedef struct smc_derived {
 #define SMC_EV_NCOLL 55
 #define SMC_EV_B 56
 #define SMC_EV_STATUS 57
+#define SMC_EV_SYSTEM 58   /* index into smc_set_systems' list (0 when the batch is unmixed) */
 
 /* Fills p with the values of the reference's parameters.dat. */
 smc_status smc_params_default(smc_params* p);
@@ -254,6 +255,19 @@ smc_status smc_diff_forward(smc_ctx* ctx, const double* theta, double* out);
 smc_status smc_diff_backward(smc_ctx* ctx, const double* cot, double* grad);
 smc_status smc_diff_topology(smc_ctx* ctx, int ev, double* uvw_proj, double* uvw_targ, double* b,
                              double* gamma_v, int cap, int* nsrc);
+
+/* Collision systems mixed in one batch (no counterpart in the reference,
+ * which runs one system per job).  systems[i] is a full parameter set of
+ * which only the nuclei (Aproj, Atarg and the WS / deformation keys) may
+ * differ from the context's, with Aproj / Atarg at most the context's.
+ * Sampled event g (smc_sample / smc_run) uses systems[pattern[g %
+ * npattern]]; fed positions keep the context's system.  An event of system
+ * s is the event a context made for s alone gives for the same seed and
+ * index.  nsys = 0 restores the context's own system.  Records get the
+ * system in slot SMC_EV_SYSTEM; smc_get_nucleons returns the event's
+ * nucleons (entries past its counts: NaN, wound -2). */
+smc_status smc_set_systems(smc_ctx* ctx, int nsys, const smc_params* systems, int npattern, const int* pattern);
+smc_status smc_get_event_system(smc_ctx* ctx, int ev, int* system);  /* -1: unmixed batch */
 
 /* Tile shapes of the exact-footprint smearing kernel (no counterpart in
  * the reference): 0 = 32 x 64 tiles, 8-row chunks (default); 1 = 16 x 64;

@@ -29,6 +29,7 @@ MAX_A = 256
 EV_STRIDE = 64
 EV_EPS_RE, EV_EPS_IM, EV_EPSP_RE, EV_EPSP_IM, EV_RN = 0, 10, 20, 30, 40
 EV_XC, EV_YC, EV_TOTAL, EV_DSDY, EV_NPART, EV_NCOLL, EV_B, EV_STATUS = 50, 51, 52, 53, 54, 55, 56, 57
+EV_SYSTEM = 58
 
 K_EVENT, K_SMEAR, K_MOMENTS, K_KLN = 0, 1, 2, 3
 
@@ -115,6 +116,8 @@ SIGNATURES = {
     "smc_diff_forward": (C.c_int, [C.c_void_p, _dp, _dp]),
     "smc_diff_backward": (C.c_int, [C.c_void_p, _dp, _dp]),
     "smc_diff_topology": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp, C.c_int, _ip]),
+    "smc_set_systems": (C.c_int, [C.c_void_p, C.c_int, _P(SmcParams), C.c_int, _ip]),
+    "smc_get_event_system": (C.c_int, [C.c_void_p, C.c_int, _ip]),
     "smc_set_tile_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "smc_autotune": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
     "smc_set_timing": (C.c_int, [C.c_void_p, C.c_int]),
@@ -285,6 +288,21 @@ class Context:
         self._check(self.lib.smc_run(self.h, seed, event0, nev, _ptr(ev, C.c_double)))
         self.nev = nev
         return ev
+
+    def set_systems(self, systems, pattern):
+        """Mixed collision systems: systems is a list of parameter sets (only
+        their nuclei may differ from this context's); sampled event g uses
+        systems[pattern[g % len(pattern)]].  An empty list restores the
+        context's own system."""
+        arr = (SmcParams * max(1, len(systems)))(*systems) if systems else None
+        pat = np.ascontiguousarray(np.asarray(pattern, np.int32)) if systems else None
+        self._check(self.lib.smc_set_systems(self.h, len(systems), arr, len(pattern) if systems else 0,
+                                             _ptr(pat, C.c_int)))
+
+    def event_system(self, ev):
+        s = C.c_int()
+        self._check(self.lib.smc_get_event_system(self.h, ev, C.byref(s)))
+        return s.value
 
     def set_tile_variant(self, variant):
         self._check(self.lib.smc_set_tile_variant(self.h, variant))

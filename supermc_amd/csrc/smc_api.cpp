@@ -144,6 +144,13 @@ struct smc_ctx {
   uint2 graph_key{0, 0};
   unsigned long long graph_event0 = 0;
   int graph_nev = 0, graph_warm = 0, graph_cm_nu = 0;
+  // mixed collision systems (smc_set_systems)
+  std::vector<NucSys> mix_sys;  // 2 per system
+  std::vector<int> mix_of;      // system of event g: mix_of[g % size]
+  NucSys* d_mix_sys = nullptr;
+  int* d_mix_of = nullptr;
+  unsigned long long batch_event0 = 0;
+  int batch_mode = 0;
   int cap = 0;        // max events
   int nev = 0;        // events currently held
   int have_events = 0, have_density = 0, have_thickness = 0, have_moments = 0;
@@ -684,7 +691,21 @@ void run_moments(smc_ctx* c, int mode) {
   ts.done();
 }
 
+// system of event ev of the current batch, -1 when the batch is unmixed
+static int event_system(const smc_ctx* c, int ev) {
+  if (c->mix_of.empty() || c->batch_mode == 1) return -1;
+  return c->mix_of[(c->batch_event0 + (unsigned long long)ev) % c->mix_of.size()];
+}
+
+// record slot SMC_EV_SYSTEM of copied-out records
+static void tag_systems(const smc_ctx* c, double* ev_out, int nev) {
+  if (!ev_out || event_system(c, 0) < 0) return;
+  for (int e = 0; e < nev; ++e) ev_out[(size_t)e * SMC_EV_STRIDE + SMC_EV_SYSTEM] = event_system(c, e);
+}
+
 void run_event(smc_ctx* c, int nev, uint64_t seed, uint64_t event0, int fed) {
+  c->batch_event0 = event0;
+  c->batch_mode = fed;
   TimerScope ts(c, SMC_K_EVENT);
   check_launch(launch_event(c->cfg, c->buf, nev, seed, event0, fed, c->stream), "k_event");
   ts.done();
@@ -1049,6 +1070,8 @@ void smc_destroy(smc_ctx* c) {
   if (!c) return;
   if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
   if (c->graph) (void)hipGraphDestroy(c->graph);
+  if (c->d_mix_sys) (void)hipFree(c->d_mix_sys);
+  if (c->d_mix_of) (void)hipFree(c->d_mix_of);
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
@@ -1156,16 +1179,24 @@ smc_status smc_get_nucleons(smc_ctx* c, int ev, double* xy_proj, int* wound_proj
     HIPCHK(hipMemcpyAsync(wd.data(), c->buf.wound + base, wd.size() * sizeof(int), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(fl.data(), c->buf.fluct + base, fl.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
     sync(c);
+    // a mixed batch's event has its system's nucleon counts (at most the
+    // context's); the entries past them read NaN / -2 / 0
+    const int sys = event_system(c, ev);
     for (int nu = 0; nu < 2; ++nu) {
-      const int n = nu ? c->p.Atarg : c->p.Aproj;
+      const int cap = nu ? c->p.Atarg : c->p.Aproj;
+      const int n = sys >= 0 ? c->mix_sys[2 * sys + nu].A : cap;
       double* xy = nu ? xy_targ : xy_proj;
       int* w = nu ? wound_targ : wound_proj;
       float* f = nu ? fluct_targ : fluct_proj;
-      for (int i = 0; i < n; ++i) {
+      for (int i = 0; i < cap; ++i) {
         const size_t k = (size_t)nu * kAMax + i;
-        if (xy) { xy[2 * i] = pos[2 * k] * kQ; xy[2 * i + 1] = pos[2 * k + 1] * kQ; }
-        if (w) w[i] = wd[k];
-        if (f) f[i] = fl[k];
+        const bool in = i < n;
+        if (xy) {
+          xy[2 * i] = in ? pos[2 * k] * kQ : std::nan("");
+          xy[2 * i + 1] = in ? pos[2 * k + 1] * kQ : std::nan("");
+        }
+        if (w) w[i] = in ? wd[k] : -2;
+        if (f) f[i] = in ? fl[k] : 0.f;
       }
     }
     return SMC_OK;
@@ -1250,6 +1281,7 @@ smc_status smc_moments(smc_ctx* c, double* ev_out) {
       HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)c->nev * SMC_EV_STRIDE * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
     sync(c);
+    tag_systems(c, ev_out, c->nev);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }
@@ -1303,6 +1335,7 @@ smc_status smc_run(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* 
       HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
       sync(c);
+      tag_systems(c, ev_out, nev);
     }
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
@@ -1389,6 +1422,8 @@ smc_status smc_run_graph(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, do
       c->graph_event0 = event0;
       HIPCHK(hipGraphExecKernelNodeSetParams(c->graph_exec, c->graph_event_node, &c->graph_event_params));
       HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
+      c->batch_event0 = event0;
+      c->batch_mode = 0;
       c->nev = nev;
       c->have_events = c->have_density = 1;
       c->have_thickness = 0;
@@ -1399,6 +1434,7 @@ smc_status smc_run_graph(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, do
       HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double),
                             hipMemcpyDeviceToHost, c->stream));
       sync(c);
+      tag_systems(c, ev_out, nev);
     }
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
@@ -1412,6 +1448,8 @@ smc_status smc_set_tile_variant(smc_ctx* c, int variant) {
   }
   c->cfg.tile_variant = variant;
   c->have_density = c->have_moments = 0;  // the centre-of-mass partials follow the tile shape
+  graph_release(c);
+  c->graph_warm = 0;
   return SMC_OK;
 }
 
@@ -1450,11 +1488,86 @@ smc_status smc_autotune(smc_ctx* c, int reps, int* best, double* ms) {
     (void)hipEventDestroy(b);
     c->cfg.tile_variant = pick;
     c->have_density = c->have_thickness = c->have_moments = 0;
+    graph_release(c);
+    c->graph_warm = 0;
     if (best) *best = pick;
     if (ms)
       for (int v = 0; v < kTileVariants; ++v) ms[v] = t[v];
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
+}
+
+// Per-event collision systems in one batch (config "mixed systems"): each
+// entry of systems is a full parameter set of which only the nuclei (Aproj,
+// Atarg, WS and deformation keys) may differ from the context's; sampled
+// event g uses systems[pattern[g % npattern]].  nsys = 0 restores the
+// context's own system.
+smc_status smc_set_systems(smc_ctx* c, int nsys, const smc_params* systems, int npattern, const int* pattern) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nsys < 0 || nsys > 64) throw SmcError(SMC_ERR_ARG, "nsys must be in [0, 64]");
+    std::vector<NucSys> sys;
+    std::vector<int> of;
+    if (nsys > 0) {
+      if (!systems || !pattern || npattern < 1) throw SmcError(SMC_ERR_ARG, "systems and a pattern are required");
+      for (int i = 0; i < nsys; ++i) {
+        smc_ctx t;  // host-side derivation only
+        t.p = systems[i];
+        t.cap = 1;
+        try {
+          validate_and_derive(&t);
+        } catch (const SmcError& e) {
+          throw SmcError(e.code, "system " + std::to_string(i) + ": " + e.what());
+        }
+        const smc_derived &a = t.d, &b = c->d;
+        const smc_params &pa = t.p, &pb = c->p;
+        const bool same = a.siginNN == b.siginNN && a.width == b.width && a.dsq == b.dsq && a.dmin == b.dmin &&
+                          a.nx == b.nx && a.ny == b.ny && pa.dx == pb.dx && pa.dy == pb.dy && pa.maxx == pb.maxx &&
+                          pa.maxy == pb.maxy && pa.which_mc_model == pb.which_mc_model && pa.alpha == pb.alpha &&
+                          pa.cc_fluctuation_model == pb.cc_fluctuation_model &&
+                          pa.cc_fluctuation_Gamma_theta == pb.cc_fluctuation_Gamma_theta && pa.bmin == pb.bmin &&
+                          pa.bmax == pb.bmax && pa.Npmin == pb.Npmin && pa.Npmax == pb.Npmax;
+        if (!same) throw SmcError(SMC_ERR_ARG, "system " + std::to_string(i) + " differs beyond its nuclei");
+        if (pa.Aproj > pb.Aproj || pa.Atarg > pb.Atarg)
+          throw SmcError(SMC_ERR_CAPACITY, "system " + std::to_string(i) + ": nuclei larger than the context's");
+        sys.push_back(t.cfg.nuc[0]);
+        sys.push_back(t.cfg.nuc[1]);
+      }
+      for (int k = 0; k < npattern; ++k) {
+        if (pattern[k] < 0 || pattern[k] >= nsys) throw SmcError(SMC_ERR_ARG, "pattern entry outside [0, nsys)");
+        of.push_back(pattern[k]);
+      }
+    }
+    graph_release(c);
+    c->graph_warm = 0;
+    if (c->d_mix_sys) (void)hipFree(c->d_mix_sys);
+    if (c->d_mix_of) (void)hipFree(c->d_mix_of);
+    c->d_mix_sys = nullptr;
+    c->d_mix_of = nullptr;
+    c->cfg.sys = nullptr;
+    c->cfg.sys_of = nullptr;
+    c->cfg.sys_n = 0;
+    if (nsys > 0) {
+      c->d_mix_sys = dalloc<NucSys>(sys.size());
+      c->d_mix_of = dalloc<int>(of.size());
+      HIPCHK(hipMemcpy(c->d_mix_sys, sys.data(), sys.size() * sizeof(NucSys), hipMemcpyHostToDevice));
+      HIPCHK(hipMemcpy(c->d_mix_of, of.data(), of.size() * sizeof(int), hipMemcpyHostToDevice));
+      c->cfg.sys = c->d_mix_sys;
+      c->cfg.sys_of = c->d_mix_of;
+      c->cfg.sys_n = (int)of.size();
+    }
+    c->mix_sys = sys;
+    c->mix_of = of;
+    c->have_events = c->have_density = c->have_thickness = c->have_moments = 0;
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_get_event_system(smc_ctx* c, int ev, int* system) {
+  if (!c || !system) return SMC_ERR_ARG;
+  if (!c->have_events || ev < 0 || ev >= c->nev) return SMC_ERR_ARG;
+  *system = event_system(c, ev);
+  return SMC_OK;
 }
 
 smc_status smc_fetch_events(smc_ctx* c, int nev, double* ev_out) {
@@ -1464,6 +1577,7 @@ smc_status smc_fetch_events(smc_ctx* c, int nev, double* ev_out) {
     HIPCHK(hipMemcpyAsync(ev_out, c->evout, (size_t)nev * SMC_EV_STRIDE * sizeof(double), hipMemcpyDeviceToHost,
                           c->stream));
     sync(c);
+    tag_systems(c, ev_out, nev);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }

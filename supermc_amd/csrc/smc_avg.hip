@@ -165,7 +165,7 @@ __global__ __launch_bounds__(256) void k_spectator_src(DevCfg cfg, EvBufs buf, E
     const int A = cfg.nuc[nu].A;
     for (int i0 = 0; i0 < A; i0 += 256) {
       const int i = i0 + t;
-      const bool spec = i < A && wound[nu * kAMax + i] < 0;
+      const bool spec = i < A && wound[nu * kAMax + i] == -1;  // -2: no nucleon in this slot
       const unsigned long long m = __ballot(spec);
       __syncthreads();
       if (lane == 0) s_cnt[wv] = __popcll(m);

@@ -71,6 +71,12 @@ struct DevCfg {
   // output scale 1 / (K tile_w)
   float tile_k, tile_w, tile_out;
   int tile_variant;      // k_tile shape, 0 .. kTileVariants - 1 (smc_tile.hip)
+  // collision systems mixed in one batch (smc_set_systems): sampled event g
+  // uses nuclei sys[2 s], sys[2 s + 1] with s = sys_of[g % sys_n]; sys_n = 0:
+  // every event uses nuc[]
+  const NucSys* sys;
+  const int* sys_of;
+  int sys_n;
   int hb_h;              // box half-width 4*max(w,qw) in half-quanta (rounded)
   float alpha, pref_part;
   int use_binary;        // alpha > 1e-8 (MCnucl.cpp:724)

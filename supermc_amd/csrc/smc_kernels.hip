@@ -46,8 +46,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   __shared__ double s_b;
   __shared__ int s_xc;
   __shared__ double s_red[2][2];
+  __shared__ NucSys s_nuc[2];  // this event's nuclei
 
-  const int A = cfg.nuc[0].A, B = cfg.nuc[1].A;
+  // sampled events of a mixed batch take their system's nuclei; fed
+  // positions keep the context's
+  if (t < 2) {
+    if (cfg.sys_n > 0 && !fed) s_nuc[t] = cfg.sys[2 * cfg.sys_of[gev % (unsigned long long)cfg.sys_n] + t];
+    else s_nuc[t] = t ? cfg.nuc[1] : cfg.nuc[0];
+  }
+  __syncthreads();
+  const int A = s_nuc[0].A, B = s_nuc[1].A;
   const int NN[2] = {A, B};
   int attempt = 0, status = kStOk;
   int npa = 0, npb = 0, ncoll = 0;
@@ -84,7 +92,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       }
       // proton / deuteron fill directly (Nucleus.cpp:199-210, 342-359)
       if (wv < 2) {
-        const NucSys& ns = cfg.nuc[wv];
+        const NucSys ns = s_nuc[wv];
         const int xcen = (wv == 0) ? s_xc : -s_xc;
         if (ns.kind == kProton) {
           if (lane == 0) { px[wv][0] = xcen; py[wv][0] = 0; pz[wv][0] = 0.f; s_acc[wv] = 1; }
@@ -114,12 +122,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       // sequential acceptance among the round's candidates in list order.
       const int nu = wv & 1, sub = wv >> 1;
       for (unsigned round = 0;; ++round) {
-        const bool need0 = cfg.nuc[0].kind == kWoodsSaxon && s_acc[0] < A;
-        const bool need1 = cfg.nuc[1].kind == kWoodsSaxon && s_acc[1] < B;
+        const bool need0 = s_nuc[0].kind == kWoodsSaxon && s_acc[0] < A;
+        const bool need1 = s_nuc[1].kind == kWoodsSaxon && s_acc[1] < B;
         if (!need0 && !need1) break;
         if (round > 4096) { status |= kStNoEvent; break; }
         const bool need = nu ? need1 : need0;
-        const NucSys& ns = cfg.nuc[nu];
+        const NucSys ns = s_nuc[nu];
         int nc[2] = {0, 0};  // candidates so far (uniform over the workgroup)
         // candidates wanted this round: the nucleons still missing plus a
         // margin for the hard core, at most 64
@@ -253,7 +261,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       }
       // CM recentering (Nucleus.cpp:296-309) in FP64, shift rounded to the
       // quantum, then the nucleus centre +-b/2.
-      if (wv < 2 && cfg.nuc[wv].kind == kWoodsSaxon) {
+      if (wv < 2 && s_nuc[wv].kind == kWoodsSaxon) {
         const int An = NN[wv];
         double sxm = 0.0, sym = 0.0;
         for (int i = lane; i < An; i += 64) { sxm += (double)px[wv][i]; sym += (double)py[wv][i]; }
@@ -371,17 +379,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     if (woundA >= 0) fA = gamma_draw(cfg.gk_part, cfg.gs_part, key, gev, kStrFluctA, (unsigned)t);
     if (woundB >= 0) fB = gamma_draw(cfg.gk_part, cfg.gs_part, key, gev, kStrFluctB, (unsigned)t);
   }
+  // slots past this event's nucleus sizes read -2 (no nucleon; -1 is a
+  // nucleon that was not hit): readers bound their scans by the context's
+  // sizes, which a mixed batch's event may not fill
+  static_assert(kAMax <= 256, "one slot per thread");
   if (t < A) {
     buf.pos[(nbase + t) * 2] = sx[0][t];
     buf.pos[(nbase + t) * 2 + 1] = sy[0][t];
     buf.wound[nbase + t] = woundA;
     buf.fluct[nbase + t] = fA;
+  } else if (t < kAMax) {
+    buf.wound[nbase + t] = -2;
   }
   if (t < B) {
     buf.pos[(nbase + kAMax + t) * 2] = sx[1][t];
     buf.pos[(nbase + kAMax + t) * 2 + 1] = sy[1][t];
     buf.wound[nbase + kAMax + t] = woundB;
     buf.fluct[nbase + kAMax + t] = fB;
+  } else if (t < kAMax) {
+    buf.wound[nbase + kAMax + t] = -2;
   }
   const int nsrc_part = npa + npb;
   int4* src = buf.src + (size_t)e * buf.src_cap;

@@ -168,3 +168,51 @@ def test_proton_sits_at_the_centre():
             b = ev[e, 56]
             assert np.allclose(nuc["xy_proj"][0], [b / 2, 0.0], atol=1e-6)
             assert (nuc["wound_proj"] >= 0).sum() == 1  # accepted events have a collision
+
+
+def test_mixed_systems_in_one_batch_equal_single_system_runs():
+    """One batch cycling p+Pb, d+Au, Cu+Cu, U+U and Au+Au (common sigma_NN,
+    grid and fluctuations): every event equals the event a context made
+    for its system alone gives for the same seed and index, record for
+    record and nucleon for nucleon."""
+    from supermc_amd import Context, SmcError
+
+    def prm(name, **kw):
+        A, B, over = SYSTEMS["AuAu521" if name == "AuAu" else name]
+        over = {k: v for k, v in over.items() if k not in ("dx", "dy")}
+        over["sigma_nn_mb"] = 42.0
+        return _params(name if name != "AuAu" else "AuAu521", **{**over, **kw, "dx": 0.1, "dy": 0.1})
+
+    names = ["pPb", "dAu", "CuCu", "UU", "AuAu"]
+    systems = [prm(n, max_events=30) for n in names]
+    pattern = [0, 1, 2, 3, 4, 3, 2]
+    seed, ev0, nev = 77, 100, 30
+    with Context(prm("UU", max_events=30)) as mix:
+        mix.set_systems(systems, pattern)
+        rec = mix.run(seed=seed, event0=ev0, nev=nev)
+        assert (rec[:, 57] == 0).all()
+        sys_of = [pattern[(ev0 + e) % len(pattern)] for e in range(nev)]
+        assert [int(r) for r in rec[:, 58]] == sys_of
+        assert [mix.event_system(e) for e in range(nev)] == sys_of
+        mix_nuc = [mix.nucleons(e) for e in range(nev)]
+        for s, p in enumerate(systems):
+            with Context(p) as one:
+                r1 = one.run(seed=seed, event0=ev0, nev=nev)
+                for e in (e for e in range(nev) if sys_of[e] == s):
+                    assert np.array_equal(rec[e, :58], r1[e, :58]), (names[s], e)
+                    n1 = one.nucleons(e)
+                    for side, A in (("proj", p.Aproj), ("targ", p.Atarg)):
+                        assert np.array_equal(mix_nuc[e]["xy_" + side][:A], n1["xy_" + side])
+                        assert np.isnan(mix_nuc[e]["xy_" + side][A:]).all()
+                        assert np.array_equal(mix_nuc[e]["wound_" + side][:A], n1["wound_" + side])
+        # the context's own system again
+        mix.set_systems([], [])
+        with Context(prm("UU", max_events=30)) as uu:
+            assert np.array_equal(mix.run(seed=seed, event0=ev0, nev=nev)[:, :58],
+                                  uu.run(seed=seed, event0=ev0, nev=nev)[:, :58])
+        # systems must share everything but their nuclei and fit the context
+        with pytest.raises(SmcError):
+            mix.set_systems([prm("CuCu", sigma_nn_mb=64.0)], [0])
+    with Context(prm("CuCu")) as small:
+        with pytest.raises(SmcError):
+            small.set_systems([prm("UU")], [0])
