@@ -382,6 +382,8 @@ def rotate(cth, phi, xyz):
 
 
 def run_events(cfg, seed, nev, want_eps=False):
-    eps = np.zeros(2 * nev) if want_eps else None
+    """MakeDensity.cpp:2143-2224 event loop; with want_eps, per accepted event
+    (Re eps_2, Im eps_2 (r^2 weight), b, Npart, Ncoll)."""
+    eps = np.zeros(5 * nev) if want_eps else None
     n = lib().orc_run_events(C.byref(cfg), seed, nev, _p(eps) if eps is not None else None)
-    return n, (eps.reshape(nev, 2) if eps is not None else None)
+    return n, (eps.reshape(nev, 5) if eps is not None else None)

@@ -843,7 +843,14 @@ int orc_run_events(const orc_event_cfg* c, uint64_t seed, int nev, double* eps2_
                         ncoll, cx, cy, cf, s);
     double mr[10], mi[10], mpr[10], mpi[10], rn[10], cm[3];
     orc_moments(&g, s, npa + npb, boxes, ncoll, c->from_order, c->to_order, mr, mi, mpr, mpi, rn, cm);
-    if (eps2_out) { eps2_out[2 * done] = mr[2]; eps2_out[2 * done + 1] = mi[2]; }
+    if (eps2_out) {
+      double* o = eps2_out + 5 * (size_t)done;
+      o[0] = mr[2];
+      o[1] = mi[2];
+      o[2] = b;
+      o[3] = npa + npb;
+      o[4] = ncoll;
+    }
     done++;
   }
   free(s); free(TA); free(TB); free(xa); free(ya); free(xb); free(yb);

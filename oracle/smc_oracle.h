@@ -157,8 +157,9 @@ typedef struct {
   double maxx, dx;
   int from_order, to_order;
 } orc_event_cfg;
-int orc_run_events(const orc_event_cfg* c, uint64_t seed, int nev,
-                   double* eps2_out /* nev x 2, may be NULL */);
+/* eps2_out (may be NULL): nev x 5 per accepted event: Re eps_2, Im eps_2
+ * (r^2 weight), b, Npart, Ncoll */
+int orc_run_events(const orc_event_cfg* c, uint64_t seed, int nev, double* eps2_out);
 
 /* ---- 3D profiles, scripts/generate_3d_profiles/profile_3d.cpp ---- */
 /* profile_3d.cpp:24-27 and 133-155: y_beam, sigma_inelastic [fm^2] and the
