@@ -19,7 +19,7 @@
 namespace smc {
 
 // ------------------------------------------------------------- k_event
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
                                                unsigned long long event0, int mode) {
   // mode 0: impact parameter sampled, events accepted as MakeDensity.cpp:2147-2162
   // mode 1: fed positions; mode 2: sampled nuclei at the given b, one attempt
