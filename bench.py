@@ -405,6 +405,13 @@ def main():
     roof["traffic"] = traffic
 
     eps_err = eps_error_vs_fp64(ctx, ev, args.parity_events) if args.parity_events > 0 else None
+    # further batches with the event records copied to host memory (reported
+    # beside `value`, which leaves them in device memory)
+    ctx.set_timing(False)
+    th = time.perf_counter()
+    for k in range(2):
+        ctx.run(args.seed, (args.warmup + args.steps + k) * nev, nev, fetch=True)
+    host_rate = 2 * nev / (time.perf_counter() - th)
 
     kln = time_kln(args.kln_events, 2, args.seed) if args.kln_events > 0 else None
     adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
@@ -439,7 +446,8 @@ def main():
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
                    "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
-                   "graph_replay": graph, "tile_autotune": tune, "mixed_systems_521": mixed},
+                   "graph_replay": graph, "tile_autotune": tune, "mixed_systems_521": mixed,
+                   "records_to_host_events_per_s": host_rate},
         "roofline": roof,
         "cpu_baseline": cpu,
     }
