@@ -201,6 +201,101 @@ def time_mixed(nev, steps, seed):
                 tile_variant=best, tile_ms=[round(float(m), 3) for m in ms], kernels_avg_ms=ker)
 
 
+def time_avg(nev, steps, seed):
+    """Operation 3 (MakeDensity::generate_profile_average, pT-integrated) on
+    the main Pb+Pb workload: per batch the events (smc_run), then
+    smc_average_profiles for orders 2..5 with the reaction-plane sums --
+    per order and event two recentrings (one also turned by the order-n
+    angle) and two re-smearings, summed over the batch into FP64 grids on
+    the device; the 8 summed grids come back to the host."""
+    from supermc_amd import Context
+    with Context(bench_params(nev)) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)
+        ctx.average_profiles(2, 5, reaction_plane=True)  # warm-up (allocates the sums)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
+            sums, _ = ctx.average_profiles(2, 5, reaction_plane=True)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ms, n = ctx.timing(1)
+        ctx.set_timing(False)
+        # the sources are left turned: the last batch again (same counters,
+        # same events) for its acceptance
+        ev = ctx.run(seed, steps * nev, nev)
+    acc = int((ev[:, 57] == 0).sum())
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, orders=[2, 5], reaction_plane=True,
+                smearings_per_event=1 + 2 * 4, k_smear_avg_ms=ms / max(n, 1), k_smear_launches=n,
+                accepted_last_batch=acc, mean_dsdy_order2=float(sums[0].sum() * 0.01 / max(acc, 1)))
+
+
+def time_jet(nev, steps, seed):
+    """Operation 2 grids (MakeDensity::generate_profile_ebe_Jet) on the main
+    Pb+Pb workload: per batch the events (smc_run), the binary-collision
+    density (smc_rho_binary) and the projectile and target spectator
+    densities (smc_spectator_density), all three grids left in device
+    memory."""
+    from supermc_amd import Context
+    with Context(bench_params(nev)) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)
+        ctx.rho_binary(host=False)
+        ctx.spectator_density(host=False)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
+            ctx.rho_binary(host=False)
+            ctx.spectator_density(host=False)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ms, n = ctx.timing(1)
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps,
+                grids_per_event=["density", "rho_binary", "spectators A", "spectators B"],
+                k_smear_avg_ms=ms / max(n, 1), k_smear_launches=n)
+
+
+def cpu_ops_baseline(seed, nev=3):
+    """The oracle's FP64 restatements of one operation-3 event (orders 2..5,
+    with the reaction-plane profiles) and one event's operation-2 grids
+    (collision term of the pairs, spectator thicknesses), on the first nev
+    events of a device batch, one core.  Returns (op3, op2) events/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    import parity_helpers as H
+    from supermc_amd import Context
+    p = bench_params(nev)
+    g = oracle.grid(p.maxx, p.dx, p.maxy, p.dy)
+    inputs = []
+    with Context(p) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)
+        w = ctx.derived.width
+        for e in range(nev):
+            o = H.oracle_event(ctx, e)
+            nuc = o["nuc"]
+            pa, pb, pf = o["gpu_pairs"]
+            xa, xb, ia, ib = nuc["xy_proj"], nuc["xy_targ"], o["ia"], o["ib"]
+            inputs.append((xa[ia], nuc["fluct_proj"][ia].astype(float), xb[ib], nuc["fluct_targ"][ib].astype(float),
+                           (xa[pa] + xb[pb]) / 2.0, pf.astype(float), xa[nuc["wound_proj"] == -1],
+                           xb[nuc["wound_targ"] == -1]))
+    t = time.perf_counter()
+    for a, fa, b, fb, mid, fm, _, _ in inputs:
+        oracle.rotated_profiles(g, w, p.alpha, a, fa, b, fb, mid, fm, 2, 5, p.finalFactor, reaction_plane=True)
+    op3 = nev / (time.perf_counter() - t)
+    no = np.zeros((0, 2))
+    t = time.perf_counter()
+    for a, fa, b, fb, mid, fm, sa, sb in inputs:
+        oracle.density_glauber(g, w, p.alpha, a, fa, b, fb, mid, fm)
+        oracle.density_glauber(g, w, 1.0, no, np.zeros(0), no, np.zeros(0), mid, np.ones(len(mid)))
+        oracle.thickness(g, w, sa)
+        oracle.thickness(g, w, sb)
+    op2 = nev / (time.perf_counter() - t)
+    return op3, op2
+
+
 def time_graph(seed):
     """Batches launched eagerly (smc_run) and replayed from a captured
     hipGraph (smc_run_graph), at 10^3 and 10^5 events per batch."""
@@ -309,7 +404,10 @@ def main():
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
     ap.add_argument("--mixed-events", type=int, default=50000,
                     help="mixed-system 521^2 events per batch (0: skip)")
-    ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed"), default=None,
+    ap.add_argument("--avg-events", type=int, default=20000,
+                    help="operation-3 averaged-profile events per batch (0: skip)")
+    ap.add_argument("--jet-events", type=int, default=20000, help="operation-2 grid events per batch (0: skip)")
+    ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed", "avg", "jet"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
@@ -333,6 +431,12 @@ def main():
         elif args.only == "p3d":
             r = time_p3d(args.p3d_events, args.steps, args.seed)
             metric, wl = "events/s 3D profiles 101x261x261", "eta-sampled 3D Gaussians of Pb+Pb operation-2 lists"
+        elif args.only == "avg":
+            r = time_avg(args.avg_events, args.steps, args.seed)
+            metric, wl = "events/s averaged rotated profiles Pb+Pb 261^2", "operation 3, orders 2..5, reaction plane"
+        elif args.only == "jet":
+            r = time_jet(args.jet_events, args.steps, args.seed)
+            metric, wl = "events/s operation-2 grids Pb+Pb 261^2", "rho_binary and spectator densities"
         else:
             r = time_mixed(args.mixed_events, args.steps, args.seed)
             metric, wl = "events/s mixed systems 521^2", "p+Pb, d+Au, Cu+Cu, Au+Au, U+U in one batch"
@@ -418,12 +522,20 @@ def main():
     p3d = time_p3d(args.p3d_events, 2, args.seed) if args.p3d_events > 0 else None
     graph = time_graph(args.seed) if not args.no_graph else None
     mixed = time_mixed(args.mixed_events, 2, args.seed) if args.mixed_events > 0 else None
+    avg = time_avg(args.avg_events, 2, args.seed) if args.avg_events > 0 else None
+    jet = time_jet(args.jet_events, 2, args.seed) if args.jet_events > 0 else None
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
         if p3d is not None:
             p3d["cpu_oracle_events_per_s_1core"] = cpu_p3d_baseline(args.seed)
+        if avg is not None or jet is not None:
+            op3, op2 = cpu_ops_baseline(args.seed)
+            if avg is not None:
+                avg["cpu_oracle_events_per_s_1core"] = op3
+            if jet is not None:
+                jet["cpu_oracle_events_per_s_1core"] = op2
 
     line = {
         "metric": "events/s Pb+Pb 261^2 grid (Glauber, KLN); fwd+adjoint; eps_n err vs FP64",
@@ -447,6 +559,7 @@ def main():
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
                    "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
                    "graph_replay": graph, "tile_autotune": tune, "mixed_systems_521": mixed,
+                   "averaged_profiles_op3": avg, "jet_grids_op2": jet,
                    "records_to_host_events_per_s": host_rate},
         "roofline": roof,
         "cpu_baseline": cpu,

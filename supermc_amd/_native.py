@@ -327,14 +327,19 @@ class Context:
         self._check(self.lib.smc_fetch_events(self.h, nev, _ptr(ev, C.c_double)))
         return ev
 
-    def rho_binary(self):
-        rho = np.zeros((self.nev, self.nx, self.ny), np.float32)
+    def rho_binary(self, host=True):
+        """The binary-collision grids [nev][nx][ny]; host=False leaves them
+        in device memory and returns None."""
+        rho = np.zeros((self.nev, self.nx, self.ny), np.float32) if host else None
         self._check(self.lib.smc_rho_binary(self.h, _ptr(rho, C.c_float)))
         return rho
 
-    def spectator_density(self):
-        a = np.zeros((self.nev, self.nx, self.ny), np.float32)
-        b = np.zeros((self.nev, self.nx, self.ny), np.float32)
+    def spectator_density(self, host=True):
+        """Projectile and target spectator grids; host=False leaves them in
+        device memory and returns (None, None)."""
+        shape = (self.nev, self.nx, self.ny)
+        a = np.zeros(shape, np.float32) if host else None
+        b = np.zeros(shape, np.float32) if host else None
         self._check(self.lib.smc_spectator_density(self.h, _ptr(a, C.c_float), _ptr(b, C.c_float)))
         return a, b
 

@@ -172,6 +172,7 @@ struct smc_ctx {
   double* avg_rot = nullptr;          // averaged profiles: per event {xc, yc, angle, total}
   unsigned char* avg_acc = nullptr;   // per event: included in the average
   double* avg_sum = nullptr;          // [2][avg_nord][nx][ny] sums (rotated, recentred)
+  double* avg_part = nullptr;         // [kAccumSegments][nx][ny] partial sums over event segments
   int avg_nord = 0;
   int* nmrect = nullptr;    // [cap]
   double* kln_table = nullptr;
@@ -775,6 +776,7 @@ void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept,
   if (!c->avg_rot) {
     c->avg_rot = dalloc<double>((size_t)c->cap * 4);
     c->avg_acc = dalloc<unsigned char>((size_t)c->cap);
+    c->avg_part = dalloc<double>((size_t)kAccumSegments * N);
   }
   if (c->avg_nord < nord) {
     if (c->avg_sum) HIPCHK(hipFree(c->avg_sum));
@@ -804,7 +806,9 @@ void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept,
       c->have_density = c->have_thickness = c->have_moments = 0;
       run_density(c);
       double* dst = c->avg_sum + ((size_t)(pass == 0 ? nord : 0) + o) * N;
-      check_launch(launch_accum_avg(c->grid, N, nev, c->avg_acc, ff, dst, c->stream), "k_accum_avg");
+      check_launch(launch_accum_avg(c->grid, c->buf.counts, c->d.nx, c->d.ny, nev, c->avg_acc, ff, c->avg_part, dst,
+                                    c->stream),
+                   "k_accum_avg");
     }
   }
   std::vector<double> h(2 * (size_t)nord * N);
@@ -1075,8 +1079,8 @@ void smc_destroy(smc_ctx* c) {
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
-                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->aux0, c->aux1, c->spec_src, c->spec_swin,
-                  c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
+                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->avg_part, c->aux0,
+                  c->aux1, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (void* q : c->dif_alloc)

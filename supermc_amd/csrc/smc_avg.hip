@@ -14,8 +14,10 @@
 //                 MathBasics.cpp:41-50), back to half-quanta; footprints and
 //                 the source rectangle are recomputed from the new
 //                 positions (Particle::calculateBounds);
-//   k_accum_avg   sum over the batch's accepted events of finalFactor * rho
-//                 into an FP64 grid, one thread per cell, events in order.
+//   k_accum_part, k_accum_fin
+//                 sum over the batch's accepted events of finalFactor * rho
+//                 into an FP64 grid: per (cell, segment of events) partials
+//                 in event order, then the segments in order.
 #include "smc_device.h"
 
 namespace smc {
@@ -137,14 +139,56 @@ __global__ __launch_bounds__(256) void k_rotate_src(DevCfg cfg, EvBufs buf, cons
   }
 }
 
-__global__ __launch_bounds__(256) void k_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept,
-                                                   double ff, double* sum) {
-  const size_t c = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+// Pass 1 of the sum: thread (cell, segment) adds the cell over the events
+// of its segment in order, skipping excluded events and events whose source
+// rectangle misses the cell (the smearing writes exact zeros there).  The
+// rectangles of 256 events at a time are staged in LDS (an excluded event
+// gets an empty one), and eight loads are issued before their adds, so each
+// wave keeps eight in flight.
+__global__ __launch_bounds__(256) void k_accum_part(const float* grid, const int* counts, int nx, int ny, int nev,
+                                                    int per_seg, const unsigned char* accept, double* part) {
+  __shared__ int2 s_rect[256];
+  const int ncell = nx * ny, t = threadIdx.x;
+  const int c = blockIdx.x * 256 + t, seg = blockIdx.y;
+  const bool on = c < ncell;
+  const int ix = on ? c / ny : 0, iy = on ? c - ix * ny : 0;
+  const int e0 = seg * per_seg, e1 = min(nev, e0 + per_seg);
+  const float* g = grid + (on ? c : 0);
+  auto cell = [&](int e, int k) -> float {
+    const int2 r = s_rect[k];
+    const bool in = ix >= (r.x & 0xffff) && ix < (r.x >> 16) && iy >= (r.y & 0xffff) && iy < (r.y >> 16);
+    return in ? g[(size_t)e * ncell] : 0.f;
+  };
+  double s = 0.0;
+  for (int base = e0; base < e1; base += 256) {
+    const int n = min(256, e1 - base);
+    __syncthreads();  // s_rect free
+    if (t < n) {
+      const int e = base + t;
+      s_rect[t] = accept[e] ? make_int2(counts[(size_t)e * 8 + 6], counts[(size_t)e * 8 + 7]) : make_int2(0, 0);
+    }
+    __syncthreads();
+    if (!on) continue;
+    int k = 0;
+    for (; k + 8 <= n; k += 8) {
+      float v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = cell(base + k + j, k + j);
+#pragma unroll
+      for (int j = 0; j < 8; ++j) s += (double)v[j];
+    }
+    for (; k < n; ++k) s += (double)cell(base + k, k);
+  }
+  if (on) part[(size_t)seg * ncell + c] = s;
+}
+
+// Pass 2: sum[c] += ff * (segment partials added in segment order).
+__global__ __launch_bounds__(256) void k_accum_fin(const double* part, int nseg, int ncell, double ff, double* sum) {
+  const int c = blockIdx.x * 256 + threadIdx.x;
   if (c >= ncell) return;
   double s = 0.0;
-  for (int e = 0; e < nev; ++e)
-    if (accept[e]) s += ff * (double)grid[(size_t)e * ncell + c];
-  sum[c] += s;
+  for (int g = 0; g < nseg; ++g) s += part[(size_t)g * ncell + c];
+  sum[c] += ff * s;
 }
 
 // Spectators (MCnucl::getSpectators, MCnucl.cpp:1222-1245): the nucleons
@@ -233,11 +277,16 @@ int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int
   return (int)hipGetLastError();
 }
 
-int launch_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept, double ff, double* sum,
-                     void* stream) {
+int launch_accum_avg(const float* grid, const int* counts, int nx, int ny, int nev, const unsigned char* accept,
+                     double ff, double* part, double* sum, void* stream) {
   if (nev <= 0) return 0;
-  hipLaunchKernelGGL(k_accum_avg, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, (hipStream_t)stream, grid,
-                     ncell, nev, accept, ff, sum);
+  const int ncell = nx * ny;
+  const int per_seg = (nev + kAccumSegments - 1) / kAccumSegments;
+  const int nseg = (nev + per_seg - 1) / per_seg;  // <= kAccumSegments: part holds kAccumSegments x ncell
+  const unsigned bx = (unsigned)((ncell + 255) / 256);
+  hipLaunchKernelGGL(k_accum_part, dim3(bx, (unsigned)nseg), dim3(256), 0, (hipStream_t)stream, grid, counts, nx, ny,
+                     nev, per_seg, accept, part);
+  hipLaunchKernelGGL(k_accum_fin, dim3(bx), dim3(256), 0, (hipStream_t)stream, part, nseg, ncell, ff, sum);
   return (int)hipGetLastError();
 }
 

@@ -178,8 +178,10 @@ int launch_cm_angle(const DevCfg& cfg, const EvBufs& b, const float* grid, const
 int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int nev, void* stream);
 // spectators of every event as a source list of their own (sb.counts / src / swin)
 int launch_spectator_src(const DevCfg& cfg, const EvBufs& b, const EvBufs& sb, int nev, void* stream);
-int launch_accum_avg(const float* grid, size_t ncell, int nev, const unsigned char* accept, double ff, double* sum,
-                     void* stream);
+// part: kAccumSegments x nx x ny doubles of scratch (segment partials)
+constexpr int kAccumSegments = 32;
+int launch_accum_avg(const float* grid, const int* counts, int nx, int ny, int nev, const unsigned char* accept,
+                     double ff, double* part, double* sum, void* stream);
 constexpr int kTileVariants = 4;
 int tile_units(const DevCfg& cfg);  // smearing tiles per event of cfg's variant (units of SmearSpec::cm_out)
 int tile_units_max(const DevCfg& cfg);  // the most over all variants (buffer sizing)

@@ -92,6 +92,26 @@ def test_rotated_average_sampled_events_status_default():
     assert tot == pytest.approx(rec[rec[:, 57] == 0, 53].sum(), rel=1e-3)
 
 
+def test_average_sums_equal_host_sum_of_turned_grids():
+    """The device sums (event segments, rectangle skipping) against a host
+    FP64 sum of the same turned grids: 301 events (segments of 10: one group
+    of eight plus a remainder), every seventh event excluded."""
+    from supermc_amd import Context
+    p = H.glauber_params("PbPb", max_events=301, finalFactor=1.3)
+    with Context(p) as ctx:
+        ctx.run(seed=9, event0=0, nev=301, fetch=False)
+        accept = (np.arange(301) % 7 != 3).astype(np.uint8)
+        sums = ctx.average_profiles(2, 2, accept=accept)
+        rho, _, _ = ctx.density()  # the grids are left turned by the order-2 angle
+    ref = p.finalFactor * rho[accept == 1].astype(np.float64).sum(axis=0)
+    np.testing.assert_allclose(sums[0], ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    # no accepted event: zero sums
+    with Context(p) as ctx:
+        ctx.run(seed=9, event0=0, nev=301, fetch=False)
+        s2 = ctx.average_profiles(2, 2, accept=np.zeros(301, np.uint8))
+    assert not s2.any()
+
+
 def test_rho_binary_and_spectators_match_oracle():
     from supermc_amd import Context
     p = H.glauber_params("PbPb", max_events=5)
