@@ -43,41 +43,64 @@ __device__ inline WsInt ws_integrals(double r, double R, double a, const double2
   return s;
 }
 
-// r = F^{-1}(u; R, a) on [0, R + 2.5] and dr/dR, dr/da.  tab (optional):
-// r at v = u^(1/3) = i/(kWsTab-1) for this (R, a), the Newton start.
-constexpr int kWsTab = 2049;
-__device__ inline void ws_finv(double u, double R, double a, const double2* gl, int ngl, double& r, double& rR,
-                               double& ra, const double* tab = nullptr) {
+// Normalisation Z = N(R + 2.5) and its R and a derivatives (the upper limit
+// moves with R).
+struct WsNorm {
+  double n, R, a;
+};
+__device__ inline WsNorm ws_norm(double R, double a, const double2* gl, int ngl) {
   const double rmax = R + 2.5;
   const WsInt Z = ws_integrals(rmax, R, a, gl, ngl);
   const double rhomax = 1.0 / (1.0 + exp(2.5 / a));
-  const double ZR = Z.nR + rmax * rmax * rhomax, Za = Z.na;
+  return WsNorm{Z.n, Z.nR + rmax * rmax * rhomax, Z.na};
+}
+
+// r = F^{-1}(u; R, a) on [0, R + 2.5] and dr/dR, dr/da.  tab (optional):
+// kWsTab quantiles r at v = u^(1/3) = i/(kWsTab-1) for this (R, a), then
+// {Z, Z_R, Z_a} (k_ws_table).  With it, Newton starts from the cubic
+// interpolation of the table (error ~1e-12 fm) and usually stops after one
+// step.  Newton stops once a step is below 1e-9 (R + 2.5): in the quadratic
+// regime the root is then within ~1e-16 fm of the new point.  The R and a
+// derivatives come from the last evaluation, within that step of r.
+__device__ inline void ws_finv(double u, double R, double a, const double2* gl, int ngl, double& r, double& rR,
+                               double& ra, const double* tab = nullptr) {
+  const double rmax = R + 2.5;
+  const WsNorm Z = tab ? WsNorm{tab[kWsTab], tab[kWsTab + 1], tab[kWsTab + 2]} : ws_norm(R, a, gl, ngl);
   const double target = u * Z.n;
   double lo = 0.0, hi = rmax, x = rmax * cbrt(u);
   if (tab) {
     const double v = cbrt(u) * (kWsTab - 1);
     const int i = min((int)v, kWsTab - 2);
     const double f = v - i;
-    x = tab[i] + f * (tab[i + 1] - tab[i]);
+    if (i >= 1 && i + 2 < kWsTab) {  // Lagrange cubic through nodes i-1 .. i+2
+      const double wm = -f * (f - 1.0) * (f - 2.0) / 6.0, w0 = (f + 1.0) * (f - 1.0) * (f - 2.0) / 2.0;
+      const double w1 = -(f + 1.0) * f * (f - 2.0) / 2.0, w2 = (f + 1.0) * f * (f - 1.0) / 6.0;
+      x = wm * tab[i - 1] + w0 * tab[i] + w1 * tab[i + 1] + w2 * tab[i + 2];
+    } else {
+      x = tab[i] + f * (tab[i + 1] - tab[i]);
+    }
   }
+  WsInt N{0.0, 0.0, 0.0};
+  double xe = x, rho = 0.0;  // last evaluation point and rho there
   for (int it = 0; it < 100; ++it) {
-    const WsInt N = ws_integrals(x, R, a, gl, ngl);
-    const double f = N.n - target;
-    if (f < 0.0) lo = x;
+    N = ws_integrals(x, R, a, gl, ngl);
+    const double fv = N.n - target;
+    if (fv < 0.0) lo = x;
     else hi = x;
-    const double rho = 1.0 / (1.0 + exp((x - R) / a));
-    double xn = x - f / (x * x * rho);
-    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-    const bool done = fabs(xn - x) <= 1e-14 * rmax;
+    rho = 1.0 / (1.0 + exp((x - R) / a));
+    double xn = x - fv / (x * x * rho);
+    // a step below the resolution of x (xn == x) is a converged root
+    const bool newton = (xn > lo && xn < hi) || xn == x;
+    if (!newton) xn = 0.5 * (lo + hi);
+    const bool done = fabs(xn - x) <= (newton ? 1e-9 : 1e-14) * rmax;
+    xe = x;
     x = xn;
     if (done) break;
   }
   r = x;
-  const WsInt N = ws_integrals(x, R, a, gl, ngl);
-  const double rho = 1.0 / (1.0 + exp((x - R) / a));
-  const double Fr = x * x * rho / Z.n;
-  const double FR = N.nR / Z.n - N.n * ZR / (Z.n * Z.n);
-  const double Fa = N.na / Z.n - N.n * Za / (Z.n * Z.n);
+  const double Fr = xe * xe * rho / Z.n;
+  const double FR = N.nR / Z.n - N.n * Z.R / (Z.n * Z.n);
+  const double Fa = N.na / Z.n - N.n * Z.a / (Z.n * Z.n);
   rR = -FR / Fr;
   ra = -Fa / Fr;
 }
@@ -150,10 +173,17 @@ __device__ __forceinline__ double diff_shape(int kind, double alpha, double kg) 
   return (kind == kSrcPair) ? (alpha + 1e-8) * kg : (1.0 - alpha + 1e-8) * 0.5 * kg;
 }
 
-// The radial quantile at v = u^(1/3) = i/(kWsTab-1), i < kWsTab.
+// The radial quantile at v = u^(1/3) = i/(kWsTab-1), i < kWsTab; thread 0
+// also stores {Z, Z_R, Z_a} after the table.
 __global__ __launch_bounds__(256) void k_ws_table(double R, double a, const double2* gl, int ngl, double* tab) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= kWsTab) return;
+  if (i == 0) {
+    const WsNorm z = ws_norm(R, a, gl, ngl);
+    tab[kWsTab] = z.n;
+    tab[kWsTab + 1] = z.R;
+    tab[kWsTab + 2] = z.a;
+  }
   const double v = (double)i / (kWsTab - 1);
   double r, rR, ra;
   ws_finv(v * v * v, R, a, gl, ngl, r, rR, ra);

@@ -124,8 +124,10 @@ struct DiffBufs {
   double* gev;        // [ev][5] dL/dtheta per event
   const double2* gl;  // Gauss-Legendre nodes/weights on [0, 1]
   int ngl;
-  double* wstab;      // radial quantile table of the current (R, a): Newton starts
+  double* wstab;      // [kWsTabLen] radial quantiles of the current (R, a) (Newton starts), then Z, Z_R, Z_a
 };
+constexpr int kWsTab = 2049;              // radial quantile nodes, uniform in u^(1/3)
+constexpr int kWsTabLen = kWsTab + 3;     // + the normalisation and its R, a derivatives
 // per-event scalars of the differentiable moments
 enum : int {
   kDsS = 0, kDsXc, kDsYc, kDsA2r, kDsA2i, kDsA3r, kDsA3i, kDsN, kDsMx, kDsMy,
