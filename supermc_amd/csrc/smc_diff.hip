@@ -142,22 +142,33 @@ __device__ inline double gamma_p(double s, double x, double lg1, double psi1, do
   return P;
 }
 
-// Unit-scale Gamma quantile x(v; s) (P(s, x) = v) and dx/ds.
+// Unit-scale Gamma quantile x(v; s) (P(s, x) = v) and dx/ds.  Halley's
+// iteration (P' = pdf, P'' = pdf ((s-1)/x - 1)) from the small-x form
+// x0 = (v Gamma(s+1))^(1/s), which lies left of the root for every s > 0
+// (e^-x sum_n x^n / ((s+1)...(s+n)) < 1).  The bracket [lo, hi] is built
+// from the evaluations; a step that leaves it bisects (or doubles while hi
+// is open).  A Halley step below 1e-5 x leaves an error ~1e-15 x, so the
+// iteration stops there: about 3-4 evaluations of P per quantile.
 __device__ inline void gamma_qf(double v, double s, double& x, double& dxds) {
+  if (!(v > 0.0)) {
+    x = 0.0;
+    dxds = 0.0;
+    return;
+  }
   const double lg1 = lgamma(s + 1.0), psi1 = digamma_d(s + 1.0), lgs = lg1 - log(s);
-  double lo = 0.0, hi = 1.0;
-  while (gamma_p(s, hi, lg1, psi1, nullptr) < v && hi < 1e3) hi *= 2.0;
-  // small-x form of P as the first guess, inside the bracket
+  double lo = 0.0, hi = HUGE_VAL;
   double xx = exp((log(v) + lg1) / s);
-  if (!(xx > lo && xx < hi)) xx = 0.5 * (lo + hi);
   for (int it = 0; it < 200; ++it) {
     const double f = gamma_p(s, xx, lg1, psi1, nullptr) - v;
     if (f < 0.0) lo = xx;
     else hi = xx;
     const double pdf = exp(-xx + (s - 1.0) * log(xx) - lgs);
-    double xn = xx - f / pdf;
-    if (!(xn > lo && xn < hi)) xn = 0.5 * (lo + hi);
-    const bool done = fabs(xn - xx) <= 1e-15 * xx;
+    const double d2 = pdf * ((s - 1.0) / xx - 1.0);
+    const double den = 2.0 * pdf * pdf - f * d2;
+    double xn = (den > 0.0) ? xx - 2.0 * f * pdf / den : xx - f / pdf;
+    const bool halley = (xn > lo && xn < hi) || xn == xx;
+    if (!halley) xn = (hi < HUGE_VAL) ? 0.5 * (lo + hi) : 2.0 * xx;
+    const bool done = fabs(xn - xx) <= (halley ? 1e-5 : 1e-15) * xx;
     xx = xn;
     if (done) break;
   }
@@ -380,108 +391,152 @@ __global__ __launch_bounds__(256) void k_diff_src(DevCfg cfg, EvBufs buf, DiffBu
 // mu dS/dy, from the per-event scalars of k_diff_moments (layout kDs*).
 __global__ __launch_bounds__(256) void k_diff_cot(DevCfg cfg, DiffBufs d, const double* cot, double ffdxdy,
                                                   float* G) {
+  // per event and n = 2, 3: lam cx / S, lam cy / S (the centre's dependence),
+  // lam Ar / (Am N), lam Ai / (Am N), lam Am / N^2 (the masked g_n terms)
+  __shared__ double cf[2][5];
+  __shared__ double c0;
   const int e = blockIdx.y;
   const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
+  const double* sc = d.dsc + (size_t)e * kDsStride;
+  if (threadIdx.x == 0) {
+    const double S = sc[kDsS], N = sc[kDsN] + 1e-15;
+    const double dNx = -2.0 * sc[kDsMx], dNy = -2.0 * sc[kDsMy];
+    c0 = cot[(size_t)e * 3 + 2] * ffdxdy;
+    for (int q = 0; q < 2; ++q) {
+      const double lam = cot[(size_t)e * 3 + q];
+      const double Ar = sc[kDsA2r + 2 * q], Ai = sc[kDsA2i + 2 * q];
+      const double Am = sqrt(Ar * Ar + Ai * Ai);
+      double* f = cf[q];
+      f[0] = f[1] = f[2] = f[3] = f[4] = 0.0;
+      if (lam == 0.0 || !(Am > 0.0)) continue;
+      double dAxr, dAxi, dAyr, dAyi;
+      if (q == 0) {  // d(z^2)/dx = 2z, d(z^2)/dy = 2iz
+        dAxr = -2.0 * sc[kDsMx];
+        dAxi = -2.0 * sc[kDsMy];
+        dAyr = 2.0 * sc[kDsMy];
+        dAyi = -2.0 * sc[kDsMx];
+      } else {
+        dAxr = -sc[kDsB3xr];
+        dAxi = -sc[kDsB3xi];
+        dAyr = -sc[kDsB3yr];
+        dAyi = -sc[kDsB3yi];
+      }
+      const double cx = (Ar * dAxr + Ai * dAxi) / (Am * N) - Am * dNx / (N * N);
+      const double cy = (Ar * dAyr + Ai * dAyi) / (Am * N) - Am * dNy / (N * N);
+      f[0] = lam * cx / S;
+      f[1] = lam * cy / S;
+      f[2] = lam * Ar / (Am * N);
+      f[3] = lam * Ai / (Am * N);
+      f[4] = lam * Am / (N * N);
+    }
+  }
+  __syncthreads();
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)nx * ny) return;
   const int ix = (int)(i / ny), iy = (int)(i % ny);
-  const double* sc = d.dsc + (size_t)e * kDsStride;
-  const double S = sc[kDsS], xc = sc[kDsXc], yc = sc[kDsYc], N = sc[kDsN] + 1e-15;
-  const double lam[2] = {cot[(size_t)e * 3], cot[(size_t)e * 3 + 1]};
-  const double mu = cot[(size_t)e * 3 + 2];
-  const double x = cfg.xmin + ix * cfg.dx - xc, y = cfg.ymin + iy * cfg.dy - yc;
+  const double x = cfg.xmin + ix * cfg.dx - sc[kDsXc], y = cfg.ymin + iy * cfg.dy - sc[kDsYc];
   const double r2 = x * x + y * y;
   const bool m = (d.mask[(size_t)e * nx * nwy + (size_t)ix * nwy + (iy >> 5)] >> (iy & 31)) & 1u;
-  double g = mu * ffdxdy;
-  // n = 2, 3: A_n, dA_n/dxc, dA_n/dyc (complex), dN/dxc, dN/dyc
-  const double dNx = -2.0 * sc[kDsMx], dNy = -2.0 * sc[kDsMy];
-  for (int q = 0; q < 2; ++q) {
-    if (lam[q] == 0.0) continue;
-    const double Ar = sc[kDsA2r + 2 * q], Ai = sc[kDsA2i + 2 * q];
-    const double Am = sqrt(Ar * Ar + Ai * Ai);
-    if (!(Am > 0.0)) continue;
-    double dAxr, dAxi, dAyr, dAyi;
-    if (q == 0) {  // d(z^2)/dx = 2z, d(z^2)/dy = 2iz
-      dAxr = -2.0 * sc[kDsMx];
-      dAxi = -2.0 * sc[kDsMy];
-      dAyr = 2.0 * sc[kDsMy];
-      dAyi = -2.0 * sc[kDsMx];
-    } else {
-      dAxr = -sc[kDsB3xr];
-      dAxi = -sc[kDsB3xi];
-      dAyr = -sc[kDsB3yr];
-      dAyi = -sc[kDsB3yi];
+  double g = c0;
+  g += (cf[0][0] + cf[1][0]) * x + (cf[0][1] + cf[1][1]) * y;
+  if (m && r2 > 0.0) {
+    // g_n(z) = z^n r^(2-n)
+    const double gr = x * x - y * y, gi = 2.0 * x * y;
+    g += cf[0][2] * gr + cf[0][3] * gi - cf[0][4] * r2;
+    if (cf[1][2] != 0.0 || cf[1][3] != 0.0 || cf[1][4] != 0.0) {
+      const double ir = 1.0 / sqrt(r2);
+      const double tr = (gr * x - gi * y) * ir, ti = (gr * y + gi * x) * ir;
+      g += cf[1][2] * tr + cf[1][3] * ti - cf[1][4] * r2;
     }
-    const double cx = (Ar * dAxr + Ai * dAxi) / (Am * N) - Am * dNx / (N * N);
-    const double cy = (Ar * dAyr + Ai * dAyi) / (Am * N) - Am * dNy / (N * N);
-    double v = (cx * x + cy * y) / S;
-    if (m && r2 > 0.0) {
-      // g_n(z) = z^n r^(2-n)
-      double gr = x * x - y * y, gi = 2.0 * x * y;
-      if (q == 1) {
-        const double ir = 1.0 / sqrt(r2);
-        const double tr = gr * x - gi * y, ti = gr * y + gi * x;
-        gr = tr * ir;
-        gi = ti * ir;
-      }
-      v += (Ar * gr + Ai * gi) / (Am * N) - Am * r2 / (N * N);
-    }
-    g += lam[q] * v;
   }
   G[(size_t)e * nx * ny + i] = (float)g;
 }
 
 // One wavefront per source: sums of G E, G E dx, G E dy, G E d^2 over the
 // frozen footprint (E = exp(-d^2 / 2w^2)), then dL/dW, dL/dx, dL/dy, dL/dw.
+// kGatherBlocks workgroups per event loop over its sources; workgroup ids
+// are mapped so that all of an event's workgroups run on one XCD (one L2
+// holds its cotangent grid).  Per chunk of 64 window rows, lane j computes
+// row j's offset and Gaussian once; the row loop reads them with readlane.
+constexpr int kGatherBlocks = 32;
 __global__ __launch_bounds__(256) void k_diff_gather(DevCfg cfg, EvBufs buf, SmearSpec spec, DiffBufs d,
-                                                     const float* G, double w, double alpha) {
-  const int e = blockIdx.y;
+                                                     const float* G, double w, double alpha, int nev) {
+  const unsigned L = blockIdx.x, rg = L % (8u * kGatherBlocks);
+  const int e = (int)(8u * (L / (8u * kGatherBlocks)) + rg % 8u), kb = (int)(rg / 8u);
+  if (e >= nev) return;
   const int lane = threadIdx.x & 63;
-  const int k = blockIdx.x * 4 + (threadIdx.x >> 6);
   const int K = buf.counts[(size_t)e * 8 + 5];
-  if (k >= K) return;
   const int nx = cfg.nx, ny = cfg.ny;
-  const int4 s = buf.src[(size_t)e * buf.src_cap + k];
-  const int4 wv = buf.swin[(size_t)e * buf.src_cap + k];
-  const int kind = s.w;
-  const int xw = (spec.win[kind] == kWinBox) ? wv.x : wv.z;
-  const int yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
-  const int xlo = xw & 0xffff, xhi = xw >> 16, ylo = yw & 0xffff, yhi = yw >> 16;
   const float c2 = (float)(1.0 / (2.0 * w * w) * 1.4426950408889634);
   const float cut2 = cfg.cut2;
   const float* g = G + (size_t)e * nx * ny;
-  float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
-  for (int c0 = ylo; c0 < yhi; c0 += 64) {
-    const int col = c0 + lane;
-    if (col >= yhi) continue;
-    const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
-    const float dy = (float)(ygh - s.y) * kHf;
-    const float dy2 = dy * dy;
-    for (int ix = xlo; ix < xhi; ++ix) {
-      const int rowh = (int)llrint((cfg.xmin + ix * cfg.dx) / kH);
-      const float dx = (float)(rowh - s.x) * kHf;
-      const float dx2 = dx * dx;
-      if (!(dy2 <= cut2 - dx2)) continue;
-      const float E = __builtin_amdgcn_exp2f(-dx2 * c2) * __builtin_amdgcn_exp2f(-dy2 * c2);
-      const float gv = g[(size_t)ix * ny + col] * E;
-      e0 += gv;
-      e1 = fmaf(gv, dx, e1);  // dx, dy: cell minus source
-      e2 = fmaf(gv, dy, e2);
-      e3 = fmaf(gv, dx2 + dy2, e3);
+  for (int k = kb * 4 + (threadIdx.x >> 6); k < K; k += 4 * kGatherBlocks) {
+    const int4 s = buf.src[(size_t)e * buf.src_cap + k];
+    const int4 wv = buf.swin[(size_t)e * buf.src_cap + k];
+    const int kind = s.w;
+    const int xw = (spec.win[kind] == kWinBox) ? wv.x : wv.z;
+    const int yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
+    const int xlo = xw & 0xffff, xhi = xw >> 16, ylo = yw & 0xffff, yhi = yw >> 16;
+    float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
+    for (int r0 = xlo; r0 < xhi; r0 += 64) {
+      const int nr = min(64, xhi - r0);
+      float dxl, exl;  // row r0 + lane: offset from the source and its Gaussian
+      {
+        const int rowh = (int)llrint((cfg.xmin + (r0 + lane) * cfg.dx) / kH);
+        dxl = (float)(rowh - s.x) * kHf;
+        exl = __builtin_amdgcn_exp2f(-(dxl * dxl) * c2);
+      }
+      for (int c0 = ylo; c0 < yhi; c0 += 64) {
+        const int col = c0 + lane;
+        const bool on = col < yhi;
+        const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
+        const float dy = (float)(ygh - s.y) * kHf;
+        const float dy2 = dy * dy;
+        const float Ey = __builtin_amdgcn_exp2f(-dy2 * c2);
+        const float* gr = g + (size_t)r0 * ny + (on ? col : ylo);
+        auto cell = [&](int j, float& gv, float& dx, float& dx2) {
+          dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dxl), j));
+          const float Ex = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(exl), j));
+          dx2 = dx * dx;
+          const bool in = on && dy2 <= cut2 - dx2;
+          gv = in ? gr[(size_t)j * ny] * (Ex * Ey) : 0.f;
+        };
+        int j = 0;
+        for (; j + 4 <= nr; j += 4) {  // four rows' loads in flight, added in row order
+          float gv[4], dx[4], dx2[4];
+#pragma unroll
+          for (int q = 0; q < 4; ++q) cell(j + q, gv[q], dx[q], dx2[q]);
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            e0 += gv[q];
+            e1 = fmaf(gv[q], dx[q], e1);  // dx, dy: cell minus source
+            e2 = fmaf(gv[q], dy, e2);
+            e3 = fmaf(gv[q], dx2[q] + dy2, e3);
+          }
+        }
+        for (; j < nr; ++j) {
+          float gv, dx, dx2;
+          cell(j, gv, dx, dx2);
+          e0 += gv;
+          e1 = fmaf(gv, dx, e1);
+          e2 = fmaf(gv, dy, e2);
+          e3 = fmaf(gv, dx2 + dy2, e3);
+        }
+      }
     }
-  }
-  const double s0 = wave_sum((double)e0), s1 = wave_sum((double)e1), s2 = wave_sum((double)e2),
-               s3 = wave_sum((double)e3);
-  if (lane == 0) {
-    const double norm = 1.0 / (2.0 * M_PI * w * w);
-    const double f = d.dsrc[(size_t)e * buf.src_cap + k].x;
-    const double W = ((kind == kSrcPair) ? alpha : 0.5 * (1.0 - alpha)) * f;
-    double4 o;
-    o.x = norm * s0;                                  // dL/dW
-    o.y = W * norm * s1 / (w * w);                    // dL/dx_source
-    o.z = W * norm * s2 / (w * w);                    // dL/dy_source
-    o.w = W * norm * (s3 / (w * w * w) - 2.0 * s0 / w);  // dL/dw through this source
-    d.gsrc[(size_t)e * buf.src_cap + k] = o;
+    const double s0 = wave_sum((double)e0), s1 = wave_sum((double)e1), s2 = wave_sum((double)e2),
+                 s3 = wave_sum((double)e3);
+    if (lane == 0) {
+      const double norm = 1.0 / (2.0 * M_PI * w * w);
+      const double f = d.dsrc[(size_t)e * buf.src_cap + k].x;
+      const double W = ((kind == kSrcPair) ? alpha : 0.5 * (1.0 - alpha)) * f;
+      double4 o;
+      o.x = norm * s0;                                     // dL/dW
+      o.y = W * norm * s1 / (w * w);                       // dL/dx_source
+      o.z = W * norm * s2 / (w * w);                       // dL/dy_source
+      o.w = W * norm * (s3 / (w * w * w) - 2.0 * s0 / w);  // dL/dw through this source
+      d.gsrc[(size_t)e * buf.src_cap + k] = o;
+    }
   }
 }
 
@@ -569,8 +624,9 @@ int launch_diff_backward(const DevCfg& cfg, const EvBufs& b, const SmearSpec& sp
   const size_t ncell = (size_t)cfg.nx * cfg.ny;
   hipLaunchKernelGGL(k_diff_cot, dim3((unsigned)((ncell + 255) / 256), nev), dim3(256), 0, st, cfg, d, cot_dev,
                      ffdxdy, G);
-  hipLaunchKernelGGL(k_diff_gather, dim3((unsigned)((b.src_cap + 3) / 4), nev), dim3(256), 0, st, cfg, b, spec, d,
-                     (const float*)G, theta[0], theta[3]);
+  const unsigned nev8 = (unsigned)((nev + 7) / 8 * 8);  // k_diff_gather: whole groups of 8 events (one per XCD)
+  hipLaunchKernelGGL(k_diff_gather, dim3(nev8 * kGatherBlocks), dim3(256), 0, st, cfg, b, spec, d, (const float*)G,
+                     theta[0], theta[3], nev);
   hipLaunchKernelGGL(k_diff_chain, dim3(nev), dim3(256), 0, st, cfg, b, d, theta[3], theta[4]);
   return (int)hipGetLastError();
 }
