@@ -410,7 +410,7 @@ struct BandLayout {
 
 // Pass over one row band of one event.
 template <int NORD, int MAXC, bool FIX25>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? 4 : 1, 8))) void k_mom_band(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? (MAXC <= 5 ? 4 : 3) : 1, 8))) void k_mom_band(
     DevCfg cfg, EvBufs buf, const float* grid, const double* cmp, int nu, const uint2* rects, const int* nrect,
     double* band) {
   extern __shared__ unsigned bm[];
@@ -668,8 +668,9 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
   const int nord = cfg.to_order - cfg.from_order + 1;
   const size_t shm = (size_t)((cfg.nx + kNB - 1) / kNB) * ((cfg.ny + 31) / 32) * sizeof(unsigned);
   const dim3 gb(kNB, nev);
-  if (cfg.from_order == 2 && cfg.to_order == 5 && narrow) {
-    hipLaunchKernelGGL((k_mom_band<4, 5, true>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+  if (cfg.from_order == 2 && cfg.to_order == 5) {
+    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, true>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    else hipLaunchKernelGGL((k_mom_band<4, 9, true>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
     hipLaunchKernelGGL((k_mom_final<4, true>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
   } else if (nord <= 4) {
     if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
