@@ -361,7 +361,8 @@ __device__ void band_mask(const DevCfg& cfg, const uint2* rects, int np, const R
 // r^n-weighted sums, one r^2 normalisation for all orders.
 //   a[0..9] <r^n> numerators, a[10] sum_mask r^2 d, per order n:
 //   a[11 + 5 (n - FROM) + {0,1}] = sum_mask z^n r^(2-n) d, {2,3} = z^n d,
-//   {4} = r^n d (z = x + i y about the centre).
+//   {4} = r^n d (z = x + i y about the centre); for n = 1 (FROM = 1) the
+//   reference weights z r d, z r^2 d and r^3 d instead.
 template <int FROM, int TO>
 __device__ __forceinline__ void cell_fixed(double* a, double x, double y, double d, bool masked) {
   const double r2 = fma(x, x, y * y);
@@ -384,6 +385,15 @@ __device__ __forceinline__ void cell_fixed(double* a, double x, double y, double
   }
   if (masked && r2 > 0.0) {
     a[10] = fma(r2, d, a[10]);
+    if constexpr (FROM == 1) {  // n = 1: z r and z r^2, r^3 (MakeDensity.cpp:2408-2413)
+      double* o = a + 11;
+      const double rd = r * d, r2d = r2 * d;
+      o[0] = fma(x, rd, o[0]);
+      o[1] = fma(y, rd, o[1]);
+      o[2] = fma(x, r2d, o[2]);
+      o[3] = fma(y, r2d, o[3]);
+      o[4] += pw[3];
+    }
     double zr = x, zi = y, dw = d;  // z^n and d r^(2-n)
 #pragma unroll
     for (int n = 2; n <= TO; ++n) {
@@ -403,18 +413,20 @@ __device__ __forceinline__ void cell_fixed(double* a, double x, double y, double
   }
 }
 
-template <int NORD, bool FIX25>
+// FIX = 0: any order range (moment_cell); FIX = f > 0: orders f .. f + NORD - 1
+// fixed at compile time (cell_fixed), with one shared r^2 denominator.
+template <int NORD, int FIX>
 struct BandLayout {
-  static constexpr int NV = FIX25 ? 11 + 5 * 4 : 10 + 6 * NORD;
+  static constexpr int NV = FIX ? 11 + 5 * NORD : 10 + 6 * NORD;
 };
 
 // Pass over one row band of one event.
-template <int NORD, int MAXC, bool FIX25>
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? (MAXC <= 5 ? 4 : 3) : 1, 8))) void k_mom_band(
+template <int NORD, int MAXC, int FIX>
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX ? (NORD > 4 ? 2 : (MAXC <= 5 ? 4 : 3)) : 1, 8))) void k_mom_band(
     DevCfg cfg, EvBufs buf, const float* grid, const double* cmp, int nu, const uint2* rects, const int* nrect,
     double* band) {
   extern __shared__ unsigned bm[];
-  constexpr int NV = BandLayout<NORD, FIX25>::NV;
+  constexpr int NV = BandLayout<NORD, FIX>::NV;
   static_assert(NV <= kNVMax, "band partials");
   const int b = blockIdx.x, e = blockIdx.y, lane = threadIdx.x;
   const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
@@ -430,7 +442,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? (MAX
 #pragma unroll
   for (int i = 0; i < NV; ++i) a[i] = 0.0;
   MomAcc<NORD> M;
-  if constexpr (!FIX25) {
+  if constexpr (!FIX) {
 #pragma unroll
     for (int n = 0; n < 10; ++n) M.rn[n] = 0.0;
 #pragma unroll
@@ -452,8 +464,8 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? (MAX
       if (cur[j] != 0.f) {
         const int iy = R.y0 + lane + 64 * j;
         const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
-        if constexpr (FIX25)
-          cell_fixed<2, 5>(a, x, yv[j], (double)cur[j], masked);
+        if constexpr (FIX != 0)
+          cell_fixed<FIX, FIX + NORD - 1>(a, x, yv[j], (double)cur[j], masked);
         else
           moment_cell<NORD>(M, from, to, (double)cur[j], x, yv[j], masked);
       }
@@ -461,7 +473,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? (MAX
 #pragma unroll
     for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
   }
-  if constexpr (!FIX25) {
+  if constexpr (!FIX) {
 #pragma unroll
     for (int n = 0; n < 10; ++n) a[n] = M.rn[n];
 #pragma unroll
@@ -484,7 +496,7 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX25 ? (MAX
 }
 
 // Event record from the band sums T (layout of k_mom_band).
-template <int NORD, bool FIX25>
+template <int NORD, int FIX>
 __device__ void write_record(const DevCfg& cfg, const EvBufs& buf, int e, const double* T, double total, double xc,
                              double yc, double* evout) {
   const int* cnt = buf.counts + (size_t)e * 8;
@@ -492,9 +504,10 @@ __device__ void write_record(const DevCfg& cfg, const EvBufs& buf, int e, const 
   for (int i = 0; i < 40; ++i) out[i] = 0.0;
   for (int n = 0; n < 10; ++n) out[SMC_EV_RN + n] = T[n] / total;
   const double eps = 1e-15;
-  if constexpr (FIX25) {
-    for (int n = 2; n <= 5; ++n) {
-      const double* o = T + 11 + 5 * (n - 2);
+  if constexpr (FIX != 0) {
+    for (int n = FIX; n < FIX + NORD; ++n) {
+      if (n < cfg.from_order || n > cfg.to_order) continue;  // sums of a wider fixed range
+      const double* o = T + 11 + 5 * (n - FIX);
       out[SMC_EV_EPS_RE + n] = -o[0] / (T[10] + eps);
       out[SMC_EV_EPS_IM + n] = -o[1] / (T[10] + eps);
       out[SMC_EV_EPSP_RE + n] = -o[2] / (o[4] + eps);
@@ -521,11 +534,11 @@ __device__ void write_record(const DevCfg& cfg, const EvBufs& buf, int e, const 
   for (int i = SMC_EV_STATUS + 1; i < SMC_EV_STRIDE; ++i) out[i] = 0.0;  // reserved slots
 }
 
-template <int NORD, bool FIX25>
+template <int NORD, int FIX>
 __global__ __launch_bounds__(64) void k_mom_final(DevCfg cfg, EvBufs buf, const double* cmp, int nu,
                                                   const double* band, double* evout) {
   __shared__ double T[kNVMax];
-  constexpr int NV = BandLayout<NORD, FIX25>::NV;
+  constexpr int NV = BandLayout<NORD, FIX>::NV;
   const int e = blockIdx.x, lane = threadIdx.x;
   double total, xc, yc;
   event_cm(cfg, cmp, nu, e, total, xc, yc);
@@ -535,7 +548,7 @@ __global__ __launch_bounds__(64) void k_mom_final(DevCfg cfg, EvBufs buf, const 
     T[lane] = s;
   }
   __syncthreads();
-  if (lane == 0) write_record<NORD, FIX25>(cfg, buf, e, T, total, xc, yc, evout);
+  if (lane == 0) write_record<NORD, FIX>(cfg, buf, e, T, total, xc, yc, evout);
 }
 
 // Sum of the grid only (dndy, MCnucl.cpp:817-820), no finalFactor.
@@ -668,18 +681,22 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
   const int nord = cfg.to_order - cfg.from_order + 1;
   const size_t shm = (size_t)((cfg.nx + kNB - 1) / kNB) * ((cfg.ny + 31) / 32) * sizeof(unsigned);
   const dim3 gb(kNB, nev);
+  // fixed-order forms for orders 2..5 (the eccentricity table) and 1..9
+  // (the parameter file's default, and every range of more than four
+  // orders, whose records keep only the requested ones); ranges of up to
+  // four other orders take the general form
   if (cfg.from_order == 2 && cfg.to_order == 5) {
-    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, true>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    else hipLaunchKernelGGL((k_mom_band<4, 9, true>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    hipLaunchKernelGGL((k_mom_final<4, true>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
-  } else if (nord <= 4) {
-    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    else hipLaunchKernelGGL((k_mom_band<4, 9, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    hipLaunchKernelGGL((k_mom_final<4, false>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
+    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, 2>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    else hipLaunchKernelGGL((k_mom_band<4, 9, 2>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    hipLaunchKernelGGL((k_mom_final<4, 2>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
+  } else if (nord > 4) {
+    if (narrow) hipLaunchKernelGGL((k_mom_band<9, 5, 1>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    else hipLaunchKernelGGL((k_mom_band<9, 9, 1>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    hipLaunchKernelGGL((k_mom_final<9, 1>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
   } else {
-    if (narrow) hipLaunchKernelGGL((k_mom_band<9, 5, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    else hipLaunchKernelGGL((k_mom_band<9, 9, false>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    hipLaunchKernelGGL((k_mom_final<9, false>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
+    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, 0>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    else hipLaunchKernelGGL((k_mom_band<4, 9, 0>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    hipLaunchKernelGGL((k_mom_final<4, 0>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
   }
   return (int)hipGetLastError();
 }

@@ -118,6 +118,29 @@ def test_fed_positions_full_chain(system, path):
             _check_event_record(rec[e], o, p)
 
 
+@pytest.mark.parametrize("orders,dx", [((1, 9), 0.1), ((1, 9), 0.05), ((3, 7), 0.1), ((1, 3), 0.1), ((3, 5), 0.05)])
+def test_order_ranges_match_oracle(orders, dx):
+    """eps_1..eps_9 (the parameter file's default range, with n = 1's r^3
+    weight, MakeDensity.cpp:2408-2413) on the fixed-order moments kernel at
+    261^2 and 521^2, a wider-than-four range on the same kernel (records
+    keep the requested orders), and ranges of up to four orders on the
+    general kernel, against the oracle on fed positions."""
+    p = H.glauber_params("PbPb", max_events=5, ecc_from_order=orders[0], ecc_to_order=orders[1], dx=dx, dy=dx)
+    bvals = np.array([0.0, 3.0, 6.5, 9.5, 12.0])
+    xa, xb, bs = H.cpu_configs(p, len(bvals), seed=77, bvals=bvals)
+    with _ctx(p) as ctx:
+        ctx.set_positions(xa, xb, bs, fluct_seed=5)
+        ctx.collide(allow=(4,))
+        ctx.density()
+        rec = ctx.moments()
+        for e in range(len(bvals)):
+            _check_event_record(rec[e], H.oracle_event(ctx, e), p)
+        # orders outside the range stay zero
+        for n in range(10):
+            if not orders[0] <= n <= orders[1]:
+                assert (rec[:, [n, 10 + n, 20 + n, 30 + n]] == 0).all()
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_sampled_events_match_oracle(path):
     p = H.glauber_params("PbPb", max_events=48, smear_path=path)
