@@ -57,6 +57,11 @@ typedef struct smc_params {
   /* nuclei (parameters.dat:29-35) */
   int Aproj, Atarg, proj_deformed, targ_deformed;
   /* nucleons (parameters.dat:38-60) */
+  /* collision_criterion (MCnucl.cpp:357-385): 1 disk b^2 <= dsq; 2 Gaussian
+   * probability 1 - exp(-sigma_gg e^{-b^2/4w^2} / (4 pi w^2)) per pair whose
+   * 8w boxes overlap (GaussianNucleonsCal.cpp:59-67); other values follow
+   * shape_of_entropy (2 -> Gaussian).  3 and 4 (quark / density collisions)
+   * are not on this path. */
   int shape_of_nucleons, collision_criterion, shape_of_entropy;
   double quark_width, gauss_nucl_width;
   /* collision (parameters.dat:64-68) */
@@ -102,6 +107,10 @@ typedef struct smc_derived {
   double kln_dT;       /* table step (MCnucl.cpp:915-919) */
   int kln_tmax;
   double ws_R[2], ws_a[2], beta2[2], beta4[2], dmin;
+  int criterion;       /* collision criterion in effect (1 disk, 2 Gaussian) */
+  double sigma_gg;     /* effective cross section [fm^2] of the Gaussian test
+                          (GaussianNucleonsCal.cpp:130-163) */
+  int64_t box_lim;     /* half the tested box, 8w, in quanta (Particle.cpp:20) */
 } smc_derived;
 
 /* Per-event result record written by smc_moments / smc_run.

@@ -176,6 +176,62 @@ def collide(xy_a, xy_b, w, dsq):
                 npart_a=npa.value, npart_b=npb.value, pair_a=pa[:nc], pair_b=pb[:nc])
 
 
+def sig_eff(sigin_mb, w):
+    """sigma_gg [fm^2] of the Gaussian collision test: Newton on
+    sigma_in = int_0^{5w} 2 pi b db (1 - exp(-sigma_gg T_pp(b))),
+    T_pp(b) = exp(-b^2/4w^2) / (4 pi w^2), with a 38-node Gauss-Legendre rule,
+    started at 10 and stopped once a step is at most 1e-4
+    (GaussianNucleonsCal.cpp:130-163)."""
+    xg, wg = np.polynomial.legendre.leggauss(38)
+    xg, wg = (xg + 1.0) / 2.0, wg / 2.0
+    sigin, bmax, w4 = sigin_mb * 0.1, 5.0 * w, 4.0 * w * w
+    s = 10.0
+    while True:
+        s0 = s
+        b, db = xg * bmax, wg * bmax
+        tpp = np.exp(-b * b / w4) / (np.pi * w4)
+        total = float(np.sum(2 * np.pi * b * db * (1.0 - np.exp(-s * tpp))))
+        dn = float(np.sum(2 * np.pi * b * db * tpp * np.exp(-s * tpp)))
+        s -= (total - sigin) / dn
+        if abs(s - s0) <= 1e-4:
+            return s
+
+
+def gauss_hit_prob(b, w, sigma_gg):
+    """Collision probability of the Gaussian test at impact parameter b
+    (GaussianNucleonsCal.cpp:59-67)."""
+    b = np.asarray(b, np.float64)
+    return 1.0 - np.exp(-sigma_gg * np.exp(-b * b / (4 * w * w)) / (4 * np.pi * w * w))
+
+
+def gauss_tested_pairs(xy_a, xy_b, w):
+    """Pairs (i, j) the reference's sweep hands to hit() under the Gaussian
+    criterion (MCnucl.cpp:244-289): nucleons sorted by x, boxes of side 8w
+    (Particle.cpp:20).  For each projectile the targets from the first whose
+    box reaches the projectile's, while the previous target's box does --
+    so one target past the box is tested after an in-range one -- and of
+    those the ones whose boxes overlap in y."""
+    h = 4.0 * w
+    out = []
+    start, nb = 0, len(xy_b)
+    last = None  # the box most recently read, as the reference's targBox
+    for i, (xp, yp) in enumerate(xy_a):
+        pxl, pxr, pyl, pyr = xp - h, xp + h, yp - h, yp + h
+        while start < nb:
+            last = xy_b[start]
+            if last[0] + h >= pxl:
+                break
+            start += 1
+        j = start
+        while j < nb and last is not None and pxr >= last[0] - h:
+            xt, yt = xy_b[j]
+            last = xy_b[j]
+            if pyl <= yt + h and pyr >= yt - h:
+                out.append((i, j))
+            j += 1
+    return out
+
+
 def collide_bruteforce(xy_a, xy_b, dsq):
     xa, ya = np.ascontiguousarray(xy_a[:, 0]), np.ascontiguousarray(xy_a[:, 1])
     xb, yb = np.ascontiguousarray(xy_b[:, 0]), np.ascontiguousarray(xy_b[:, 1])

@@ -63,6 +63,7 @@ class SmcDerived(C.Structure):
         ("hit_n_max", C.c_int64), ("kln_dT", C.c_double), ("kln_tmax", C.c_int),
         ("ws_R", C.c_double * 2), ("ws_a", C.c_double * 2), ("beta2", C.c_double * 2),
         ("beta4", C.c_double * 2), ("dmin", C.c_double),
+        ("criterion", C.c_int), ("sigma_gg", C.c_double), ("box_lim", C.c_int64),
     ]
 
 

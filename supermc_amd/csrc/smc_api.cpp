@@ -230,6 +230,33 @@ void check_launch(int rc, const char* what) {
   if (rc != 0) throw SmcError(SMC_ERR_HIP, std::string(what) + ": " + hipGetErrorString((hipError_t)rc));
 }
 
+void gauss_legendre01(int n, std::vector<double>& x, std::vector<double>& w);
+
+// sigma_gg of the Gaussian collision test: Newton on
+//   sigma_in = int_0^{5w} 2 pi b db (1 - exp(-sigma_gg T_pp(b))),
+//   T_pp(b) = e^{-b^2/4w^2} / (4 pi w^2),
+// with a 38-node Gauss-Legendre rule, started at 10 fm^2 and stopped once a
+// step is at most 1e-4 -- the iteration of GaussianNucleonsCal.cpp:130-163.
+double sig_eff(double sigin_mb, double w) {
+  std::vector<double> x, wt;
+  gauss_legendre01(38, x, wt);
+  const double sigin = sigin_mb * 0.1, bmax = 5.0 * w, w4 = 4.0 * w * w;
+  double s = 10.0, s0;
+  int it = 0;
+  do {
+    s0 = s;
+    double sum = 0.0, dn = 0.0;
+    for (int i = 0; i < 38; ++i) {
+      const double b = x[i] * bmax, db = wt[i] * bmax;
+      const double tpp = std::exp(-b * b / w4) / (M_PI * w4);
+      sum += 2 * M_PI * b * db * (1.0 - std::exp(-s * tpp));
+      dn += 2 * M_PI * b * db * tpp * std::exp(-s * tpp);
+    }
+    s -= (sum - sigin) / dn;
+  } while (std::fabs(s - s0) > 1e-4 && ++it < 1000);
+  return s;
+}
+
 void validate_and_derive(smc_ctx* c) {
   smc_params& p = c->p;
   smc_derived& d = c->d;
@@ -241,10 +268,13 @@ void validate_and_derive(smc_ctx* c) {
     throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN requires sub_model 7 (KLN uGD)");
   if (!(p.shape_of_nucleons == 2 || p.shape_of_nucleons == 4))
     throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_nucleons must be 2 or 4 (Gaussian nucleons)");
-  if (p.collision_criterion != 1)
-    throw SmcError(SMC_ERR_UNSUPPORTED, "collision_criterion must be 1 (disk)");
   if (p.shape_of_entropy != 2)
     throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_entropy must be 2 (Gaussian)");
+  // MCnucl.cpp:363-383: 1 disk, 2 Gaussian; other values follow
+  // shape_of_entropy (2 here, so the Gaussian test)
+  d.criterion = (p.collision_criterion >= 1 && p.collision_criterion <= 4) ? p.collision_criterion : 2;
+  if (d.criterion > 2)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "collision_criterion 3 and 4 (quark / density collisions) are not on this path");
   if (!(p.cc_fluctuation_model == 0 || p.cc_fluctuation_model == 6))
     throw SmcError(SMC_ERR_UNSUPPORTED, "cc_fluctuation_model must be 0 or 6");
   const int As[2] = {p.Aproj, p.Atarg};
@@ -266,8 +296,10 @@ void validate_and_derive(smc_ctx* c) {
   d.dsq = 0.1 * d.siginNN / M_PI / 1;
   const double w = d.width;
   if (!(w > 0.05 && w <= 1.0)) throw SmcError(SMC_ERR_ARG, "nucleon width must be in (0.05, 1] fm");
-  if (std::sqrt(d.dsq) > 8 * w)
+  if (d.criterion == 1 && std::sqrt(d.dsq) > 8 * w)
     throw SmcError(SMC_ERR_UNSUPPORTED, "collision disk wider than the nucleon box (sweep pruning differs)");
+  d.sigma_gg = sig_eff(d.siginNN, w);
+  d.box_lim = (int64_t)std::floor(8.0 * w / kQ);
   // grid (MCnucl.cpp:33-40)
   d.xmin = -p.maxx;
   d.ymin = -p.maxy;
@@ -337,6 +369,10 @@ void validate_and_derive(smc_ctx* c) {
   g.max_pairs = p.max_pairs > 0 ? p.max_pairs : std::min(p.Aproj * p.Atarg, 8192);
   g.hit_n = d.hit_n_max;
   g.hit_lim = (int)std::floor(std::sqrt((double)d.hit_n_max)) + 1;
+  g.crit_gauss = d.criterion == 2;
+  g.box_lim = (int)d.box_lim;
+  g.gauss_k = (float)(1.4426950408889634 * kQ * kQ / (4.0 * w * w));
+  g.gauss_A = (float)(d.sigma_gg / (4.0 * M_PI * w * w));
   g.fluct = p.cc_fluctuation_model == 6;
   {
     // MCnucl.cpp:1271-1301 (shape_of_entropy != 3)
@@ -824,6 +860,7 @@ void diff_check_params(const smc_ctx* c) {
   const smc_params& p = c->p;
   const smc_derived& d = c->d;
   if (p.which_mc_model != 5) throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need which_mc_model 5");
+  if (d.criterion != 1) throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need the disk criterion (1)");
   for (int nu = 0; nu < 2; ++nu)
     if (c->cfg.nuc[nu].kind != kWoodsSaxon || c->cfg.nuc[nu].deformed)
       throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need spherical Woods-Saxon nuclei");
@@ -1387,7 +1424,7 @@ void graph_capture(smc_ctx* c, int nev, uint64_t seed, uint64_t event0) {
     if (t != hipGraphNodeTypeKernel) continue;
     hipKernelNodeParams kp{};
     HIPCHK(hipGraphKernelNodeGetParams(node, &kp));
-    if (kp.func == event_kernel_symbol() && kp.kernelParams) {
+    if (kp.func == event_kernel_symbol(c->cfg) && kp.kernelParams) {
       c->graph_event_node = node;
       c->graph_event_params = kp;
       c->graph_args.assign(kp.kernelParams, kp.kernelParams + kEventKernelArgs);

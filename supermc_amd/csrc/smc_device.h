@@ -34,6 +34,7 @@ enum Stream : unsigned {
   kStrDiffB = 8,        // differentiable batches: impact parameter
   kStrDiffNucleon = 9,  //   nucleon candidates (radial quantile, direction)
   kStrDiffGamma = 10,   //   Gamma quantile uniforms
+  kStrCollide = 11,     // collision criterion 2: one uniform per tested pair
 };
 
 __device__ __forceinline__ uint4 draw(uint2 key, unsigned long long ev, unsigned stream,

@@ -60,6 +60,10 @@ struct DevCfg {
   int max_pairs;
   long long hit_n;       // collide iff dxq^2+dyq^2 <= hit_n (quanta^2)
   int hit_lim;           // |dxq| > hit_lim cannot collide
+  int crit_gauss;        // collision criterion 2: a draw per pair of overlapping 8w boxes
+  int box_lim;           // 8w in quanta: boxes overlap iff |dxq|, |dyq| <= box_lim
+  float gauss_k;         // log2(e) q^2 / (4 w^2): e^{-b^2/4w^2} = 2^(-n gauss_k), n in quanta^2
+  float gauss_A;         // sigma_gg / (4 pi w^2)
   int fluct;             // Gamma factors on
   float gk_part, gs_part, gk_bin, gs_bin;  // MCnucl.cpp:1271-1301
   // grid
@@ -157,7 +161,7 @@ struct SmearSpec {
 // Launch wrappers (smc_kernels.hip)
 // k_event's host handle (matches the func of its captured graph node); its
 // kernel parameters are (cfg, buf, key, event0, mode)
-const void* event_kernel_symbol();
+const void* event_kernel_symbol(const DevCfg& cfg);  // the k_event instance cfg launches
 constexpr int kEventKernelArgs = 5, kEventArgKey = 2, kEventArgEvent0 = 3;
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
                  unsigned long long event0, int mode, void* stream);

@@ -19,6 +19,9 @@
 namespace smc {
 
 // ------------------------------------------------------------- k_event
+// GAUSS: collision criterion 2 (the disk criterion's instance carries none
+// of its code)
+template <bool GAUSS>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
                                                unsigned long long event0, int mode) {
   // mode 0: impact parameter sampled, events accepted as MakeDensity.cpp:2147-2162
@@ -298,7 +301,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
 #pragma unroll
         for (int w = 0; w < kAMax / 32; ++w) bits[w] = 0u;
         const int xa = sx[0][t], ya = sy[0][t];
-        const int lim = cfg.hit_lim;
+        // criterion 2 tests every target whose 8w box overlaps the
+        // projectile's (MCnucl.cpp:244-289), with one uniform each
+        constexpr bool g2 = GAUSS;
+        const int lim = g2 ? cfg.box_lim : cfg.hit_lim;
         int lo = 0, hi = B;
         while (lo < hi) {
           const int mid = (lo + hi) >> 1;
@@ -310,6 +316,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         for (int j4 = lo >> 2; 4 * j4 < B; ++j4) {
           const int4 xb4 = sx4[j4], yb4 = sy4[j4];
           const int xbs[4] = {xb4.x, xb4.y, xb4.z, xb4.w}, ybs[4] = {yb4.x, yb4.y, yb4.z, yb4.w};
+          uint4 rr = make_uint4(0u, 0u, 0u, 0u);
+          if constexpr (g2) rr = draw(key, gev, kStrCollide, (unsigned)t, (unsigned)j4, (unsigned)attempt, 0u);
+          const unsigned rs[4] = {rr.x, rr.y, rr.z, rr.w};
           bool past = false;
 #pragma unroll
           for (int u = 0; u < 4; ++u) {
@@ -317,12 +326,23 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             if (j < lo || j >= B || past) continue;
             if (xbs[u] > xa + lim) {
               past = true;
-              continue;
+              // the reference's loop tests one more target, the first past
+              // the box, when it tested the one before (its condition reads
+              // the previous target's box); only a draw can make it collide
+              if (!g2 || j == lo) continue;
             }
             const int dx = xa - xbs[u], dy = ya - ybs[u];
             if (dy > lim || dy < -lim) continue;
             const long long n = (long long)dx * dx + (long long)dy * dy;
-            if (n <= cfg.hit_n) {
+            bool h;
+            if constexpr (g2) {  // u < 1 - exp(-sigma_gg T_pp(b)) (GaussianNucleonsCal.cpp:59-67)
+              const float q = cfg.gauss_A * __builtin_amdgcn_exp2f(-(float)n * cfg.gauss_k);
+              const float P = (q < 1e-3f) ? q * (1.f - 0.5f * q * (1.f - q * (1.f / 3.f))) : 1.f - __expf(-q);
+              h = ((float)rs[u] + 0.5f) * 0x1p-32f < P;
+            } else {
+              h = n <= cfg.hit_n;
+            }
+            if (h) {
 #pragma unroll
               for (int w = 0; w < kAMax / 32; ++w)
                 if (w == (j >> 5)) bits[w] |= 1u << (j & 31);
@@ -606,13 +626,16 @@ __global__ __launch_bounds__(256) void k_smear(DevCfg cfg, EvBufs buf, SmearSpec
 // ------------------------------------------------------------ launchers
 static hipStream_t S(void* s) { return (hipStream_t)s; }
 
-const void* event_kernel_symbol() { return reinterpret_cast<const void*>(&k_event); }
+const void* event_kernel_symbol(const DevCfg& cfg) {
+  return cfg.crit_gauss ? reinterpret_cast<const void*>(&k_event<true>) : reinterpret_cast<const void*>(&k_event<false>);
+}
 
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
                  unsigned long long event0, int mode, void* stream) {
   if (nev <= 0) return 0;
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  hipLaunchKernelGGL(k_event, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
+  if (cfg.crit_gauss) hipLaunchKernelGGL(k_event<true>, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
+  else hipLaunchKernelGGL(k_event<false>, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
   return (int)hipGetLastError();
 }
 
