@@ -296,6 +296,30 @@ def cpu_ops_baseline(seed, nev=3):
     return op3, op2
 
 
+def time_defaults(nev, steps, seed):
+    """The reference parameter file's defaults (smc_params_default):
+    sqrt(T_A T_B) (which_mc_model 7), shape_of_nucleons 2, the Gaussian
+    collision criterion, Gamma theta 0.75, eps_1..eps_9, Pb+Pb at 5020 GeV on
+    301^2 cells, tile shape autotuned."""
+    from supermc_amd import Context, default_params
+    with Context(default_params(max_events=nev)) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)
+        ctx.synchronize()
+        ctx.autotune(reps=1)
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ker = {name: ctx.timing(kid)[0] / max(ctx.timing(kid)[1], 1)
+               for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments"))}
+        ev = ctx.fetch_events(nev)
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, grid=[301, 301],
+                collision_criterion=2, orders=[1, 9], mean_npart=float(ev[:, 54].mean()),
+                failed_events=int((ev[:, 57] != 0).sum()), kernels_avg_ms=ker)
+
+
 def time_graph(seed):
     """Batches launched eagerly (smc_run) and replayed from a captured
     hipGraph (smc_run_graph), at 10^3 and 10^5 events per batch."""
@@ -407,6 +431,8 @@ def main():
     ap.add_argument("--avg-events", type=int, default=20000,
                     help="operation-3 averaged-profile events per batch (0: skip)")
     ap.add_argument("--jet-events", type=int, default=20000, help="operation-2 grid events per batch (0: skip)")
+    ap.add_argument("--default-events", type=int, default=100000,
+                    help="events per batch with the reference parameter file's defaults (0: skip)")
     ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed", "avg", "jet"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
@@ -524,6 +550,7 @@ def main():
     mixed = time_mixed(args.mixed_events, 2, args.seed) if args.mixed_events > 0 else None
     avg = time_avg(args.avg_events, 2, args.seed) if args.avg_events > 0 else None
     jet = time_jet(args.jet_events, 2, args.seed) if args.jet_events > 0 else None
+    dflt = time_defaults(args.default_events, 2, args.seed) if args.default_events > 0 else None
 
     cpu = None
     if not args.no_cpu:
@@ -559,7 +586,7 @@ def main():
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
                    "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
                    "graph_replay": graph, "tile_autotune": tune, "mixed_systems_521": mixed,
-                   "averaged_profiles_op3": avg, "jet_grids_op2": jet,
+                   "averaged_profiles_op3": avg, "jet_grids_op2": jet, "reference_defaults_301": dflt,
                    "records_to_host_events_per_s": host_rate},
         "roofline": roof,
         "cpu_baseline": cpu,
