@@ -62,7 +62,7 @@ typedef struct smc_params {
    * 8w boxes overlap (GaussianNucleonsCal.cpp:59-67); other values follow
    * shape_of_entropy (2 -> Gaussian).  3 and 4 (quark / density collisions)
    * are not on this path. */
-  int shape_of_nucleons, collision_criterion, shape_of_entropy;
+  int shape_of_nucleons, collision_criterion, shape_of_entropy;  /* shape 2, 3 or 4 (Gaussian nucleons) */
   double quark_width, gauss_nucl_width;
   /* collision (parameters.dat:64-68) */
   double ecm, bmin, bmax;
@@ -94,6 +94,8 @@ typedef struct smc_params {
                                   2 exact reference footprint, per-cell exponentials
                                   3 as 1 with BF16x3 split-precision MFMA (bf16 hi + lo
                                     factors, three products, FP32 accumulation) */
+  double gaussian_lambda;      /* shape_of_nucleons 3: width and sigma_gg from lambda
+                                  (GaussianNucleonsCal.cpp:41-46); parameter file 4.14 */
 } smc_params;
 
 /* Derived quantities (for the facade and tests) */

@@ -53,6 +53,7 @@ class SmcParams(C.Structure):
         ("kln_n_pt", C.c_int), ("kln_n_kt", C.c_int), ("kln_n_phi", C.c_int),
         ("kln_pt_min", C.c_double), ("kln_pt_max", C.c_double), ("kln_direct", C.c_int),
         ("max_events", C.c_int), ("max_pairs", C.c_int), ("max_attempts", C.c_int), ("smear_path", C.c_int),
+        ("gaussian_lambda", C.c_double),
     ]
 
 

@@ -266,8 +266,10 @@ void validate_and_derive(smc_ctx* c) {
     throw SmcError(SMC_ERR_UNSUPPORTED, "MC-Glauber sub_model 2 is not on this path");
   if (p.which_mc_model == 1 && p.sub_model != 7)
     throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN requires sub_model 7 (KLN uGD)");
-  if (!(p.shape_of_nucleons == 2 || p.shape_of_nucleons == 4))
-    throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_nucleons must be 2 or 4 (Gaussian nucleons)");
+  if (!(p.shape_of_nucleons >= 2 && p.shape_of_nucleons <= 4))
+    throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_nucleons must be 2, 3 or 4 (Gaussian nucleons)");
+  if (p.shape_of_nucleons == 3 && !(p.gaussian_lambda > 0))
+    throw SmcError(SMC_ERR_ARG, "shape_of_nucleons 3 needs gaussian_lambda > 0");
   if (p.shape_of_entropy != 2)
     throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_entropy must be 2 (Gaussian)");
   // MCnucl.cpp:363-383: 1 disk, 2 Gaussian; other values follow
@@ -291,14 +293,31 @@ void validate_and_derive(smc_ctx* c) {
 
   // sigma_inel and width (MCnucl.cpp:58-64, 93; GaussianNucleonsCal.cpp:34-54)
   d.siginNN = (p.sigma_nn_mb > 0) ? p.sigma_nn_mb : regge_sigma_inel(p.ecm);
+  double ratio3 = 0.0;  // shape 3: sigma_in / sigma_gg (GaussianNucleonsCal.cpp:41-46)
+  if (p.shape_of_nucleons == 3) {
+    // (gamma + int_lambda^{lambda+100} e^-t / t dt + ln lambda) / lambda, the
+    // integral by 16-node Gauss-Legendre on each of 100 unit intervals
+    std::vector<double> gx, gw;
+    gauss_legendre01(16, gx, gw);
+    const double lam = p.gaussian_lambda;
+    double e1 = 0.0;
+    for (int k = 0; k < 100; ++k)
+      for (int i = 0; i < 16; ++i) {
+        const double t = lam + k + gx[i];
+        e1 += gw[i] * std::exp(-t) / t;
+      }
+    ratio3 = (0.57721566490153286 + e1 + std::log(lam)) / lam;
+  }
   if (p.shape_of_nucleons == 2) d.width = std::sqrt(0.1 * d.siginNN / M_PI) / std::sqrt(8);
+  else if (p.shape_of_nucleons == 3)
+    d.width = std::sqrt(d.siginNN * 0.1 / (4 * M_PI * p.gaussian_lambda * ratio3));
   else d.width = p.gauss_nucl_width;
   d.dsq = 0.1 * d.siginNN / M_PI / 1;
   const double w = d.width;
   if (!(w > 0.05 && w <= 1.0)) throw SmcError(SMC_ERR_ARG, "nucleon width must be in (0.05, 1] fm");
   if (d.criterion == 1 && std::sqrt(d.dsq) > 8 * w)
     throw SmcError(SMC_ERR_UNSUPPORTED, "collision disk wider than the nucleon box (sweep pruning differs)");
-  d.sigma_gg = sig_eff(d.siginNN, w);
+  d.sigma_gg = (p.shape_of_nucleons == 3) ? d.siginNN * 0.1 / ratio3 : sig_eff(d.siginNN, w);
   d.box_lim = (int64_t)std::floor(8.0 * w / kQ);
   // grid (MCnucl.cpp:33-40)
   d.xmin = -p.maxx;
@@ -1045,6 +1064,7 @@ smc_status smc_params_default(smc_params* p) {
   p->proj_deformed = 0;
   p->targ_deformed = 0;
   p->shape_of_nucleons = 2;
+  p->gaussian_lambda = 4.14;
   p->collision_criterion = 2;
   p->shape_of_entropy = 2;
   p->quark_width = 0.3;

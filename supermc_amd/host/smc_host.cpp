@@ -29,6 +29,7 @@ smc_params params_from_reader(ParameterReader& r, int max_events) {
   p.shape_of_entropy = (int)r.getVal("shape_of_entropy");
   p.quark_width = r.getVal("quark_width");
   p.gauss_nucl_width = r.getVal("gauss_nucl_width");
+  p.gaussian_lambda = r.getVal("gaussian_lambda");  // read unconditionally, as GaussianNucleonsCal.cpp:27
   p.ecm = r.getVal("ecm");
   p.bmin = r.getVal("bmin");
   p.bmax = r.getVal("bmax");

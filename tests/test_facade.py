@@ -36,6 +36,7 @@ collision_criterion = 1
 shape_of_entropy = 2
 quark_width = 0.3
 gauss_nucl_width = 0.5
+gaussian_lambda = 4.14
 ecm = 2760.
 bmin = 0.
 bmax = 20.

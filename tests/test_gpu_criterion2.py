@@ -50,6 +50,20 @@ def test_derived_sigma_gg_box_and_criterion():
         assert ctx.derived.criterion == 2
 
 
+@pytest.mark.parametrize("lam", [4.14, 2.0])
+def test_shape3_width_and_sigma_gg(lam):
+    """shape_of_nucleons 3: width sqrt(0.1 sig / (4 pi lam ratio)) and
+    sigma_gg = 0.1 sig / ratio, ratio = Ein(lam)/lam (GaussianNucleonsCal.cpp:41-46)."""
+    from scipy import special
+    from supermc_amd import Context
+    p = _params(max_events=4, shape_of_nucleons=3, gaussian_lambda=lam, sigma_nn_mb=68.0)
+    with Context(p) as ctx:
+        d = ctx.derived
+    ratio = (np.euler_gamma + np.log(lam) + special.exp1(lam)) / lam
+    assert d.width == pytest.approx(np.sqrt(6.8 / (4 * np.pi * lam * ratio)), rel=1e-12)
+    assert d.sigma_gg == pytest.approx(6.8 / ratio, rel=1e-12)
+
+
 def test_collisions_follow_the_gaussian_law_on_fed_positions():
     from supermc_amd import Context
     reps = 64

@@ -68,3 +68,21 @@ def test_tested_pairs_are_box_overlaps_plus_one_extra_target():
                 if abs(xb[j, 1] - xa[i, 1]) <= 8 * w:
                     expect_extra.add((i, j))
         assert got - box == expect_extra
+
+
+@pytest.mark.parametrize("sig,lam", [(68.0, 4.14), (42.0, 2.0), (64.0, 7.5)])
+def test_shape3_width_and_sigma_gg_closed_form(sig, lam):
+    """shape_of_nucleons 3 (GaussianNucleonsCal.cpp:41-46): ratio =
+    (gamma + int_lam^{lam+100} e^-t/t dt + ln lam)/lam = Ein(lam)/lam up to
+    E1(lam+100) < 1e-45; width = sqrt(0.1 sig / (4 pi lam ratio)),
+    sigma_gg = 0.1 sig / ratio.  The device values (GPU test) are held to
+    these; here the closed form is checked against direct quadrature."""
+    ratio = _ein(lam) / lam
+    quad_int, _ = integrate.quad(lambda t: np.exp(-t) / t, lam, lam + 100, epsabs=0, epsrel=1e-13, limit=400)
+    assert (np.euler_gamma + quad_int + np.log(lam)) / lam == pytest.approx(ratio, rel=1e-12)
+    # the Gaussian test at sigma_gg reproduces sigma_in over the whole plane:
+    # int 2 pi b (1 - exp(-sigma_gg T_pp)) db = 4 pi w^2 Ein(A) with A = lam
+    w = np.sqrt(0.1 * sig / (4 * np.pi * lam * ratio))
+    S = 0.1 * sig / ratio
+    assert S / (4 * np.pi * w * w) == pytest.approx(lam, rel=1e-12)
+    assert 4 * np.pi * w * w * _ein(lam) == pytest.approx(0.1 * sig, rel=1e-12)
