@@ -728,10 +728,11 @@ void run_density(smc_ctx* c) {
     return;
   }
   SmearSpec spec = density_spec(c);
-  // the tile kernel adds the centre-of-mass partials of a single-accumulator grid
+  // the tile kernel adds the centre-of-mass partials of the stored grid
+  // (one accumulator, or sqrt(T_A T_B) combined in the same launch)
   const bool tile = c->p.smear_path != 2 && !strip_path(c, spec);  // run_smear's choice
-  if (tile && spec.combine == 0) spec.cm_out = c->cm;
-  run_smear(c, spec, grid_buffer(c, &c->grid), spec.combine ? grid_buffer(c, &c->tmp) : nullptr);
+  if (tile) spec.cm_out = c->cm;
+  run_smear(c, spec, grid_buffer(c, &c->grid), (spec.combine && !tile) ? grid_buffer(c, &c->tmp) : nullptr);
   c->cm_nu = spec.cm_out ? tile_units(c->cfg) : 0;
   c->have_density = 1;
 }

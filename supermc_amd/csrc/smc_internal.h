@@ -221,7 +221,6 @@ double regge_sigma_inel(double ecm);
 // centre-of-mass partials (tile_units per event) when set
 int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
                     const double* table, int tmax, double dT, void* stream);
-int launch_combine_sqrt(float* a, const float* b, size_t n, void* stream);
 void set_tile_scales(DevCfg& g);
 int launch_diff_topo(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, int nev, unsigned long long seed,
                      unsigned long long event0, double R0, double a0, void* stream);

@@ -294,16 +294,6 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(NACC == 1 ? 
     }
 }
 
-__global__ void k_combine_sqrt(float* a, const float* b, size_t n) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = sqrtf(a[i] * b[i]);
-}
-
-int launch_combine_sqrt(float* a, const float* b, size_t n, void* stream) {
-  hipLaunchKernelGGL(k_combine_sqrt, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a, b, n);
-  return (int)hipGetLastError();
-}
-
 int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                  bool bf16x3, void* stream) {
   if (nev <= 0) return 0;

@@ -73,10 +73,12 @@ __device__ __forceinline__ float kln_cell(float A, float B, const KlnTab& kt, bo
   return (float)(axx * x * x + axy * x * y + ayy * y * y + bx * x + by * y + v00);
 }
 
-// KLN: the sources of accumulator 0 (T_A), then those of accumulator 1 (T_B),
-// each set in source order, and the output is dN/dy.  Otherwise the sources
-// of accumulator acc_sel only.
-template <int TR, int CH, bool PAIR, bool KLN>
+// EPI 1 (MC-KLN) and 2 (sqrt(T_A T_B), MCnucl.cpp:762-811): the sources of
+// accumulator 0 (T_A), then those of accumulator 1 (T_B), each set in source
+// order, and the output is the cell's dN/dy from the table, or
+// sqrtf(T_A T_B) of the stored floats.  EPI 0: the sources of accumulator
+// acc_sel only.
+template <int TR, int CH, bool PAIR, int EPI>
 __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec spec, int acc_sel, int nev, float* out,
                                              KlnTab kt) {
   static_assert(TR % CH == 0 && CH % 4 == 0 && TR <= 64 && (!PAIR || TR <= 32), "tile rows");
@@ -107,7 +109,8 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
   const int4* swin = buf.swin + (size_t)e * buf.src_cap;
   float* rdf = reinterpret_cast<float*>(&rd[0][0]);
 
-  float acc[TR], accA[KLN ? TR : 1];  // accA: T_A while the second pass fills acc with T_B
+  constexpr bool TWO = EPI != 0;
+  float acc[TR], accA[TWO ? TR : 1];  // accA: T_A while the second pass fills acc with T_B
 #pragma unroll
   for (int r = 0; r < TR; ++r) acc[r] = 0.f;
   // tiles outside the event's source rectangle stay zero
@@ -137,15 +140,15 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
     return (col >= ylo && col < yhi) ? __builtin_amdgcn_exp2f(-(dy * dy) * cq) : 0.f;
   };
 
-  for (int pass = 0; pass < (KLN ? 2 : 1); ++pass) {
-   if (KLN && pass == 1) {
+  for (int pass = 0; pass < (TWO ? 2 : 1); ++pass) {
+   if (TWO && pass == 1) {
 #pragma unroll
     for (int r = 0; r < TR; ++r) {
-      accA[KLN ? r : 0] = acc[r];
+      accA[TWO ? r : 0] = acc[r];
       acc[r] = 0.f;
     }
    }
-   const int want = KLN ? pass : acc_sel;
+   const int want = TWO ? pass : acc_sel;
    for (int sub = 0; live && sub < K; sub += 64) {
     const int k = sub + lane;
     bool sel = false;
@@ -199,14 +202,18 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
    }
   }
   const float so = cfg.tile_out;
-  if constexpr (KLN) {  // dN/dy from (T_A, T_B) = (accA, acc) as stored floats
+  if constexpr (EPI == 2) {  // sqrt(T_A T_B) of the stored floats, as a separate combine step
+#pragma unroll
+    for (int r = 0; r < TR; ++r) acc[r] = sqrtf((accA[r] * so) * (acc[r] * so));
+  }
+  if constexpr (EPI == 1) {  // dN/dy from (T_A, T_B) = (accA, acc) as stored floats
     bool off = false;
 #pragma unroll
     for (int r = 0; r < TR; ++r)
       acc[r] = (live && col < ny && r0 + r < nx) ? kln_cell(accA[r] * so, acc[r] * so, kt, off) : 0.f;
     if (off) atomicOr(&buf.counts[(size_t)e * 8 + 3], kStKlnTable);
   }
-  const float os = KLN ? 1.f : so;  // stored value = acc * os
+  const float os = TWO ? 1.f : so;  // stored value = acc * os
   if (spec.cm_out) {  // centre-of-mass partials of the stored grid values
     double s0 = 0.0, s1 = 0.0;
 #pragma unroll
@@ -266,12 +273,16 @@ static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, in
                       hipStream_t st) {
   // one dimension: tiles x events padded to a multiple of the 8 XCDs
   const dim3 grid((unsigned)(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64)) * (unsigned)((nev + 7) / 8 * 8));
+  const KlnTab none{nullptr, 0, 1.0};
+  if (s.combine == 1) {  // sqrt(T_A T_B) in one launch, with the partials of the combined grid
+    hipLaunchKernelGGL((k_tile<TR, CH, PAIR, 2>), grid, dim3(64), 0, st, cfg, b, s, 0, nev, out0, none);
+    return;
+  }
   SmearSpec s1 = s;  // partials only for a single accumulator
   if (s.combine != 0) s1.cm_out = nullptr;
-  const KlnTab none{nullptr, 0, 1.0};
-  hipLaunchKernelGGL((k_tile<TR, CH, PAIR, false>), grid, dim3(64), 0, st, cfg, b, s1, 0, nev, out0, none);
+  hipLaunchKernelGGL((k_tile<TR, CH, PAIR, 0>), grid, dim3(64), 0, st, cfg, b, s1, 0, nev, out0, none);
   if (s.combine != 0)
-    hipLaunchKernelGGL((k_tile<TR, CH, PAIR, false>), grid, dim3(64), 0, st, cfg, b, s1, 1, nev, out1, none);
+    hipLaunchKernelGGL((k_tile<TR, CH, PAIR, 0>), grid, dim3(64), 0, st, cfg, b, s1, 1, nev, out1, none);
 }
 
 int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
@@ -281,7 +292,7 @@ int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int 
     return (int)hipErrorInvalidValue;
   const dim3 grid((unsigned)tile_units_kln(cfg) * (unsigned)((nev + 7) / 8 * 8));
   const KlnTab kt{table, tmax, dT};
-  hipLaunchKernelGGL((k_tile<kTileRows, 8, true, true>), grid, dim3(64), 0, (hipStream_t)stream, cfg, b, s, 0, nev,
+  hipLaunchKernelGGL((k_tile<kTileRows, 8, true, 1>), grid, dim3(64), 0, (hipStream_t)stream, cfg, b, s, 0, nev,
                      out, kt);
   return (int)hipGetLastError();
 }
@@ -297,7 +308,6 @@ int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev,
     case 3: launch_tr<64, 8, false>(cfg, b, s, nev, out0, out1, st); break;
     default: launch_tr<kTileRows, 8, true>(cfg, b, s, nev, out0, out1, st); break;
   }
-  if (s.combine == 1) return launch_combine_sqrt(out0, out1, (size_t)nev * cfg.nx * cfg.ny, stream);
   return (int)hipGetLastError();
 }
 
