@@ -433,7 +433,7 @@ def main():
     ap.add_argument("--jet-events", type=int, default=20000, help="operation-2 grid events per batch (0: skip)")
     ap.add_argument("--default-events", type=int, default=100000,
                     help="events per batch with the reference parameter file's defaults (0: skip)")
-    ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed", "avg", "jet"), default=None,
+    ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed", "avg", "jet", "defaults"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
@@ -460,6 +460,9 @@ def main():
         elif args.only == "avg":
             r = time_avg(args.avg_events, args.steps, args.seed)
             metric, wl = "events/s averaged rotated profiles Pb+Pb 261^2", "operation 3, orders 2..5, reaction plane"
+        elif args.only == "defaults":
+            r = time_defaults(args.default_events, args.steps, args.seed)
+            metric, wl = "events/s reference defaults Pb+Pb 5.02 TeV 301^2", "parameters.dat defaults"
         elif args.only == "jet":
             r = time_jet(args.jet_events, args.steps, args.seed)
             metric, wl = "events/s operation-2 grids Pb+Pb 261^2", "rho_binary and spectator densities"
