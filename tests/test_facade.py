@@ -162,6 +162,41 @@ def test_driver_ecc_table_matches_c_abi(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_runs_reference_default_parameters(tmp_path):
+    """operation 9 with the reference parameter file's model defaults:
+    sqrt(T_A T_B), Gaussian nucleons (shape 2), the Gaussian collision
+    criterion, orders 1..9, use_sd and use_ed.  Every order's table has one
+    row per event, the en_ tables equal the sn_ ones (setEd gives the same
+    profile here), and the values are those of the C-ABI on the same
+    parameters."""
+    _need(EXE)
+    over = dict(which_mc_model=7, sub_model=1, alpha=0.118, shape_of_nucleons=2, collision_criterion=2,
+                shape_of_entropy=2, gauss_nucl_width=0.812, gaussian_lambda=4.14, ecm="5020.", maxx="15.",
+                maxy="15.", ecc_from_order=1, ecc_to_order=9, cc_fluctuation_model=6,
+                cc_fluctuation_Gamma_theta=0.75, finalFactor="40.0", use_sd=1, use_ed=1, operation=9)
+    _write_params(tmp_path, **over)
+    r = subprocess.run([EXE, "nev=30"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", max_events=30, which_mc_model=7, alpha=0.118, shape_of_nucleons=2,
+                         collision_criterion=2, gauss_nucl_width=0.812, ecm=5020.0, maxx=15.0, maxy=15.0,
+                         ecc_from_order=1, ecc_to_order=9, cc_fluctuation_model=6,
+                         cc_fluctuation_Gamma_theta=0.75, finalFactor=40.0)
+    with Context(p) as ctx:
+        ev = ctx.run(seed=1234, event0=0, nev=30)
+    for n in range(1, 10):
+        sn = _read_table(tmp_path / "data" / f"sn_ecc_eccp_{n}.dat")
+        en = _read_table(tmp_path / "data" / f"en_ecc_eccp_{n}.dat")
+        assert sn.shape == (30, 9) and np.array_equal(sn, en)
+        np.testing.assert_allclose(sn[:, 0], ev[:, n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_allclose(sn[:, 2], ev[:, 20 + n], rtol=6e-8, atol=1e-9)
+        np.testing.assert_array_equal(sn[:, 5], ev[:, 54])
+        np.testing.assert_array_equal(sn[:, 6], ev[:, 55])
+    assert (ev[:, 55] > 0).all()
+
+
+@pytest.mark.gpu
 def test_driver_profiles_match_c_abi(tmp_path):
     """operation 1: data/sd_event_<i>_block.dat = finalFactor * s."""
     _need(EXE)
