@@ -88,12 +88,17 @@ typedef struct smc_params {
   int max_events;              /* events per batch held on the device */
   int max_pairs;               /* per-event collision-pair capacity (<=0: Aproj*Atarg) */
   int max_attempts;            /* per-event acceptance attempts */
-  int smear_path;              /* 0 exact reference footprint, separable VALU tiles [default]
-                                  1 MFMA fp32 strips: separable Gx.diag(w).Gy^T over the
-                                    square window (no r <= 5w corner cut)
-                                  2 exact reference footprint, per-cell exponentials
+  int smear_path;              /* every path applies the reference footprint (integer window
+                                  and r <= 5w, Particle.cpp:122-132) and takes every model:
+                                  0 separable VALU tiles [default]
+                                  1 FP32 MFMA: Gx.diag(w).Gy^T over the square window, then
+                                    the products outside the disk subtracted per cell
+                                  2 per-cell exponentials
                                   3 as 1 with BF16x3 split-precision MFMA (bf16 hi + lo
-                                    factors, three products, FP32 accumulation) */
+                                    factors, three products, FP32 accumulation; each term
+                                    within 3 * 2^-16 relative)
+                                  4 as 1 with BF16x6 (hi + mid + lo, six products; each
+                                    term within about 2^-22 relative) */
   double gaussian_lambda;      /* shape_of_nucleons 3: width and sigma_gg from lambda
                                   (GaussianNucleonsCal.cpp:41-46); parameter file 4.14 */
 } smc_params;

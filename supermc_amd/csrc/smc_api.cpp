@@ -154,6 +154,7 @@ struct smc_ctx {
   int cap = 0;        // max events
   int nev = 0;        // events currently held
   int have_events = 0, have_density = 0, have_thickness = 0, have_moments = 0;
+  int grid_sparse = 0;      // grid holds only the tiles inside each event's source rectangle
   int* dtab = nullptr;
   float* grid = nullptr;
   float* ta = nullptr;
@@ -289,7 +290,7 @@ void validate_and_derive(smc_ctx* c) {
   if (p.ecc_from_order < 1 || p.ecc_to_order > 9 || p.ecc_from_order > p.ecc_to_order)
     throw SmcError(SMC_ERR_ARG, "ecc orders must satisfy 1 <= from <= to <= 9");
   if (p.bmax < p.bmin || p.bmin < 0) throw SmcError(SMC_ERR_ARG, "bad impact parameter range");
-  if (p.smear_path < 0 || p.smear_path > 3) throw SmcError(SMC_ERR_ARG, "smear_path must be 0, 1, 2 or 3");
+  if (p.smear_path < 0 || p.smear_path > 4) throw SmcError(SMC_ERR_ARG, "smear_path must be 0, 1, 2, 3 or 4");
 
   // sigma_inel and width (MCnucl.cpp:58-64, 93; GaussianNucleonsCal.cpp:34-54)
   d.siginNN = (p.sigma_nn_mb > 0) ? p.sigma_nn_mb : regge_sigma_inel(p.ecm);
@@ -625,27 +626,46 @@ SmearSpec thickness_spec(const smc_ctx* c) {
   return s;
 }
 
-// smear_path: 0 exact footprint on VALU tiles (32 x 64, one wave, disk cut
-// by flushing denormal products); 1 MFMA strips Gx.diag(w).Gy^T (square
-// window, no r <= 5w corner cut; grids up to 272 wide, not for
-// sqrt(T_A T_B)); 2 exact footprint, per-cell exponentials (256-thread
-// tiles); 3 as 1 with BF16x3 split-precision MFMA.
-static bool strip_path(const smc_ctx* c, const SmearSpec& spec) {
-  const int sp = c->p.smear_path;
-  return (sp == 1 || sp == 3) && c->d.ny <= 272 && spec.combine != 1;
+// smear_path: 0 exact footprint on VALU tiles (k_tile: 32 x 64 by default,
+// one wave, disk cut by flushing denormal products); 1 the same footprint as
+// an FP32 MFMA contraction over the square window with the out-of-disk
+// products subtracted (k_mtile); 2 exact footprint, per-cell exponentials
+// (k_smear); 3 and 4 as 1 with BF16x3 and BF16x6 split-precision MFMA.
+// Every path takes every model and grid size.
+static int mfma_prec(const smc_ctx* c) {
+  switch (c->p.smear_path) {
+    case 1: return 0;
+    case 3: return 1;
+    case 4: return 2;
+    default: return -1;
+  }
+}
+
+// centre-of-mass units per event written by the density launch of the
+// current path (0: none)
+static int density_cm_units(const smc_ctx* c) {
+  if (c->p.smear_path == 2) return 0;
+  return mfma_prec(c) >= 0 ? mtile_units(c->cfg) : tile_units(c->cfg);
 }
 
 void run_smear(smc_ctx* c, const SmearSpec& spec, float* out0, float* out1) {
   TimerScope ts(c, SMC_K_SMEAR);
-  const int sp = c->p.smear_path;
-  if (strip_path(c, spec))
-    check_launch(launch_strip(c->cfg, c->buf, spec, c->nev, out0, out1, sp == 3, c->stream),
-                 sp == 3 ? "k_strip3" : "k_strip");
-  else if (sp != 2)
-    check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
-  else
+  const int prec = mfma_prec(c);
+  if (prec >= 0)
+    check_launch(launch_mtile(c->cfg, c->buf, spec, c->nev, out0, out1, prec, nullptr, 0, 0.0, c->stream), "k_mtile");
+  else if (c->p.smear_path == 2)
     check_launch(launch_smear(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_smear");
+  else
+    check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
   ts.done();
+}
+
+// Completes the density grid of a sparse launch (tiles outside an event's
+// source rectangle not stored) before whole grids are read.
+void densify_grid(smc_ctx* c) {
+  if (!c->grid_sparse) return;
+  check_launch(launch_zero_outside(c->grid, c->buf.counts, c->d.nx, c->d.ny, c->nev, c->stream), "k_zero_outside");
+  c->grid_sparse = 0;
 }
 
 void kln_points(smc_ctx* c, int n, const double* ta_dev, const double* tb_dev, double* out_dev) {
@@ -685,28 +705,76 @@ void ensure_kln_table(smc_ctx* c) {
 // from the quadrature in every cell (kln_direct).  With the table on the
 // default smearing path both steps run in one tile launch, which also
 // writes the centre-of-mass partials of the dN/dy grid.
-void run_kln_density(smc_ctx* c) {
-  c->cm_nu = 0;
-  if (!c->p.kln_direct && c->p.smear_path == 0) {
-    ensure_kln_table(c);
-    float* s = grid_buffer(c, &c->grid);
-    SmearSpec spec = thickness_spec(c);
-    spec.cm_out = c->cm;
-    TimerScope ts(c, SMC_K_SMEAR);
-    check_launch(launch_tile_kln(c->cfg, c->buf, spec, c->nev, s, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
-                                 c->stream),
-                 "k_tile (KLN)");
-    ts.done();
-    c->cm_nu = tile_units_kln(c->cfg);
+// The smearing launch of the density step on the current path and tile
+// shape -- what run_density and run_kln_density time as SMC_K_SMEAR -- with
+// no timing and no state change.  MC-KLN through the table (not on the
+// per-cell-exponential path): T_A, T_B and the table step in one launch into
+// the grid; other MC-KLN paths: T_A, T_B into their grids.
+void density_smear_launch(smc_ctx* c) {
+  const int prec = mfma_prec(c);
+  const int nu = density_cm_units(c);
+  if (c->p.which_mc_model == 1) {
+    if (!c->p.kln_direct && c->p.smear_path != 2) {
+      SmearSpec spec = thickness_spec(c);
+      spec.cm_out = c->cm;
+      spec.sparse = 1;
+      float* s = grid_buffer(c, &c->grid);
+      if (prec >= 0)
+        check_launch(launch_mtile(c->cfg, c->buf, spec, c->nev, s, nullptr, prec, c->kln_table, c->d.kln_tmax,
+                                  c->d.kln_dT, c->stream),
+                     "k_mtile (KLN)");
+      else
+        check_launch(launch_tile_kln(c->cfg, c->buf, spec, c->nev, s, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
+                                     c->stream),
+                     "k_tile (KLN)");
+      return;
+    }
+    const SmearSpec spec = thickness_spec(c);
+    float* a = grid_buffer(c, &c->ta);
+    float* b = grid_buffer(c, &c->tb);
+    if (prec >= 0) check_launch(launch_mtile(c->cfg, c->buf, spec, c->nev, a, b, prec, nullptr, 0, 0.0, c->stream), "k_mtile");
+    else if (c->p.smear_path == 2) check_launch(launch_smear(c->cfg, c->buf, spec, c->nev, a, b, c->stream), "k_smear");
+    else check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, a, b, c->stream), "k_tile");
     return;
   }
+  SmearSpec spec = density_spec(c);
+  // the tile kernels add the centre-of-mass partials of the stored grid (one
+  // accumulator, or sqrt(T_A T_B) combined in the same launch) and skip the
+  // stores of tiles outside each event's source rectangle
+  if (nu) {
+    spec.cm_out = c->cm;
+    spec.sparse = 1;
+  }
+  float* g = grid_buffer(c, &c->grid);
+  float* t = (spec.combine && !nu) ? grid_buffer(c, &c->tmp) : nullptr;
+  if (prec >= 0) check_launch(launch_mtile(c->cfg, c->buf, spec, c->nev, g, t, prec, nullptr, 0, 0.0, c->stream), "k_mtile");
+  else if (c->p.smear_path == 2) check_launch(launch_smear(c->cfg, c->buf, spec, c->nev, g, t, c->stream), "k_smear");
+  else check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, g, t, c->stream), "k_tile");
+}
+
+// MCnucl::setDensity for which_mc_model 1 (MCnucl.cpp:654-687): T_A, T_B
+// (MCnucl.cpp:398-478), then per-cell dN/dy from the table (reference) or
+// from the quadrature in every cell (kln_direct).  Through the table both
+// steps run in one tile launch, which also writes the centre-of-mass
+// partials of the dN/dy grid.
+void run_kln_density(smc_ctx* c) {
+  c->cm_nu = 0;
+  if (!c->p.kln_direct) ensure_kln_table(c);
+  {
+    TimerScope ts(c, SMC_K_SMEAR);
+    density_smear_launch(c);
+    ts.done();
+  }
+  if (!c->p.kln_direct && c->p.smear_path != 2) {
+    c->cm_nu = density_cm_units(c);
+    c->grid_sparse = 1;
+    return;
+  }
+  c->have_thickness = 1;
   float* a = grid_buffer(c, &c->ta);
   float* b = grid_buffer(c, &c->tb);
-  run_smear(c, thickness_spec(c), a, b);
-  c->have_thickness = 1;
   float* s = grid_buffer(c, &c->grid);
   const size_t ncell_ev = (size_t)c->d.nx * c->d.ny;
-  if (!c->p.kln_direct) ensure_kln_table(c);
   TimerScope ts(c, SMC_K_KLN);
   if (c->p.kln_direct) {
     check_launch(launch_kln_direct(a, b, s, ncell_ev * c->nev, c->kln_ptf, c->kln_nodes_f, c->kln_npt, c->kln_nn,
@@ -718,6 +786,7 @@ void run_kln_density(smc_ctx* c) {
                  "k_kln_table");
   }
   ts.done();
+  c->grid_sparse = 0;
 }
 
 void run_density(smc_ctx* c) {
@@ -727,13 +796,11 @@ void run_density(smc_ctx* c) {
     c->have_density = 1;
     return;
   }
-  SmearSpec spec = density_spec(c);
-  // the tile kernel adds the centre-of-mass partials of the stored grid
-  // (one accumulator, or sqrt(T_A T_B) combined in the same launch)
-  const bool tile = c->p.smear_path != 2 && !strip_path(c, spec);  // run_smear's choice
-  if (tile) spec.cm_out = c->cm;
-  run_smear(c, spec, grid_buffer(c, &c->grid), (spec.combine && !tile) ? grid_buffer(c, &c->tmp) : nullptr);
-  c->cm_nu = spec.cm_out ? tile_units(c->cfg) : 0;
+  TimerScope ts(c, SMC_K_SMEAR);
+  density_smear_launch(c);
+  ts.done();
+  c->cm_nu = density_cm_units(c);
+  c->grid_sparse = c->cm_nu ? 1 : 0;
   c->have_density = 1;
 }
 
@@ -1315,7 +1382,10 @@ smc_status smc_density(smc_ctx* c, float* s, double* dndy) {
   try {
     run_density(c);
     const size_t n = (size_t)c->nev * c->d.nx * c->d.ny;
-    if (s) HIPCHK(hipMemcpyAsync(s, c->grid, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    if (s) {
+      densify_grid(c);
+      HIPCHK(hipMemcpyAsync(s, c->grid, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    }
     int st = SMC_OK;
     if (dndy) {
       run_moments(c, 1);
@@ -1515,19 +1585,18 @@ smc_status smc_set_tile_variant(smc_ctx* c, int variant) {
   return SMC_OK;
 }
 
-// Times the exact-footprint smearing (k_tile) of the ctx's current events
-// in every tile shape, reps launches each after one untimed launch, and
-// keeps the fastest.  The grids do not depend on the shape.
+// Times the density step's smearing launch (density_smear_launch: exactly
+// what run_density launches, centre-of-mass partials and the MC-KLN table
+// step included) of the ctx's current events in every k_tile shape, reps
+// launches each after one untimed launch, and keeps the fastest.  The grids
+// do not depend on the shape.  Only the VALU tile path has shapes.
 smc_status smc_autotune(smc_ctx* c, int reps, int* best, double* ms) {
   if (!c) return SMC_ERR_ARG;
   try {
     if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_run first");
     if (reps < 1) throw SmcError(SMC_ERR_ARG, "reps must be positive");
-    const bool kln = c->p.which_mc_model == 1;
-    SmearSpec spec = kln ? thickness_spec(c) : density_spec(c);
-    float* out0 = kln ? grid_buffer(c, &c->ta) : grid_buffer(c, &c->grid);
-    float* out1 = spec.combine ? (kln ? grid_buffer(c, &c->tb) : grid_buffer(c, &c->tmp)) : nullptr;
-    if (spec.combine == 0) spec.cm_out = c->cm;
+    if (c->p.smear_path != 0) throw SmcError(SMC_ERR_UNSUPPORTED, "tile shapes exist on smear_path 0 only");
+    if (c->p.which_mc_model == 1 && !c->p.kln_direct) ensure_kln_table(c);
     hipEvent_t a, b;
     HIPCHK(hipEventCreate(&a));
     HIPCHK(hipEventCreate(&b));
@@ -1535,10 +1604,9 @@ smc_status smc_autotune(smc_ctx* c, int reps, int* best, double* ms) {
     int pick = 0;
     for (int v = 0; v < kTileVariants; ++v) {
       c->cfg.tile_variant = v;
-      check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
+      density_smear_launch(c);
       HIPCHK(hipEventRecord(a, c->stream));
-      for (int r = 0; r < reps; ++r)
-        check_launch(launch_tile(c->cfg, c->buf, spec, c->nev, out0, out1, c->stream), "k_tile");
+      for (int r = 0; r < reps; ++r) density_smear_launch(c);
       HIPCHK(hipEventRecord(b, c->stream));
       HIPCHK(hipEventSynchronize(b));
       float f = 0.f;

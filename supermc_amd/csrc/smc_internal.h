@@ -156,6 +156,9 @@ struct SmearSpec {
   int acc[3];       // accumulator (0/1) per kind
   int combine;      // 0: out0 = acc0; 1: out0 = sqrt(acc0*acc1); 2: out0=acc0,out1=acc1
   double* cm_out;   // combine 0, tile path: per-tile {sum v, sum x v, sum y v, 0} of out0, or null
+  int sparse;       // 1: tiles outside the event's source rectangle are not stored (every
+                    // cell there is zero; readers that need whole grids call
+                    // launch_zero_outside first); 0: whole grids are written
 };
 
 // Launch wrappers (smc_kernels.hip)
@@ -191,12 +194,18 @@ int launch_accum_avg(const float* grid, const int* counts, int nx, int ny, int n
 constexpr int kTileVariants = 4;
 int tile_units(const DevCfg& cfg);  // smearing tiles per event of cfg's variant (units of SmearSpec::cm_out)
 int tile_units_max(const DevCfg& cfg);  // the most over all variants (buffer sizing)
-int tile_units_kln(const DevCfg& cfg);  // of the fused MC-KLN launch (one shape)
-// bf16x3: the BF16x3 split-precision kernel instead of the FP32 one (not for combine 1)
-int launch_strip(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
-                 bool bf16x3, void* stream);
+// exact-footprint smearing as an MFMA contraction (smc_mtile.hip): prec 0
+// FP32, 1 BF16x3, 2 BF16x6; ktab (tmax x tmax) selects the fused MC-KLN
+// epilogue.  Its centre-of-mass partials have mtile_units(cfg) units (the
+// 32 x 64 tiles of k_tile's default shape).
+int mtile_units(const DevCfg& cfg);
+int launch_mtile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1, int prec,
+                 const double* ktab, int ktmax, double kdT, void* stream);
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                 void* stream);
+// zeroes every cell outside each event's source rectangle (counts[6..7]):
+// completes grids written with SmearSpec::sparse
+int launch_zero_outside(float* grid, const int* counts, int nx, int ny, int nev, void* stream);
 // 3D profiles (smc_p3d.hip), scripts/generate_3d_profiles/profile_3d.cpp
 struct P3dCfg {
   int nx, ny, neta;
@@ -217,8 +226,9 @@ int launch_p3d_smear(const P3dCfg& c, int nev, const double* src, const int* src
 double regge_sigma_inel(double ecm);
 
 // MC-KLN density in one pass: T_A (acc 0) and T_B (acc 1) of s, then dN/dy
-// from the tmax x tmax table at (T_A/dT, T_B/dT); s.cm_out gets the output's
-// centre-of-mass partials (tile_units per event) when set
+// from the tmax x tmax table at (T_A/dT, T_B/dT), in the ctx's tile shape;
+// s.cm_out gets the output's centre-of-mass partials (tile_units per event)
+// when set
 int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
                     const double* table, int tmax, double dT, void* stream);
 void set_tile_scales(DevCfg& g);

@@ -116,6 +116,10 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
   // tiles outside the event's source rectangle stay zero
   const int rx = buf.counts[(size_t)e * 8 + 6], ry = buf.counts[(size_t)e * 8 + 7];
   const bool live = (rx & 0xffff) < r0 + TR && (rx >> 16) > r0 && (ry & 0xffff) < c0 + 64 && (ry >> 16) > c0;
+  if (!live && spec.sparse) {  // nothing to store: zero centre-of-mass partials only
+    if (spec.cm_out && lane < 4) spec.cm_out[((size_t)e * T + tile) * 4 + lane] = 0.0;
+    return;
+  }
 
   auto fma_phase = [&](const float4* rr, float gy, float wq, int xw) {
     const int xlo = xw & 0xffff, xhi = xw >> 16;
@@ -202,9 +206,11 @@ __global__ __launch_bounds__(64) void k_tile(DevCfg cfg, EvBufs buf, SmearSpec s
    }
   }
   const float so = cfg.tile_out;
-  if constexpr (EPI == 2) {  // sqrt(T_A T_B) of the stored floats, as a separate combine step
+  if constexpr (EPI == 2) {
+    // sqrt(T_A T_B) of the stored floats; the product is formed in double,
+    // so the result does not depend on the f32 denormal mode of this file
 #pragma unroll
-    for (int r = 0; r < TR; ++r) acc[r] = sqrtf((accA[r] * so) * (acc[r] * so));
+    for (int r = 0; r < TR; ++r) acc[r] = (float)sqrt((double)(accA[r] * so) * (double)(acc[r] * so));
   }
   if constexpr (EPI == 1) {  // dN/dy from (T_A, T_B) = (accA, acc) as stored floats
     bool off = false;
@@ -266,7 +272,6 @@ int tile_units_max(const DevCfg& cfg) {
   for (int v = 0; v < kTileVariants; ++v) m = max(m, units_for_rows(cfg, kVariantRows[v]));
   return m;
 }
-int tile_units_kln(const DevCfg& cfg) { return units_for_rows(cfg, kTileRows); }
 
 template <int TR, int CH, bool PAIR>
 static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
@@ -285,15 +290,50 @@ static void launch_tr(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, in
     hipLaunchKernelGGL((k_tile<TR, CH, PAIR, 0>), grid, dim3(64), 0, st, cfg, b, s1, 1, nev, out1, none);
 }
 
+template <int TR, int CH, bool PAIR>
+static void launch_tr_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
+                          const KlnTab& kt, hipStream_t st) {
+  const dim3 grid((unsigned)(((cfg.nx + TR - 1) / TR) * ((cfg.ny + 63) / 64)) * (unsigned)((nev + 7) / 8 * 8));
+  hipLaunchKernelGGL((k_tile<TR, CH, PAIR, 1>), grid, dim3(64), 0, st, cfg, b, s, 0, nev, out, kt);
+}
+
+// The fused MC-KLN launch in the ctx's tile shape (the same shapes as
+// launch_tile; its centre-of-mass partials have tile_units(cfg) units).
 int launch_tile_kln(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out,
                     const double* table, int tmax, double dT, void* stream) {
   if (nev <= 0) return 0;
   if (!(cfg.tile_k > 0.f && cfg.tile_w > 0.f && cfg.tile_out > 0.f) || !table || tmax < 3 || !(dT > 0))
     return (int)hipErrorInvalidValue;
-  const dim3 grid((unsigned)tile_units_kln(cfg) * (unsigned)((nev + 7) / 8 * 8));
   const KlnTab kt{table, tmax, dT};
-  hipLaunchKernelGGL((k_tile<kTileRows, 8, true, 1>), grid, dim3(64), 0, (hipStream_t)stream, cfg, b, s, 0, nev,
-                     out, kt);
+  hipStream_t st = (hipStream_t)stream;
+  switch (variant_of(cfg)) {
+    case 1: launch_tr_kln<16, 8, true>(cfg, b, s, nev, out, kt, st); break;
+    case 2: launch_tr_kln<32, 4, true>(cfg, b, s, nev, out, kt, st); break;
+    case 3: launch_tr_kln<64, 8, false>(cfg, b, s, nev, out, kt, st); break;
+    default: launch_tr_kln<kTileRows, 8, true>(cfg, b, s, nev, out, kt, st); break;
+  }
+  return (int)hipGetLastError();
+}
+
+// Cells outside each event's source rectangle set to zero (the grid of a
+// sparse launch, completed before whole grids are read).
+__global__ __launch_bounds__(256) void k_zero_outside(float* grid, const int* counts, int nx, int ny) {
+  const int e = blockIdx.y;
+  const int rx = counts[(size_t)e * 8 + 6], ry = counts[(size_t)e * 8 + 7];
+  const int x0 = rx & 0xffff, x1 = rx >> 16, y0 = ry & 0xffff, y1 = ry >> 16;
+  float* g = grid + (size_t)e * nx * ny;
+  const int n = nx * ny;
+  for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    const int ix = i / ny, iy = i - ix * ny;
+    if (ix < x0 || ix >= x1 || iy < y0 || iy >= y1) g[i] = 0.f;
+  }
+}
+
+int launch_zero_outside(float* g, const int* counts, int nx, int ny, int nev, void* stream) {
+  if (nev <= 0) return 0;
+  const int per = (nx * ny + 255) / 256;
+  const dim3 grid((unsigned)(per < 64 ? per : 64), (unsigned)nev);
+  hipLaunchKernelGGL(k_zero_outside, grid, dim3(256), 0, (hipStream_t)stream, g, counts, nx, ny);
   return (int)hipGetLastError();
 }
 

@@ -45,12 +45,18 @@ def cpu_configs(p, nev, seed, bvals=None):
     return np.round(xa / Q) * Q, np.round(xb / Q) * Q, bs
 
 
-def oracle_event(ctx, ev, kln_table=None):
+def oracle_event(ctx, ev, kln_table=None, sizes=None):
     """Runs the oracle on the nucleons (and fluct factors) of event ev as
     reported by the HIP path.  Returns a dict with the oracle collision
-    result, densities and moments, plus the GPU lists for comparison."""
+    result, densities and moments, plus the GPU lists for comparison.
+    sizes = (A, B): the event's nucleus sizes when they are smaller than
+    the context's (an event of a mixed batch)."""
     p, d = ctx.params, ctx.derived
     nuc = ctx.nucleons(ev)
+    if sizes is not None:
+        for side, n in zip(("proj", "targ"), sizes):
+            for key in ("xy_", "wound_", "fluct_"):
+                nuc[key + side] = nuc[key + side][:n]
     pa, pb, pf = ctx.pairs(ev)
     xa, xb = nuc["xy_proj"], nuc["xy_targ"]
     w = d.width
