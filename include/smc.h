@@ -188,6 +188,25 @@ smc_status smc_get_nucleons(smc_ctx* ctx, int ev, double* xy_proj, int* wound_pr
 smc_status smc_get_pairs(smc_ctx* ctx, int ev, int cap, int* pair_proj, int* pair_targ,
                          float* fluct, int* npairs);
 
+/* The sources of event ev as MCnucl holds them (Nucleus::getParticipants
+ * and binaryCollision, read by MCnucl::getHotSpots, MCnucl.cpp:1303-1323):
+ * projectile participants in wound order, then target participants in
+ * wound order, then the collision pairs in reference order, at their
+ * current positions (after smc_recenter: moved and turned).  xy [cap][2] fm,
+ * kind 0 projectile / 1 target participant / 2 pair, weight = fluct factor
+ * (any pointer may be NULL); nsrc = the event's source count.  Returns
+ * SMC_ERR_CAPACITY (after filling cap entries) when cap < nsrc. */
+smc_status smc_get_sources(smc_ctx* ctx, int ev, int cap, double* xy, int* kind, float* weight, int* nsrc);
+
+/* MCnucl::recenterGrid (turn = 0) and MCnucl::rotateGrid (turn = 1)
+ * (MCnucl.cpp:1119-1174) for every event of the batch: the sources move by
+ * minus the centre of mass of the current density and, with turn, rotate by
+ * the order-n angle -atan2(-Im, -Re)/n of sum r^n e^{i n theta} rho
+ * (GlueDensity::calcCMAngle, GlueDensity.cpp:87-144).  The next density
+ * call smears the moved sources.  rot: host [nev][4] = {xc, yc, angle,
+ * sum rho} (NULL allowed).  1 <= n <= 9. */
+smc_status smc_recenter(smc_ctx* ctx, int n, int turn, double* rot);
+
 /* MCnucl::calculateThickness (MCnucl.cpp:398-478).  TA/TB host [nev][nx][ny]
  * (NULL: keep on device only). */
 smc_status smc_thickness(smc_ctx* ctx, float* TA, float* TB);

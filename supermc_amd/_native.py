@@ -103,6 +103,8 @@ SIGNATURES = {
     "smc_collide": (C.c_int, [C.c_void_p, _ip, _ip, _ip]),
     "smc_get_nucleons": (C.c_int, [C.c_void_p, C.c_int, _dp, _ip, _fp, _dp, _ip, _fp]),
     "smc_get_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _fp, _ip]),
+    "smc_get_sources": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _fp, _ip]),
+    "smc_recenter": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "smc_thickness": (C.c_int, [C.c_void_p, _fp, _fp]),
     "smc_density": (C.c_int, [C.c_void_p, _fp, _dp]),
     "smc_moments": (C.c_int, [C.c_void_p, _dp]),
@@ -344,6 +346,24 @@ class Context:
         b = np.zeros(shape, np.float32) if host else None
         self._check(self.lib.smc_spectator_density(self.h, _ptr(a, C.c_float), _ptr(b, C.c_float)))
         return a, b
+
+    def sources(self, ev):
+        """Event ev's sources (smc_get_sources): xy [n][2] fm, kind (0/1
+        participant of projectile/target in wound order, 2 pair), weight."""
+        n = C.c_int()
+        self._check(self.lib.smc_get_sources(self.h, ev, 0, None, None, None, C.byref(n)), allow=(SMC_ERR_CAPACITY,))
+        k = n.value
+        xy, kind, w = np.zeros((max(k, 1), 2)), np.zeros(max(k, 1), np.int32), np.zeros(max(k, 1), np.float32)
+        self._check(self.lib.smc_get_sources(self.h, ev, k, _ptr(xy, C.c_double), _ptr(kind, C.c_int),
+                                             _ptr(w, C.c_float), C.byref(n)))
+        return xy[:k], kind[:k], w[:k]
+
+    def recenter(self, n, turn):
+        """MCnucl::recenterGrid (turn False) / rotateGrid (turn True) for the
+        batch (smc_recenter); returns {xc, yc, angle, sum rho} per event."""
+        rot = np.zeros((self.nev, 4))
+        self._check(self.lib.smc_recenter(self.h, n, 1 if turn else 0, _ptr(rot, C.c_double)))
+        return rot
 
     def average_profiles(self, from_order, to_order, accept=None, reaction_plane=False):
         """Sums over the ctx's accepted events of finalFactor * rho after the

@@ -378,6 +378,9 @@ void validate_and_derive(smc_ctx* c) {
   // KLN table step (MCnucl.cpp:915-919)
   d.kln_dT = 10.0 / d.siginNN / 1 / std::max(1, p.tmax_subdivision);
   d.kln_tmax = std::max(1, p.tmax_subdivision) * (p.tmax - 1) + 1;
+  // sixPoint2dInterp reads rows and columns i .. i + 2 (arsenal.cpp:33-54)
+  if (p.which_mc_model == 1 && !p.kln_direct && d.kln_tmax < 3)
+    throw SmcError(SMC_ERR_ARG, "the dN/dy table needs tmax_subdivision * (tmax - 1) + 1 >= 3");
 
   DevCfg& g = c->cfg;
   g.dmin2 = (float)(d.dmin * d.dmin);
@@ -1357,6 +1360,76 @@ smc_status smc_get_pairs(smc_ctx* c, int ev, int cap, int* pair_proj, int* pair_
       }
     }
     if (cnt[2] > cap) { c->err = "pair buffer too small"; return SMC_ERR_CAPACITY; }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+// The event's source list: participants of the projectile and of the target
+// in wound order, then the collision pairs in reference order (the order of
+// getHotSpots, MCnucl.cpp:1303-1323) at their current positions.
+smc_status smc_get_sources(smc_ctx* c, int ev, int cap, double* xy, int* kind, float* weight, int* nsrc) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events || ev < 0 || ev >= c->nev) throw SmcError(SMC_ERR_ARG, "event index out of range");
+    int cnt[8];
+    HIPCHK(hipMemcpyAsync(cnt, c->buf.counts + (size_t)ev * 8, sizeof(cnt), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
+    const int K = cnt[5];
+    if (nsrc) *nsrc = K;
+    const int m = std::min(K, std::max(cap, 0));
+    if (m > 0) {
+      std::vector<int4> sv(m);
+      HIPCHK(hipMemcpyAsync(sv.data(), c->buf.src + (size_t)ev * c->buf.src_cap, m * sizeof(int4),
+                            hipMemcpyDeviceToHost, c->stream));
+      sync(c);
+      for (int k = 0; k < m; ++k) {
+        if (xy) {
+          xy[2 * k] = sv[k].x * kH;
+          xy[2 * k + 1] = sv[k].y * kH;
+        }
+        if (kind) kind[k] = sv[k].w;
+        if (weight) {
+          float f;
+          std::memcpy(&f, &sv[k].z, sizeof f);
+          weight[k] = f;
+        }
+      }
+    }
+    if (K > cap) {
+      c->err = "source buffer too small";
+      return SMC_ERR_CAPACITY;
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+// MCnucl::recenterGrid / rotateGrid (MCnucl.cpp:1119-1174) for every event
+// of the batch: the centre of mass of the current density and, with turn,
+// the order-n angle -atan2(-Im, -Re) / n of sum r^n e^{in theta} rho
+// (GlueDensity::calcCMAngle, GlueDensity.cpp:87-144); the sources move by
+// -centre and turn by the angle.  The next density call smears the moved
+// sources.  rot: host [nev][4] {xc, yc, angle, sum rho} (NULL allowed).
+smc_status smc_recenter(smc_ctx* c, int n, int turn, double* rot) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_run first");
+    if (n < 1 || n > 9) throw SmcError(SMC_ERR_ARG, "order must satisfy 1 <= n <= 9");
+    if (!c->avg_rot) {
+      c->avg_rot = dalloc<double>((size_t)c->cap * 4);
+      c->avg_acc = dalloc<unsigned char>((size_t)c->cap);
+      c->avg_part = dalloc<double>((size_t)kAccumSegments * c->d.nx * c->d.ny);
+    }
+    if (!c->have_density) run_density(c);
+    ensure_cm(c);
+    check_launch(launch_cm_angle(c->cfg, c->buf, c->grid, c->cm, c->cm_nu, n, turn ? 1 : 0, c->nev, c->avg_rot,
+                                 c->stream),
+                 "k_cm_angle");
+    check_launch(launch_rotate_src(c->cfg, c->buf, c->avg_rot, c->nev, c->stream), "k_rotate_src");
+    c->have_density = c->have_thickness = c->have_moments = 0;
+    c->cm_nu = 0;
+    if (rot)
+      HIPCHK(hipMemcpyAsync(rot, c->avg_rot, (size_t)c->nev * 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    sync(c);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }

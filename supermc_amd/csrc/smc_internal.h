@@ -109,7 +109,6 @@ struct EvBufs {
 constexpr int kStOk = 0;
 constexpr int kStNoEvent = 1;
 constexpr int kStPairOverflow = 2;
-constexpr int kStPosRange = 4;
 constexpr int kStKlnTable = 8;
 
 // Differentiable batches (frozen topology): device buffers of a context

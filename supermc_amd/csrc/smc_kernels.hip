@@ -421,7 +421,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   }
   const int nsrc_part = npa + npb;
   int4* src = buf.src + (size_t)e * buf.src_cap;
-  const bool src_ok = nsrc_part + ncoll <= buf.src_cap;
+  // pairs past max_pairs are not stored (status bit 2): the source list
+  // holds the participants and the stored pairs only
+  const int npairs_st = min(ncoll, cfg.max_pairs);
+  const bool src_ok = nsrc_part + npairs_st <= buf.src_cap;
   int4* swin = buf.swin + (size_t)e * buf.src_cap;
   // bounding rectangle of every source window: all grid cells outside it
   // are zero for every density of this event
@@ -511,7 +514,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     c[2] = ncoll;
     c[3] = status;
     c[4] = attempt + 1;
-    c[5] = src_ok ? nsrc_part + ncoll : 0;
+    c[5] = src_ok ? nsrc_part + npairs_st : 0;
     const bool none = s_bb[0] >= s_bb[1];
     c[6] = none ? 0 : (s_bb[0] | (s_bb[1] << 16));
     c[7] = none ? 0 : (s_bb[2] | (s_bb[3] << 16));

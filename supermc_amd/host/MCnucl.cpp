@@ -1,5 +1,6 @@
 #include "MCnucl.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <iostream>
 
@@ -15,6 +16,7 @@ MCnucl::MCnucl(ParameterReader* paraRdr) {
   check(ctx_, smc_get_derived(ctx_, &d), "MCnucl::MCnucl");
   nx_ = d.nx;
   ny_ = d.ny;
+  box_half_ = 8 * std::max(d.width, p_.quark_width) / 2;
   dx_ = p_.dx;
   dy_ = p_.dy;
   npmin_ = p_.Npmin;
@@ -119,4 +121,42 @@ void MCnucl::getNucleons(std::vector<double>& xy_proj, std::vector<int>& wound_p
         smc_get_nucleons(ctx_, 0, xy_proj.data(), wound_proj.data(), fp.data(), xy_targ.data(), wound_targ.data(),
                          ft.data()),
         "MCnucl::getNucleons");
+}
+
+// MCnucl.cpp:1303-1323.  A participant's box is its 8 w square (with the
+// quarks at the nucleon centre, Particle.cpp:16-21, 94-99) about its
+// current position; a collision pair contributes the all-zero box.
+Box2D MCnucl::getHotSpots(std::vector<Box2D>& hotSpots) {
+  hotSpots.clear();
+  int n = 0;
+  const smc_status st = smc_get_sources(ctx_, 0, 0, nullptr, nullptr, nullptr, &n);  // the count only
+  if (st != SMC_ERR_CAPACITY) check(ctx_, st, "MCnucl::getHotSpots");
+  std::vector<double> xy(2 * (size_t)std::max(n, 1));
+  std::vector<int> kind(std::max(n, 1));
+  check(ctx_, smc_get_sources(ctx_, 0, n, xy.data(), kind.data(), nullptr, &n), "MCnucl::getHotSpots");
+  for (int k = 0; k < n; ++k) {
+    Box2D b;
+    if (kind[k] != 2) {  // baseBox.setCenter(x, y) on the zero box, then setSquareDimensions(8 w)
+      b.setCenter(xy[2 * k], xy[2 * k + 1]);
+      b.setSquareDimensions(2 * box_half_);
+    }
+    hotSpots.push_back(b);
+  }
+  if (hotSpots.empty()) {
+    std::cerr << "MCnucl::getHotSpots: the event has no sources" << std::endl;
+    std::exit(1);
+  }
+  Box2D u = hotSpots[0];
+  for (size_t i = 1; i < hotSpots.size(); ++i) u.overUnion(hotSpots[i]);
+  return u;
+}
+
+void MCnucl::recenterGrid(int iy, int n) {
+  (void)iy;
+  check(ctx_, smc_recenter(ctx_, n, 0, nullptr), "MCnucl::recenterGrid");
+}
+
+void MCnucl::rotateGrid(int iy, int n) {
+  (void)iy;
+  check(ctx_, smc_recenter(ctx_, n, 1, nullptr), "MCnucl::rotateGrid");
 }

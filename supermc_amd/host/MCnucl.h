@@ -8,7 +8,9 @@
 //    the device; getBinaryCollision() returns that event's Ncoll.
 //  - setDensity(iy, ipt) supports ipt < 0 (pT-integrated) and iy = 0.
 //  - KLN (which_mc_model 1) uses the library's dN/dy table, so setKLN /
-//    setLgX are not needed and getHotSpots is not offered.
+//    setLgX are not needed.
+//  - recenterGrid / rotateGrid move the participants and collision pairs
+//    (the density's sources); spectators keep their positions.
 //  - lastCx1 .. lastPh2 are 0 (the reference never assigns them).
 #ifndef SMC_HOST_MCNUCL_H
 #define SMC_HOST_MCNUCL_H
@@ -17,6 +19,7 @@
 #include <string>
 #include <vector>
 
+#include "Box2D.h"
 #include "ParameterReader.h"
 #include "smc.h"
 
@@ -46,6 +49,15 @@ class MCnucl {
   int CentralityCut();
   void setCentralityCut(int Nmin, int Nmax) { npmax_ = Nmax; npmin_ = Nmin; }
   void makeTable();
+  // MCnucl.cpp:1303-1323: the participant bounding boxes (projectile, then
+  // target, in wound order) and the collision pairs' boxes (a CollisionPair
+  // never sets its box: all edges 0), and their union
+  Box2D getHotSpots(std::vector<Box2D>& hotSpots);
+  // MCnucl.cpp:1119-1174: move the sources by minus the density's centre of
+  // mass; rotateGrid also turns them by the order-n angle.  Call setDensity
+  // again for the moved profile.
+  void recenterGrid(int iy, int n = 2);
+  void rotateGrid(int iy, int n = 2);
 
   // MCnucl.cpp:1177-1220: collision tables of the current event (formats of
   // the reference; quarks at their nucleon's centre)
@@ -73,6 +85,7 @@ class MCnucl {
   uint64_t seed_ = 0, counter_ = 0;
   int nx_ = 0, ny_ = 0;
   double dx_ = 0, dy_ = 0;
+  double box_half_ = 0;  // half the participant box: 4 max(width, quark_width) (Particle.cpp:16-21, 94-99)
   int npart1_ = 0, npart2_ = 0, ncoll_ = 0, npmin_ = 0, npmax_ = 0;
   double dndy_ = 0;
   bool have_event_ = false;

@@ -1,6 +1,7 @@
 // facade_check -- drives the reference-interface facade for the tests.
 //   facade_check reader <params-file> [key=value ...]   parse + print names/values (no GPU)
 //   facade_check mcnucl <params-file> b0 b1 ...         one event per b through MCnucl
+//   facade_check hotspots <params-file> b0 b1 ...       MCnucl::getHotSpots, then after rotateGrid
 //   facade_check kln <ecm> <lambda> ta0 tb0 ta1 tb1 ... KLNModel::getdNdy per point
 //   facade_check p3d <parts> <bins> nx ny neta dx deta ecm random_flag alpha <out>
 //                                                       profile_3d: both grids, entropy file
@@ -74,6 +75,33 @@ int main(int argc, char* argv[]) {
     std::vector<double> tab{-1.0, 0.0, 0.5, 2.0};
     std::printf("search %ld %ld %ld\n", p.binarySearch(&tab, 0.25, true), p.binarySearch(&tab, 3.0, true),
                 p.binarySearch(&tab, -1.0, true));
+    return 0;
+  }
+  if (!std::strcmp(mode, "hotspots")) {  // MCnucl::getHotSpots before and after rotateGrid
+    ParameterReader r;
+    r.readFromFile(argv[2]);
+    MCnucl mc(&r);
+    for (int i = 3; i < argc; ++i) {
+      mc.generateNuclei(std::atof(argv[i]));
+      const int nc = mc.getBinaryCollision();
+      std::vector<double> xa, xb;
+      std::vector<int> wa, wb;
+      mc.getNucleons(xa, wa, xb, wb);
+      std::printf("E %d %d\n", i - 3, nc);
+      for (size_t k = 0; k < wa.size(); ++k) std::printf("N 0 %d %.17g %.17g\n", wa[k], xa[2 * k], xa[2 * k + 1]);
+      for (size_t k = 0; k < wb.size(); ++k) std::printf("N 1 %d %.17g %.17g\n", wb[k], xb[2 * k], xb[2 * k + 1]);
+      if (nc == 0) continue;
+      for (const char* tag : {"B", "R"}) {
+        if (tag[0] == 'R') {
+          mc.setDensity(0, -1);
+          mc.rotateGrid(0, 2);
+        }
+        std::vector<Box2D> hs;
+        const Box2D u = mc.getHotSpots(hs);
+        for (const Box2D& b : hs) std::printf("%s %.17g %.17g %.17g %.17g\n", tag, b.getXL(), b.getXR(), b.getYL(), b.getYR());
+        std::printf("U%s %.17g %.17g %.17g %.17g\n", tag, u.getXL(), u.getXR(), u.getYL(), u.getYR());
+      }
+    }
     return 0;
   }
   if (!std::strcmp(mode, "kln")) {

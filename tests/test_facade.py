@@ -378,6 +378,53 @@ def test_mcnucl_facade_matches_batched_path(tmp_path):
 
 
 @pytest.mark.gpu
+def test_mcnucl_get_hot_spots(tmp_path):
+    """MCnucl::getHotSpots (MCnucl.cpp:1303-1323) through the facade: one box
+    per participant, projectile then target in wound order, each the
+    reference's Particle box (Particle.cpp:16-21: the zero box moved to the
+    nucleon, then made an 8 w square; oracle.particle_box, pinned to the
+    compiled Box2D), then one all-zero box per collision pair, and their
+    union.  After rotateGrid the participant boxes are 8 w squares about the
+    moved sources and the pairs' boxes stay zero."""
+    import oracle
+    _need(CHECK)
+    pfile = _write_params(tmp_path)
+    r = subprocess.run([CHECK, "hotspots", pfile, "0", "6.5", "12.0"], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    w = Context(H.glauber_params("PbPb", ecm=2760.0, max_events=1)).derived.width
+    ev = []
+    for ln in r.stdout.splitlines():
+        f = ln.split()
+        if f[0] == "E":
+            ev.append(dict(nc=int(f[2]), N=[], B=[], R=[], UB=None, UR=None))
+        elif f[0] == "N":
+            ev[-1]["N"].append((int(f[1]), int(f[2]), float(f[3]), float(f[4])))
+        elif f[0] in ("B", "R"):
+            ev[-1][f[0]].append([float(v) for v in f[1:]])
+        else:
+            ev[-1][f[0]] = [float(v) for v in f[1:]]
+    checked = 0
+    for e in ev:
+        if e["nc"] == 0:
+            continue
+        parts = sorted([n for n in e["N"] if n[1] >= 0], key=lambda n: (n[0], n[1]))
+        want = [list(oracle.particle_box(x, y, w)) for _, _, x, y in parts] + [[0.0] * 4] * e["nc"]
+        assert e["B"] == want
+        u = np.array(want)
+        assert e["UB"] == [u[:, 0].min(), u[:, 1].max(), u[:, 2].min(), u[:, 3].max()]
+        rb = np.array(e["R"])
+        assert len(rb) == len(want)
+        k = len(parts)
+        assert np.allclose(rb[:k, 1] - rb[:k, 0], 8 * w, atol=1e-12)
+        assert np.allclose(rb[:k, 3] - rb[:k, 2], 8 * w, atol=1e-12)
+        assert (rb[k:] == 0).all()
+        checked += 1
+    assert checked == 2
+
+
+@pytest.mark.gpu
 def test_klnmodel_facade_matches_oracle():
     _need(CHECK)
     import oracle

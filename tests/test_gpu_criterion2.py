@@ -117,3 +117,63 @@ def test_sampled_minbias_with_reference_defaults():
     assert np.isfinite(rec[:, :50]).all()
     # minimum bias over b <= 20 fm: central and peripheral events both present
     assert npart.max() > 350 and np.median(npart) < 150
+
+
+def _table_npart(npart, cent):
+    """The Npart column of the reference's centrality tables for sampled
+    events (scripts/centrality_cut_h5.py:43-89): events sorted by Npart,
+    descending; the row of upper edge c holds the smallest Npart among the
+    int(N (c - c_prev) / 100) - 1 events from rank int(N c_prev / 100)."""
+    s = np.sort(npart)[::-1]
+    n = len(s)
+    out = []
+    for lo, hi in zip(cent[:-1], cent[1:]):
+        off, cnt = int(n * lo / 100), int(n * (hi - lo) / 100) - 1
+        out.append(s[off:off + cnt].min())
+    return np.array(out)
+
+
+@pytest.mark.parametrize("ecm", [5020, 2760])
+def test_sampled_npart_centrality_matches_reference_tables(ecm):
+    """Npart at the centrality edges of minimum-bias Pb+Pb with the
+    reference parameter file (parameters.dat, the configuration the
+    reference's tables were made with: operation 9, bmin 0, bmax 20, Gaussian
+    collision criterion, d_min 0.9 fm, Regge sigma_NN) against
+    scripts/centrality_cut_tables/iebe_centralityCut_Npart_MCGlbPbPb<ecm>_
+    sigmaNN_gauss_d0.9_withMultFluct.dat (tests/golden/centrality_npart.json).
+
+    Tolerance per edge c: the order statistic at rank r = N c / 100 has a
+    binomial spread sqrt(N q (1 - q)) in rank (q = c / 100), i.e. sigma =
+    half the Npart difference between ranks r -+ that spread in the sorted
+    sample; the table, from its own finite sample, is taken to be no more
+    precise, so the bar is 5 sqrt(2) sigma plus 1 for the integer Npart."""
+    import json
+    import os
+    from supermc_amd import Context, default_params
+    tab = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "centrality_npart.json")))[str(ecm)]
+    rows = np.array(tab["rows"])
+    cent, ref = rows[:, 0], rows[1:, 1]
+    nb, batches = 25000, 4
+    p = default_params(ecm=float(ecm), max_events=nb, max_pairs=4096)
+    npart = []
+    with Context(p) as ctx:
+        for k in range(batches):
+            ctx.sample(seed=20261203 + ecm, event0=k * nb, nev=nb)
+            a, b, nc, _ = ctx.collide()
+            assert (nc > 0).all()
+            npart.append(a + b)
+    npart = np.concatenate(npart).astype(float)
+    n = len(npart)
+    got = _table_npart(npart, cent)
+    s = np.sort(npart)[::-1]
+    checked = 0
+    for i, c in enumerate(cent[1:]):
+        if c < 1.0 or c > 95.0:
+            continue  # the 0.1 % rows hold ~100 events; the last rows are Npart = 2
+        q = c / 100
+        r, k = int(n * q), int(np.ceil(np.sqrt(n * q * (1 - q))))
+        sigma = 0.5 * (s[max(r - k, 0)] - s[min(r + k, n - 1)])
+        tol = 5 * np.sqrt(2) * sigma + 1.0
+        assert abs(got[i] - ref[i]) <= tol, (ecm, c, got[i], ref[i], tol)
+        checked += 1
+    assert checked >= 90

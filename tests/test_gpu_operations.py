@@ -145,3 +145,39 @@ def test_rho_binary_and_spectators_match_oracle():
         all_a = oracle.thickness(g, w, nuc["xy_proj"])
         ta, _ = ctx.thickness()
         assert _rel(spa[e] + ta[e], all_a) <= TOL
+
+
+@pytest.mark.parametrize("turn", [False, True])
+def test_recenter_and_rotate_sources_match_oracle(turn):
+    """smc_recenter (MCnucl::recenterGrid / rotateGrid, MCnucl.cpp:1119-1174):
+    the centre of mass and order-2 angle of the current density against
+    the oracle's GlueDensity::calcCMAngle on the oracle density of the same
+    event, and every source moved to R(angle) (x - c), rounded to the
+    half-quantum; the re-smeared density then has its centre at the origin
+    (to the grid's discretisation) and, turned, an order-2 angle of 0."""
+    import oracle
+    from supermc_amd import Context
+    p = H.glauber_params("PbPb", max_events=6)
+    with Context(p) as ctx:
+        ctx.run(seed=404, event0=0, nev=6)
+        before = [ctx.sources(e) for e in range(6)]
+        ref = [H.oracle_event(ctx, e) for e in range(6)]
+        rot = ctx.recenter(2, turn)
+        after = [ctx.sources(e) for e in range(6)]
+        s2, _, _ = ctx.density()
+        g = ref[0]["grid"]
+        for e in range(6):
+            xc, yc, ang = oracle.cm_angle(g, ref[e]["s"], 2)
+            assert abs(rot[e, 0] - xc) <= 1e-6 and abs(rot[e, 1] - yc) <= 1e-6
+            assert abs(rot[e, 2] - (ang if turn else 0.0)) <= 1e-6
+            (xy0, k0, w0), (xy1, k1, w1) = before[e], after[e]
+            assert np.array_equal(k0, k1) and np.array_equal(w0, w1)
+            phi = rot[e, 2]
+            d = xy0 - rot[e, :2]
+            want = np.column_stack([np.cos(phi) * d[:, 0] - np.sin(phi) * d[:, 1],
+                                    np.sin(phi) * d[:, 0] + np.cos(phi) * d[:, 1]])
+            assert np.abs(xy1 - want).max() <= 2.0 ** -25
+            xc2, yc2, ang2 = oracle.cm_angle(g, s2[e].astype(np.float64), 2)
+            assert abs(xc2) < 1e-3 and abs(yc2) < 1e-3
+            if turn:
+                assert abs(np.angle(np.exp(2j * ang2))) < 1e-3
