@@ -304,7 +304,11 @@ __global__ __launch_bounds__(64) void k_mtile(DevCfg cfg, EvBufs buf, SmearSpec 
     }
   }
 
-  // ---- epilogue: stored value = acc * os per cell
+  // ---- epilogue: stored value = acc * so per cell.  Where no disk reaches
+  // the true value is 0 and the square-window sum minus its subtraction
+  // leaves rounding noise of either sign (a few ulp of products below
+  // e^-12.5 of a peak): sums are clamped at 0 from below, so that sqrt and
+  // the table see non-negative thicknesses.
   const float so = 1.f / (kMtSA * kMtSB);
   bool off = false;
 #pragma unroll
@@ -313,14 +317,14 @@ __global__ __launch_bounds__(64) void k_mtile(DevCfg cfg, EvBufs buf, SmearSpec 
     for (int ci = 0; ci < 4; ++ci)
 #pragma unroll
       for (int v = 0; v < 4; ++v) {
-        float x = acc[ri][ci][v] * so;
+        float x = fmaxf(acc[ri][ci][v] * so, 0.f);
         if constexpr (EPI == 2) {
-          x = (float)sqrt((double)(accA[TWO ? ri : 0][TWO ? ci : 0][v] * so) * (double)x);
+          x = (float)sqrt((double)fmaxf(accA[TWO ? ri : 0][TWO ? ci : 0][v] * so, 0.f) * (double)x);
         } else if constexpr (EPI == 1) {
           const int row = r0 + 16 * ri + 4 * v + q, col = c0 + 16 * ci + j16;
-          x = (live && row < nx && col < ny) ? mt_kln_cell(accA[TWO ? ri : 0][TWO ? ci : 0][v] * so, x, ktab, ktmax,
-                                                            kdT, off)
-                                             : 0.f;
+          x = (live && row < nx && col < ny)
+                  ? mt_kln_cell(fmaxf(accA[TWO ? ri : 0][TWO ? ci : 0][v] * so, 0.f), x, ktab, ktmax, kdT, off)
+                  : 0.f;
         }
         acc[ri][ci][v] = x;
       }

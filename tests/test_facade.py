@@ -421,7 +421,7 @@ def test_mcnucl_get_hot_spots(tmp_path):
         assert np.allclose(rb[:k, 3] - rb[:k, 2], 8 * w, atol=1e-12)
         assert (rb[k:] == 0).all()
         checked += 1
-    assert checked == 2
+    assert checked >= 2
 
 
 @pytest.mark.gpu

@@ -393,3 +393,183 @@ def test_p3d_smear_is_separable_and_clamped(orc):
                                          rel=1e-13)
     edge = orc.p3d_smear(61, 61, 21, 0.1, 0.1, 0.1, [[2.95, 0.0, 0.0, 0.3, 0.3, 0.5]])
     assert edge[:, 60, :].max() > 0 and 0.3 < edge.sum() * 1e-3 < 0.7
+
+
+# ---------------------------------------------------------------- deformed nuclei
+
+def test_ylm_constants_are_the_normalised_spherical_harmonics(orc):
+    """Y20 and Y40 of Nucleus.cpp:669-694 against the textbook closed forms
+    sqrt(5 / 16 pi) (3 c^2 - 1) and 3 / (16 sqrt(pi)) (35 c^4 - 30 c^2 + 3),
+    and their orthonormality on the sphere (quadrature)."""
+    from scipy import integrate
+    for c in (-1.0, -0.73, -0.2, 0.0, 0.31, 0.5, 0.9, 1.0):
+        assert orc.lib().orc_ylm(2, c) == pytest.approx(math.sqrt(5 / (16 * math.pi)) * (3 * c * c - 1), rel=1e-15,
+                                                      abs=1e-16)
+        assert orc.lib().orc_ylm(4, c) == pytest.approx(
+            3 / (16 * math.sqrt(math.pi)) * (35 * c ** 4 - 30 * c * c + 3), rel=1e-15, abs=1e-16)
+    y = lambda l, c: orc.lib().orc_ylm(l, c)
+    for l1, l2, want in ((2, 2, 1.0), (4, 4, 1.0), (2, 4, 0.0)):
+        val = 2 * math.pi * integrate.quad(lambda c: y(l1, c) * y(l2, c), -1, 1, epsabs=1e-13)[0]
+        assert val == pytest.approx(want, abs=1e-12)
+
+
+def _deformed_samples(orc, n, R, a, b2, b4, seed=11):
+    import ctypes as C
+    rng = orc.Rng(seed)
+    x, y, z = C.c_double(), C.c_double(), C.c_double()
+    out = np.zeros((n, 3))
+    for i in range(n):
+        # orientation cos = 1, phi = 0: Point3D::rotate is the identity, so the
+        # sample is in the body frame
+        orc.lib().orc_sample_ws(C.byref(rng.r), R, a, b2, b4, 1.0, 0.0, 1, C.byref(x), C.byref(y), C.byref(z))
+        out[i] = x.value, y.value, z.value
+    return out
+
+
+def test_deformed_ws_sampler_law(orc):
+    """The deformed Woods-Saxon sampler (Nucleus.cpp:578-621) against its
+    law, written from the acceptance rule: proposals uniform in the ball of
+    radius R + 2.5 and in c = cos(theta), accepted with probability
+    f(r; R(c)) / f(0; R(c)), f(r; R) = 1 / (1 + exp((r - R) / a)),
+    R(c) = R (1 + beta2 Y20(c) + beta4 Y40(c)).  So the body-frame joint
+    density is p(r, c) ~ r^2 f(r; R(c)) / f(0; R(c)).  KS of c against its
+    marginal, of r within four bins of c against the conditional law, and
+    of the azimuth against the uniform law (U: R 6.81, a 0.6, beta2 0.28,
+    beta4 0.093; a sign or constant error in R(c) moves the polar radius by
+    about a fermi)."""
+    from scipy import stats
+    R, a, b2, b4 = 6.81, 0.6, 0.28, 0.093
+    rmax = R + 2.5
+    s = _deformed_samples(orc, 40000, R, a, b2, b4)
+    r = np.linalg.norm(s, axis=1)
+    c = s[:, 2] / r
+    phi = np.arctan2(s[:, 1], s[:, 0])
+    y20 = lambda c: math.sqrt(5 / (16 * math.pi)) * (3 * c * c - 1)
+    y40 = lambda c: 3 / (16 * math.sqrt(math.pi)) * (35 * c ** 4 - 30 * c * c + 3)
+    rg = np.linspace(0.0, rmax, 4001)
+
+    def dens(cv):  # p(r, c) on the r grid for each c, shape [len(c), len(r)]
+        Rc = R * (1 + b2 * y20(cv) + b4 * y40(cv))
+        f = 1 / (1 + np.exp((rg[None, :] - Rc[:, None]) / a))
+        return rg[None, :] ** 2 * f * (1 + np.exp(-Rc[:, None] / a))
+
+    cg = np.linspace(-1, 1, 4001)
+    marg = np.trapezoid(dens(cg), rg, axis=1)
+    cdf_c = np.concatenate([[0.0], np.cumsum(0.5 * (marg[1:] + marg[:-1]) * np.diff(cg))])
+    cdf_c /= cdf_c[-1]
+    assert stats.kstest(c, lambda v: np.interp(v, cg, cdf_c)).pvalue > 1e-3
+    for lo, hi in ((0.0, 0.25), (0.25, 0.5), (0.5, 0.75), (0.75, 1.0)):
+        sel = (np.abs(c) >= lo) & (np.abs(c) < hi)
+        cb = np.linspace(lo, hi, 201)  # the law is even in c
+        pr = np.trapezoid(dens(cb), cb, axis=0)
+        cdf_r = np.concatenate([[0.0], np.cumsum(0.5 * (pr[1:] + pr[:-1]) * np.diff(rg))])
+        cdf_r /= cdf_r[-1]
+        assert stats.kstest(r[sel], lambda v: np.interp(v, rg, cdf_r)).pvalue > 1e-3, (lo, hi)
+    assert stats.kstest(phi, stats.uniform(-math.pi, 2 * math.pi).cdf).pvalue > 1e-3
+    # and the law is not the spherical one: tips reach further than the waist
+    assert np.median(r[np.abs(c) > 0.9]) > np.median(r[np.abs(c) < 0.1]) + 0.8
+
+
+# ---------------------------------------------------------------- footprints and mask
+
+def _reference_boxes(orc):
+    """(x, y, w, box) with box from the compiled Box2D: the golden boxes at
+    their construction point, and, when the reference harness is built,
+    moved boxes at quantised positions."""
+    out = [(b["x0"], b["y0"], b["w"], b["box"]) for b in GOLD["particle_box"] if b["qw"] <= b["w"]]
+    r = orc.ref()
+    if r is not None:
+        import ctypes as C
+        q = 2.0 ** -24
+        rng = np.random.default_rng(21)
+        for _ in range(60):
+            x0, y0, x1, y1 = (np.round(rng.uniform(-9, 9, 4) / q) * q).tolist()
+            o = np.zeros(4)
+            r.ref_particle_box(x0, y0, 1, x1, y1, 0.5, 0.3, o.ctypes.data_as(C.POINTER(C.c_double)))
+            out.append((x1, y1, 0.5, list(o)))
+    return out
+
+
+def test_participant_footprint_edges_follow_reference_boxes(orc):
+    """The participant density's cells (MCnucl.cpp:831-844, the box of
+    A.4): rows exactly [(int)((xL - Xmin)/dx), (int)((xR - Xmin)/dx)) of the
+    reference's own box edges; every row's columns inside
+    [(int)((yL - Ymin)/dy), (int)((yR - Ymin)/dy)); rows within 2.5 w of
+    the nucleon hold that range exactly (an edge cell lies at most 4w + dx
+    from the nucleon, and 2.5^2 w^2 + (4w + dx)^2 <= 25 w^2 at w = 0.5,
+    dx = 0.1, so the whole box row is inside r <= 5w there).  An off-by-one edge row or column -- a cell holding ~3e-5 of
+    the nucleon -- fails here whatever its value."""
+    g = orc.grid(13.0, 0.1)
+    none = np.zeros((0, 2))
+    for x, y, w, (xl, xr, yl, yr) in _reference_boxes(orc):
+        s, _ = orc.density_glauber(g, w, 0.0, np.array([[x, y]]), np.ones(1), none, np.zeros(0), none, np.zeros(0))
+        ix0, ix1 = int((xl + 13.0) / 0.1), int((xr + 13.0) / 0.1)
+        iy0, iy1 = int((yl + 13.0) / 0.1), int((yr + 13.0) / 0.1)
+        rows = np.nonzero(s.any(axis=1))[0]
+        assert rows.tolist() == list(range(ix0, ix1)), (x, y)
+        for ix in rows:
+            cols = np.nonzero(s[ix])[0]
+            assert cols.min() >= iy0 and cols.max() < iy1
+            if abs(-13.0 + 0.1 * ix - x) <= 2.5 * w:
+                assert cols.tolist() == list(range(iy0, iy1)), (x, y, ix)
+
+
+def test_hotspot_mask_lies_on_the_boxes(orc):
+    """Mask invariants (MakeDensity.cpp:2341-2359; the arithmetic itself is
+    restated from the source, parity unpinned).  A box's cells are found
+    relative to the region: trunc((XL - rXL)/dx) + trunc((rXL - Xmin)/dx),
+    which is the absolute trunc((XL - Xmin)/dx) or one less (likewise for
+    the upper edge).  So with lo, hi the absolute indices of a box's edges,
+    every marked cell lies in [lo - 1, hi) of some box, and every cell in
+    [lo, hi - 1) of every box is marked.  Collision pairs -- all-zero boxes
+    that mark nothing -- change the mask only through the region's offsets,
+    i.e. not at all when the participant boxes already hold the origin."""
+    g = orc.grid(13.0, 0.1)
+    rng = np.random.default_rng(3)
+    idx = np.arange(g.nx)
+    for trial in range(40):
+        n = int(rng.integers(1, 12))
+        ctr = rng.normal(0.0, 3.0, (n, 2)) + (np.array([6.0, 5.0]) if trial % 4 == 3 else 0.0)
+        boxes = np.array([orc.particle_box(x, y, 0.5) for x, y in ctr])
+        for ncoll in (0, 3):
+            m = orc.hotspot_mask(g, boxes, ncoll)
+            near = np.zeros_like(m)
+            inner = np.zeros_like(m)
+            for xl, xr, yl, yr in boxes:
+                lo_x, hi_x = int((xl + 13.0) / 0.1), int((xr + 13.0) / 0.1)
+                lo_y, hi_y = int((yl + 13.0) / 0.1), int((yr + 13.0) / 0.1)
+                near |= ((idx >= lo_x - 1) & (idx < hi_x))[:, None] & ((idx >= lo_y - 1) & (idx < hi_y))[None, :]
+                inner |= ((idx >= lo_x) & (idx < hi_x - 1))[:, None] & ((idx >= lo_y) & (idx < hi_y - 1))[None, :]
+            assert not (m & ~near).any(), (trial, ncoll)
+            assert not (inner & ~m).any(), (trial, ncoll)
+        holds_origin = (boxes[:, 0].min() <= 0 <= boxes[:, 1].max()) and (boxes[:, 2].min() <= 0 <= boxes[:, 3].max())
+        if holds_origin:
+            assert np.array_equal(orc.hotspot_mask(g, boxes, 0), orc.hotspot_mask(g, boxes, 3))
+
+
+def test_masked_moments_equal_full_grid_moments_when_truncations_coincide(orc):
+    """On a grid whose step is a power of two (dx = 1/8) and nucleons at
+    cell-boundary-plus-half offsets, every box edge sits at k + 1/2 cells,
+    so region-relative and absolute truncations coincide and the mask is
+    exactly the union of the participant footprints.  Participants only
+    (alpha = 0, no pairs): the density vanishes off the mask, so the masked
+    eps_n equal the full-grid sums of the same density."""
+    g = orc.grid(13.0, 0.125)
+    rng = np.random.default_rng(8)
+    none = np.zeros((0, 2))
+    for _ in range(5):
+        k = rng.integers(-30, 30, (7, 2))
+        pts = k * 0.125 + 0.0625
+        s, _ = orc.density_glauber(g, 0.5, 0.0, pts, np.ones(len(pts)), none, np.zeros(0), none, np.zeros(0))
+        boxes = np.array([orc.particle_box(x, y, 0.5) for x, y in pts])
+        m = orc.hotspot_mask(g, boxes, 0)
+        assert not (s[~m] != 0).any()
+        mom = orc.moments(g, s, boxes, 0, 2, 5)
+        x = -13.0 + 0.125 * np.arange(g.nx)
+        X, Y = np.meshgrid(x - mom["xc"], x - mom["yc"], indexing="ij")
+        r, th = np.hypot(X, Y), np.arctan2(Y, X)
+        for n in range(2, 6):
+            num = (r * r * np.exp(1j * n * th) * s).sum()
+            den = (r * r * s).sum()
+            assert mom["mom_re"][n] == pytest.approx(-num.real / (den + 1e-15), abs=1e-13)
+            assert mom["mom_im"][n] == pytest.approx(-num.imag / (den + 1e-15), abs=1e-13)
