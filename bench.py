@@ -180,30 +180,43 @@ def mixed_params(nev, A, B, nuclei):
         max_pairs=8192, **nuclei)
 
 
-def time_mixed(nev, steps, seed):
-    """The mixed-system workload: p+Pb, d+Au, Cu+Cu, Au+Au and deformed U+U
-    cycled event by event inside every batch (smc_set_systems), 521^2 cells
-    at 0.05 fm, exact-footprint smearing with the tile shape autotuned."""
+def smear_roofline(ev, fp, smear_ms, nev):
+    """Useful FP32 work of the exact-footprint smearing (2 flops per source
+    and footprint cell, fp = footprint_cells) over a batch's average
+    smearing time, against MI355X's FP32 figure and the plain-FMA rate."""
+    work = 2.0 * (float(ev[:, 54].sum()) * fp["box"] + float(ev[:, 55].sum()) * fp["disk"]) / len(ev) * nev
+    tf = work / (smear_ms * 1e-3) / 1e12
+    return dict(achieved_tflops=tf, frac=tf * 1e12 / FP32_PEAK, frac_nonpacked=tf * 1e12 / FP32_PEAK_NONPACKED)
+
+
+def time_mixed(nev, total, seed, smear_path):
+    """The large mixed-system workload: p+Pb, d+Au, Cu+Cu, Au+Au and deformed
+    U+U cycled event by event, 521^2 cells at 0.05 fm, `total` events in one
+    smc_run_stream call through a context of `nev` events (device memory of
+    one batch), records streamed to host memory."""
     from supermc_amd import Context
     systems = [mixed_params(nev, A, B, k) for _, A, B, k in MIXED]
-    with Context(mixed_params(nev, 238, 238, MIXED[-1][3])) as ctx:
+    for p in systems:
+        p.smear_path = smear_path
+    p0 = mixed_params(nev, 238, 238, MIXED[-1][3])
+    p0.smear_path = smear_path
+    with Context(p0) as ctx:
         ctx.set_systems(systems, list(range(len(systems))))
-        ctx.run(seed, 0, nev, fetch=False)
+        ctx.run(seed, 0, nev, fetch=False)  # warm-up
         ctx.synchronize()
-        best, ms = ctx.autotune(reps=1)
         ctx.set_timing(True)
+        rec = np.empty((total, 64))
         t0 = time.perf_counter()
-        for k in range(steps):
-            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
-        ctx.synchronize()
+        ctx.run_stream(seed, nev, total, out=rec)
         dt = time.perf_counter() - t0
-        ev = ctx.fetch_events(nev)
         ker = {name: ctx.timing(kid)[0] / max(ctx.timing(kid)[1], 1)
                for kid, name in ((0, "k_event"), (1, "k_smear"), (2, "k_moments"))}
-    npart = {MIXED[s][0]: float(ev[ev[:, 58] == s, 54].mean()) for s in range(len(MIXED))}
-    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, grid=[521, 521],
-                systems=[m[0] for m in MIXED], mean_npart=npart, failed_events=int((ev[:, 57] != 0).sum()),
-                tile_variant=best, tile_ms=[round(float(m), 3) for m in ms], kernels_avg_ms=ker)
+        fp = footprint_cells(ctx.derived.width, ctx.params.dx, -ctx.params.maxx)
+    npart = {MIXED[s][0]: float(rec[rec[:, 58] == s, 54].mean()) for s in range(len(MIXED))}
+    return dict(events_per_s=total / dt, events=total, events_per_batch=nev, grid=[521, 521],
+                smear_path=SMEAR_PATHS[smear_path], systems=[m[0] for m in MIXED], mean_npart=npart,
+                failed_events=int((rec[:, 57] != 0).sum()), kernels_avg_ms_per_batch=ker,
+                roofline_smear=smear_roofline(rec[:nev], fp, ker["k_smear"], nev))
 
 
 def time_avg(nev, steps, seed):
@@ -431,6 +444,8 @@ def main():
     ap.add_argument("--p3d-events", type=int, default=16, help="3D profile events per call (0: skip)")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager vs hipGraph-replay comparison")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shape")
+    ap.add_argument("--mixed-total", type=int, default=1000000,
+                    help="events of the mixed-system workload in one smc_run_stream call")
     ap.add_argument("--mixed-events", type=int, default=50000,
                     help="mixed-system 521^2 events per batch (0: skip)")
     ap.add_argument("--avg-events", type=int, default=20000,
@@ -472,8 +487,8 @@ def main():
             r = time_jet(args.jet_events, args.steps, args.seed)
             metric, wl = "events/s operation-2 grids Pb+Pb 261^2", "rho_binary and spectator densities"
         else:
-            r = time_mixed(args.mixed_events, args.steps, args.seed)
-            metric, wl = "events/s mixed systems 521^2", "p+Pb, d+Au, Cu+Cu, Au+Au, U+U in one batch"
+            r = time_mixed(args.mixed_events, args.mixed_total, args.seed, args.smear_path)
+            metric, wl = "events/s mixed systems 521^2", "p+Pb, d+Au, Cu+Cu, Au+Au, U+U in one stream"
         print(json.dumps({"metric": metric, "value": r["events_per_s"], "unit": "events/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": 1, "higher_is_better": True, "dtype": "f32",
                           "data": "synthetic", "config": dict(workload=wl, **r)}), flush=True)
@@ -559,7 +574,12 @@ def main():
     adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
     p3d = time_p3d(args.p3d_events, 2, args.seed) if args.p3d_events > 0 else None
     graph = time_graph(args.seed) if not args.no_graph else None
-    mixed = time_mixed(args.mixed_events, 2, args.seed) if args.mixed_events > 0 else None
+    mixed = None
+    if args.mixed_events > 0:
+        # BF16x3 MFMA, the precision the large-batch configuration names,
+        # beside the exact FP32 scatter path (VALU tiles) on the same events
+        mixed = dict(bf16x3_mfma=time_mixed(args.mixed_events, args.mixed_total, args.seed, 3),
+                     fp32_valu_tiles=time_mixed(args.mixed_events, args.mixed_total, args.seed, 0))
     avg = time_avg(args.avg_events, 2, args.seed) if args.avg_events > 0 else None
     jet = time_jet(args.jet_events, 2, args.seed) if args.jet_events > 0 else None
     dflt = time_defaults(args.default_events, 2, args.seed) if args.default_events > 0 else None

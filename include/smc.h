@@ -224,6 +224,14 @@ smc_status smc_moments(smc_ctx* ctx, double* ev_out);
  * moments, inputs and outputs resident on the device.  ev_out as above
  * (NULL: keep on device, fetch with smc_fetch_events). */
 smc_status smc_run(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
+/* The generateEccTable event loop (MakeDensity.cpp:2143-2224) for nev_total
+ * events of stream seed from event0, any number of them, in batches of
+ * max_events through the ctx's own device buffers: device memory stays that
+ * of one batch.  ev_out: host [nev_total][SMC_EV_STRIDE] (required); batch
+ * i's records are copied out on a second stream while batch i + 1 runs.
+ * Events equal smc_run's for the same indices.  The ctx afterwards holds
+ * the last batch. */
+smc_status smc_run_stream(smc_ctx* ctx, uint64_t seed, uint64_t event0, int64_t nev_total, double* ev_out);
 /* smc_run replayed as a hipGraph (no counterpart in the reference: for
  * callers that launch many same-sized batches).  The first call for a batch
  * size runs eagerly, the second captures the launch sequence, later calls

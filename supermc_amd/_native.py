@@ -105,6 +105,7 @@ SIGNATURES = {
     "smc_get_pairs": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _ip, _ip, _fp, _ip]),
     "smc_get_sources": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _fp, _ip]),
     "smc_recenter": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    "smc_run_stream": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, _dp]),
     "smc_thickness": (C.c_int, [C.c_void_p, _fp, _fp]),
     "smc_density": (C.c_int, [C.c_void_p, _fp, _dp]),
     "smc_moments": (C.c_int, [C.c_void_p, _dp]),
@@ -291,6 +292,15 @@ class Context:
         ev = np.zeros((nev, EV_STRIDE)) if fetch else None
         self._check(self.lib.smc_run(self.h, seed, event0, nev, _ptr(ev, C.c_double)))
         self.nev = nev
+        return ev
+
+    def run_stream(self, seed, event0, nev_total, out=None):
+        """smc_run_stream: nev_total events in batches of max_events through
+        the context's buffers; returns the records [nev_total][64]."""
+        ev = out if out is not None else np.empty((nev_total, EV_STRIDE))
+        self._check(self.lib.smc_run_stream(self.h, seed, event0, nev_total, _ptr(ev, C.c_double)))
+        cap = self.params.max_events
+        self.nev = nev_total - (nev_total - 1) // cap * cap  # the context holds the last batch
         return ev
 
     def set_systems(self, systems, pattern):

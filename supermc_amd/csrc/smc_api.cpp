@@ -155,6 +155,12 @@ struct smc_ctx {
   int nev = 0;        // events currently held
   int have_events = 0, have_density = 0, have_thickness = 0, have_moments = 0;
   int grid_sparse = 0;      // grid holds only the tiles inside each event's source rectangle
+  // smc_run_stream: second record buffer, pinned staging, the copy stream and
+  // its events (created on first use)
+  double* evout_alt = nullptr;
+  double* stage[2] = {nullptr, nullptr};
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t ev_done[2] = {nullptr, nullptr}, ev_copied[2] = {nullptr, nullptr};
   int* dtab = nullptr;
   float* grid = nullptr;
   float* ta = nullptr;
@@ -1215,6 +1221,13 @@ void smc_destroy(smc_ctx* c) {
     if (q) (void)hipFree(q);
   if (c->dif_out) (void)hipFree(c->dif_out);
   if (c->dif_cot) (void)hipFree(c->dif_cot);
+  if (c->evout_alt) (void)hipFree(c->evout_alt);
+  for (int i = 0; i < 2; ++i) {
+    if (c->stage[i]) (void)hipHostFree(c->stage[i]);
+    if (c->ev_done[i]) (void)hipEventDestroy(c->ev_done[i]);
+    if (c->ev_copied[i]) (void)hipEventDestroy(c->ev_copied[i]);
+  }
+  if (c->copy_stream) (void)hipStreamDestroy(c->copy_stream);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1603,6 +1616,74 @@ void graph_capture(smc_ctx* c, int nev, uint64_t seed, uint64_t event0) {
 }
 
 }  // namespace
+
+// The whole hot path for nev_total events, in batches of max_events through
+// the ctx's device buffers (one grid batch, one source batch: memory is
+// bounded whatever nev_total is).  Batch i's records go to pinned staging
+// on a second stream while batch i + 1 runs (two record buffers alternate),
+// then into ev_out on the host.
+smc_status smc_run_stream(smc_ctx* c, uint64_t seed, uint64_t event0, int64_t nev_total, double* ev_out) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (nev_total <= 0) throw SmcError(SMC_ERR_ARG, "nev_total must be positive");
+    if (!ev_out) throw SmcError(SMC_ERR_ARG, "ev_out is NULL");
+    const int cap = c->cap;
+    const size_t rec = (size_t)SMC_EV_STRIDE * sizeof(double);
+    if (!c->copy_stream) {
+      HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+      for (int i = 0; i < 2; ++i) {
+        HIPCHK(hipHostMalloc((void**)&c->stage[i], (size_t)cap * rec, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_done[i], hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&c->ev_copied[i], hipEventDisableTiming));
+      }
+      c->evout_alt = dalloc<double>((size_t)cap * SMC_EV_STRIDE);
+    }
+    double* const bufs[2] = {c->evout, c->evout_alt};
+    const int64_t nb = (nev_total + cap - 1) / cap;
+    int nbatch[2] = {0, 0};
+    uint64_t e0batch[2] = {0, 0};
+    auto drain = [&](int64_t i) {  // batch i's records from staging to ev_out
+      const int k = (int)(i & 1);
+      HIPCHK(hipEventSynchronize(c->ev_copied[k]));
+      double* dst = ev_out + (size_t)(i * cap) * SMC_EV_STRIDE;
+      std::memcpy(dst, c->stage[k], (size_t)nbatch[k] * rec);
+      if (!c->mix_of.empty())
+        for (int e = 0; e < nbatch[k]; ++e)
+          dst[(size_t)e * SMC_EV_STRIDE + SMC_EV_SYSTEM] = c->mix_of[(e0batch[k] + (uint64_t)e) % c->mix_of.size()];
+    };
+    try {
+      for (int64_t i = 0; i < nb; ++i) {
+        const int k = (int)(i & 1);
+        const int n = (int)std::min<int64_t>(cap, nev_total - i * cap);
+        if (i >= 2) drain(i - 2);  // staging k and record buffer k are free again
+        c->evout = bufs[k];
+        run_event(c, n, seed, event0 + (uint64_t)(i * cap), 0);
+        run_moments(c, 0);
+        HIPCHK(hipEventRecord(c->ev_done[k], c->stream));
+        HIPCHK(hipStreamWaitEvent(c->copy_stream, c->ev_done[k], 0));
+        HIPCHK(hipMemcpyAsync(c->stage[k], bufs[k], (size_t)n * rec, hipMemcpyDeviceToHost, c->copy_stream));
+        HIPCHK(hipEventRecord(c->ev_copied[k], c->copy_stream));
+        nbatch[k] = n;
+        e0batch[k] = event0 + (uint64_t)(i * cap);
+      }
+      for (int64_t i = std::max<int64_t>(0, nb - 2); i < nb; ++i) drain(i);
+    } catch (...) {
+      c->evout = bufs[0];
+      throw;
+    }
+    // the ctx keeps the last batch; its records stay readable by
+    // smc_fetch_events from the context's own record buffer
+    const int kl = (int)((nb - 1) & 1);
+    if (kl == 1)
+      HIPCHK(hipMemcpyAsync(bufs[0], bufs[1], (size_t)nbatch[1] * rec, hipMemcpyDeviceToDevice, c->stream));
+    c->evout = bufs[0];
+    sync(c);
+    c->have_moments = 1;
+    graph_release(c);
+    c->graph_warm = 0;
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
 
 // smc_run as a replayed hipGraph.  The first call for a batch size runs
 // eagerly (it allocates every buffer the batch uses); the second captures

@@ -12,7 +12,8 @@
 //   1. MFMA over the square window (all sources of a group at once);
 //   2. a VALU pass that subtracts, cell by cell, the products the MFMA added
 //      outside the disk.  It visits only the 4 x 16 patches that hold such
-//      cells (a uniform distance test per patch), and decides each cell by
+//      cells (flagged per source by a distance test, (source, patch) pairs
+//      spread over the lanes), and decides each cell by
 //      gx gy < exp(-cut2 / (2 w^2)) in float, within a few ulp of the FP64
 //      disk -- the cells there carry e^-12.5 of a source's peak.
 // So the grids are those of the exact footprint, to the parity bars, on any
@@ -89,8 +90,12 @@ __global__ __launch_bounds__(64) void k_mtile(DevCfg cfg, EvBufs buf, SmearSpec 
   constexpr int NP = PREC == 2 ? 3 : 2;     // bf16 pieces per factor
   constexpr bool TWO = EPI != 0;
   __shared__ MtEntry wl[64];
-  __shared__ __align__(16) float la[2][16][G];  // SA gx(row): [row block][physical row][source]
-  __shared__ __align__(16) float lg[4][16][G];  // gy(col):    [column block][column][source]
+  // factors for the exact-cut pass, one row per physical row / column, a
+  // padded stride so that the 16 columns of a read fall in distinct banks
+  constexpr int GS = G + 1;
+  __shared__ float la[2][16][GS];  // SA gx(row): [row block][physical row][source]
+  __shared__ float lg[4][16][GS];  // gy(col):    [column block][column][source]
+  __shared__ unsigned lmask[G];    // patches (ri * 16 + v * 4 + ci) of each source that need the cut
   const int lane = threadIdx.x, q = lane >> 4, j16 = lane & 15;
   const int nx = cfg.nx, ny = cfg.ny;
   const int ntx = (nx + 31) / 32;
@@ -221,9 +226,8 @@ __global__ __launch_bounds__(64) void k_mtile(DevCfg cfg, EvBufs buf, SmearSpec 
           unsigned rm = 0u;
 #pragma unroll
           for (int ri = 0; ri < 2; ++ri) {
-            float4* d4 = reinterpret_cast<float4*>(&la[ri][prow][8 * q]);
-            d4[0] = make_float4(va[ri][0], va[ri][1], va[ri][2], va[ri][3]);
-            d4[1] = make_float4(va[ri][4], va[ri][5], va[ri][6], va[ri][7]);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) la[ri][prow][8 * q + t] = va[ri][t];
             bool any = false;
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
@@ -237,9 +241,8 @@ __global__ __launch_bounds__(64) void k_mtile(DevCfg cfg, EvBufs buf, SmearSpec 
           }
 #pragma unroll
           for (int ci = 0; ci < 4; ++ci) {
-            float4* d4 = reinterpret_cast<float4*>(&lg[ci][j16][8 * q]);
-            d4[0] = make_float4(vb[ci][0], vb[ci][1], vb[ci][2], vb[ci][3]);
-            d4[1] = make_float4(vb[ci][4], vb[ci][5], vb[ci][6], vb[ci][7]);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) lg[ci][j16][8 * q + t] = vb[ci][t];
             v8bf bp[NP];
             bool any = false;
 #pragma unroll
@@ -268,35 +271,55 @@ __global__ __launch_bounds__(64) void k_mtile(DevCfg cfg, EvBufs buf, SmearSpec 
             }
           }
         }
-        __builtin_amdgcn_wave_barrier();
-        // ---- exact cut: subtract the products outside the disk
+        // ---- which 4 x 16 patches of which source hold cells of its square
+        // window outside its disk: lane-parallel, (source, patch) pairs spread
+        // over the lanes; a patch whose farthest cell (with a margin) lies
+        // inside the disk needs nothing
         const int ng = min(G, n - g);
-        for (int s = 0; s < ng; ++s) {
-          const MtEntry es = wl[g + s];
-          const int xlo = es.xw & 0xffff, xhi = es.xw >> 16, ylo = es.yw & 0xffff, yhi = es.yw >> 16;
-          const float sx = (float)((double)es.xh * kH), sy = (float)((double)es.yh * kH);
-          const float wsb = es.w * kMtSB;
+        {
+          constexpr int PER = 32 * G / 64;  // pairs per lane
+          // G = 32: lane l takes source l / 2, patches 16 (l % 2) + k, k < 16;
+          // G = 4: lane l takes source l / 16, patches l % 16 and l % 16 + 16
+          const int s = PER == 2 ? (lane >> 4) : (lane >> 1);
+          const int p0 = PER == 2 ? (lane & 15) : 16 * (lane & 1), pstep = PER == 2 ? 16 : 1;
+          unsigned bits = 0u;
+          if (s < ng) {
+            const MtEntry es = wl[g + s];
+            const int xlo = es.xw & 0xffff, xhi = es.xw >> 16, ylo = es.yw & 0xffff, yhi = es.yw >> 16;
+            const float sx = (float)((double)es.xh * kH), sy = (float)((double)es.yh * kH);
 #pragma unroll
-          for (int ri = 0; ri < 2; ++ri) {
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
+            for (int k = 0; k < PER; ++k) {
+              const int pp = p0 + k * pstep, ri = pp >> 4, v = (pp >> 2) & 3, ci = pp & 3;
               const int pr0 = max(r0 + 16 * ri + 4 * v, xlo), pr1 = min(r0 + 16 * ri + 4 * v + 4, xhi);
-              if (pr0 >= pr1) continue;
-              const float fx0 = (float)(cfg.xmin + pr0 * cfg.dx) - sx, fx1 = (float)(cfg.xmin + (pr1 - 1) * cfg.dx) - sx;
-              const float mx2 = fmaxf(fx0 * fx0, fx1 * fx1);
-#pragma unroll
-              for (int ci = 0; ci < 4; ++ci) {
-                const int pc0 = max(c0 + 16 * ci, ylo), pc1 = min(c0 + 16 * ci + 16, yhi);
-                if (pc0 >= pc1) continue;
-                const float fy0 = (float)(cfg.ymin + pc0 * cfg.dy) - sy,
-                            fy1 = (float)(cfg.ymin + (pc1 - 1) * cfg.dy) - sy;
-                // every cell of the patch inside the disk (with a margin): nothing to subtract
-                if (mx2 + fmaxf(fy0 * fy0, fy1 * fy1) <= cut2 * 0.9999f) continue;
-                const float t = la[ri][4 * v + q][s] * lg[ci][j16][s];
-                const float c = (t < thr) ? t * wsb : 0.f;
-                acc[ri][ci][v] -= c;
+              const int pc0 = max(c0 + 16 * ci, ylo), pc1 = min(c0 + 16 * ci + 16, yhi);
+              if (pr0 < pr1 && pc0 < pc1) {
+                const float fx0 = (float)(cfg.xmin + pr0 * cfg.dx) - sx, fx1 = (float)(cfg.xmin + (pr1 - 1) * cfg.dx) - sx;
+                const float fy0 = (float)(cfg.ymin + pc0 * cfg.dy) - sy, fy1 = (float)(cfg.ymin + (pc1 - 1) * cfg.dy) - sy;
+                if (fmaxf(fx0 * fx0, fx1 * fx1) + fmaxf(fy0 * fy0, fy1 * fy1) > cut2 * 0.9999f) bits |= 1u << k;
               }
             }
+          }
+          if constexpr (PER == 2) {  // bit i of source s's mask sits in lane 16 s + i of b0 (i < 16) or b1
+            const unsigned long long b0 = __ballot(bits & 1u), b1 = __ballot(bits & 2u);
+            if (lane < G) lmask[lane] = (unsigned)((b0 >> (16 * lane)) & 0xffffu) | ((unsigned)((b1 >> (16 * lane)) & 0xffffu) << 16);
+          } else {  // G = 32: lanes 2 s and 2 s + 1 hold source s's low and high 16 patches
+            const unsigned hi = __shfl_xor(bits, 1, 64);
+            if (!(lane & 1)) lmask[lane >> 1] = bits | (hi << 16);
+          }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // ---- exact cut: subtract, in the flagged patches, the products the
+        // MFMA added outside the disk
+        for (int s = 0; s < ng; ++s) {
+          const unsigned mk = __builtin_amdgcn_readfirstlane(lmask[s]);
+          if (!mk) continue;
+          const float wsb = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(wl[g + s].w))) * kMtSB;
+#pragma unroll
+          for (int pp = 0; pp < 32; ++pp) {
+            if (!((mk >> pp) & 1u)) continue;
+            const int ri = pp >> 4, v = (pp >> 2) & 3, ci = pp & 3;
+            const float t = la[ri][4 * v + q][s] * lg[ci][j16][s];
+            acc[ri][ci][v] -= (t < thr) ? t * wsb : 0.f;
           }
         }
         __builtin_amdgcn_wave_barrier();

@@ -90,3 +90,23 @@ def test_mixed_521_split_precision_against_exact_fp32_path():
         worst = max(H.rel_linf(s[e], s0[e]) for e in range(nev))
         assert worst <= gtol, (path, worst)
         assert np.abs(ev[:, :40] - ev0[:, :40]).max() <= etol, path
+
+
+def test_million_event_stream_mixed_521():
+    """10^6 mixed-system events on 521^2 in one smc_run_stream call through a
+    5 x 10^4-event context on the BF16x3 MFMA path -- the large batch of the
+    workload in bounded device memory: every event accepted, the system
+    pattern in the records, and the first, a middle and the last events
+    equal smc_run's for the same event indices (counter-based sampling)."""
+    from supermc_amd import Context
+    nev_total, cap = 1_000_000, 50_000
+    systems = [mixed_params(A, B, k, max_events=cap, smear_path=3) for _, A, B, k in MIXED]
+    with Context(mixed_params(238, 238, MIXED[-1][3], max_events=cap, smear_path=3)) as ctx:
+        ctx.set_systems(systems, list(range(len(MIXED))))
+        rec = ctx.run_stream(seed=99, event0=0, nev_total=nev_total)
+        assert rec.shape == (nev_total, 64)
+        assert (rec[:, 57] == 0).all()
+        assert np.array_equal(rec[:, 58], np.arange(nev_total) % len(MIXED))
+        for start in (0, 523_417, nev_total - 500):
+            one = ctx.run(seed=99, event0=start, nev=500)
+            assert np.array_equal(one[:, :58], rec[start:start + 500, :58]), start
