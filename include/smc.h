@@ -224,6 +224,21 @@ smc_status smc_moments(smc_ctx* ctx, double* ev_out);
  * moments, inputs and outputs resident on the device.  ev_out as above
  * (NULL: keep on device, fetch with smc_fetch_events). */
 smc_status smc_run(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out);
+/* smc_run for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops).  No counterpart in the reference.
+ * The same events and records as smc_run, written to ev_out_dev, a device
+ * buffer of [nev][SMC_EV_STRIDE] doubles the caller owns (slot
+ * SMC_EV_SYSTEM is 0).  The call allocates nothing: the moment kernels'
+ * per-event scratch lives in a static device array of the library (about
+ * 17 MB per GPU, held until the process ends), so nev is at most 2048
+ * (SMC_ERR_CAPACITY above), and the context must already hold the buffers
+ * of a batch (run smc_run on it once first; SMC_ERR_STATE otherwise).
+ * MC-KLN only through the dN/dy table (smear_path 0, 1, 3 or 4).  It returns
+ * without waiting: call smc_synchronize before reading ev_out_dev.  Every
+ * call on one GPU shares that GPU's scratch: calls that overlap in time on
+ * one GPU (contexts on different threads or streams) race. */
+smc_status smc_run_noalloc(smc_ctx* ctx, uint64_t seed, uint64_t event0, int nev, double* ev_out_dev);
+
 /* The generateEccTable event loop (MakeDensity.cpp:2143-2224) for nev_total
  * events of stream seed from event0, any number of them, in batches of
  * max_events through the ctx's own device buffers: device memory stays that

@@ -106,6 +106,7 @@ SIGNATURES = {
     "smc_get_sources": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _ip, _fp, _ip]),
     "smc_recenter": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
     "smc_run_stream": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64, _dp]),
+    "smc_run_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]),
     "smc_thickness": (C.c_int, [C.c_void_p, _fp, _fp]),
     "smc_density": (C.c_int, [C.c_void_p, _fp, _dp]),
     "smc_moments": (C.c_int, [C.c_void_p, _dp]),

@@ -1617,6 +1617,54 @@ void graph_capture(smc_ctx* c, int nev, uint64_t seed, uint64_t event0) {
 
 }  // namespace
 
+// smc_run for callers that cannot allocate: the same launches, no timing
+// events, the moment kernels' scratch in the static device array of
+// smc_noalloc.hip, the records written to the caller's device buffer.  Every
+// buffer the batch touches must exist already (an earlier smc_run on the
+// context allocated them).  Asynchronous on the context's stream.
+smc_status smc_run_noalloc(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, double* ev_out_dev) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!ev_out_dev) throw SmcError(SMC_ERR_ARG, "ev_out_dev is NULL");
+    if (nev <= 0 || nev > c->cap) throw SmcError(SMC_ERR_CAPACITY, "nev outside [1, max_events]");
+    if (nev > kNoAllocMaxEvents)
+      throw SmcError(SMC_ERR_CAPACITY, "nev above smc_run_noalloc's " + std::to_string(kNoAllocMaxEvents) +
+                                           " events per call (the size of its device scratch)");
+    if ((size_t)moments_band_doubles() * sizeof(double) * kNoAllocMaxEvents != kNoAllocBandBytes ||
+        (size_t)moments_cm_units_min() * 4 * sizeof(double) * kNoAllocMaxEvents != kNoAllocCmBytes)
+      throw SmcError(SMC_ERR_UNSUPPORTED, "moment scratch layout does not match the static scratch");
+    const bool kln = c->p.which_mc_model == 1;
+    if (kln && (c->p.kln_direct || c->p.smear_path == 2))
+      throw SmcError(SMC_ERR_UNSUPPORTED, "smc_run_noalloc takes MC-KLN through the dN/dy table in one launch only");
+    const SmearSpec spec = density_spec(c);
+    const bool need_tmp = !kln && spec.combine && !density_cm_units(c);
+    if (!c->grid || (kln && !c->kln_table) || (need_tmp && !c->tmp))
+      throw SmcError(SMC_ERR_STATE, "call smc_run once on this context first: it creates the buffers of the batch");
+    int rc = 0;
+    unsigned char* base = static_cast<unsigned char*>(noalloc_scratch(&rc));
+    if (!base) throw SmcError(SMC_ERR_HIP, std::string("scratch lookup: ") + hipGetErrorString((hipError_t)rc));
+    double* band = reinterpret_cast<double*>(base);
+    void* rects = base + kNoAllocBandBytes;
+    double* cms = reinterpret_cast<double*>(base + kNoAllocBandBytes + kNoAllocRectBytes);
+    int* nrect = reinterpret_cast<int*>(base + kNoAllocBandBytes + kNoAllocRectBytes + kNoAllocCmBytes);
+    check_launch(launch_event(c->cfg, c->buf, nev, seed, event0, 0, c->stream), "k_event");
+    c->batch_event0 = event0;
+    c->batch_mode = 0;
+    c->nev = nev;
+    c->have_events = 1;
+    c->have_thickness = 0;
+    density_smear_launch(c);
+    c->cm_nu = density_cm_units(c);
+    c->grid_sparse = c->cm_nu ? 1 : 0;
+    c->have_density = 1;
+    check_launch(launch_moments(c->cfg, c->buf, c->grid, nev, ev_out_dev, 0, c->cm_nu ? c->cm : nullptr, c->cm_nu, cms,
+                                band, rects, nrect, c->stream),
+                 "k_moments");
+    c->have_moments = 0;  // the context's own record buffer holds an earlier batch
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
 // The whole hot path for nev_total events, in batches of max_events through
 // the ctx's device buffers (one grid batch, one source batch: memory is
 // bounded whatever nev_total is).  Batch i's records go to pinned staging

@@ -202,6 +202,19 @@ int launch_mtile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev
                  const double* ktab, int ktmax, double kdT, void* stream);
 int launch_tile(const DevCfg& cfg, const EvBufs& b, const SmearSpec& s, int nev, float* out0, float* out1,
                 void* stream);
+// smc_run_noalloc's device scratch (smc_noalloc.hip): per event the moment
+// band sums (moments_band_doubles() doubles), the hot-spot rectangles
+// (2 kAMax 8-byte entries), centre-of-mass units (moments_cm_units_min() x 4
+// doubles) and the rectangle count, for at most kNoAllocMaxEvents events
+constexpr int kNoAllocMaxEvents = 2048;
+constexpr size_t kNoAllocBandBytes = (size_t)kNoAllocMaxEvents * 8 * 64 * sizeof(double);
+constexpr size_t kNoAllocRectBytes = (size_t)kNoAllocMaxEvents * 2 * kAMax * 8;
+constexpr size_t kNoAllocCmBytes = (size_t)kNoAllocMaxEvents * 8 * 4 * sizeof(double);
+constexpr size_t kNoAllocNrectBytes = (size_t)kNoAllocMaxEvents * sizeof(int);
+constexpr size_t kNoAllocScratchBytes = kNoAllocBandBytes + kNoAllocRectBytes + kNoAllocCmBytes + kNoAllocNrectBytes;
+// device address of the current device's scratch (looked up once per
+// device); nullptr with the HIP error in *rc on failure
+void* noalloc_scratch(int* rc);
 // zeroes every cell outside each event's source rectangle (counts[6..7]):
 // completes grids written with SmearSpec::sparse
 int launch_zero_outside(float* grid, const int* counts, int nx, int ny, int nev, void* stream);
