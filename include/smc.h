@@ -281,6 +281,32 @@ smc_status smc_spectator_density(smc_ctx* ctx, float* spec_proj, float* spec_tar
 smc_status smc_average_profiles(smc_ctx* ctx, int from_order, int to_order, const unsigned char* accept,
                                 double* sums, double* rp_sums);
 
+/* The optional averages of operation 3 (parameters.dat output_TATB,
+ * output_rho_binary, output_TA, output_spectator_density;
+ * MakeDensity.cpp:887-1200, 1290-1385), each host [to-from+1][nx][ny],
+ * added to; NULL skips it.  After each order's turn (and, for the *_rp
+ * ones, after the recentring of the reaction-plane pass, which needs
+ * rp_sums): T_A T_B of the moved participants, the binary-collision density
+ * of the moved pairs (MCnucl::calculate_rho_binary), T_A and T_B; after the
+ * turn only, the spectator densities of the projectile and the target
+ * (MCnucl::calculate_spectator_density), the spectators moved and turned
+ * with the sources (MCnucl.cpp:1139-1143, 1168-1173).  No finalFactor. */
+typedef struct smc_avg_extra {
+  double* tatb;
+  double* rho_binary;
+  double* ta;
+  double* tb;
+  double* spec_a;
+  double* spec_b;
+  double* tatb_rp;
+  double* rho_binary_rp;
+  double* ta_rp;
+  double* tb_rp;
+} smc_avg_extra;
+/* smc_average_profiles with the optional averages of extra (NULL: none). */
+smc_status smc_average_profiles_extra(smc_ctx* ctx, int from_order, int to_order, const unsigned char* accept,
+                                      double* sums, double* rp_sums, const smc_avg_extra* extra);
+
 /* MC-KLN: KLNModel::getdNdy (KLNModel.cpp:97-125) at y=0 for n (TA,TB)
  * points, by the Gauss-Legendre product rule on the device. */
 smc_status smc_kln_dndy(smc_ctx* ctx, int n, const double* ta, const double* tb, double* out);

@@ -282,19 +282,45 @@ def cm_angle(g, s, n):
     return xc, yc, -np.arctan2(-im, -re) / n
 
 
-def rotated_profiles(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c, from_order, to_order, ff, reaction_plane=False):
+def rotated_profiles(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c, from_order, to_order, ff, reaction_plane=False,
+                     spectators=None):
     """One event of MakeDensity::generate_profile_average (operation 3,
     MakeDensity.cpp:1266-1345, Glauber, pT-integrated): for each order in
     turn, optionally recentre (MCnucl::recenterGrid, MCnucl.cpp:1119-1145)
     and record ff * rho, then recentre and turn the sources by the order-n
     angle (MCnucl::rotateGrid, MCnucl.cpp:1147-1174, Point3D::rotate with
     theta = 0) and record ff * rho.  The sources stay moved between orders.
-    Returns (profiles [norder][nx][ny], recentred profiles or None)."""
+    Returns (profiles [norder][nx][ny], recentred profiles or None).
+
+    With spectators = (xy_A, xy_B) -- the nucleons without a collision,
+    moved and turned with the sources (MCnucl.cpp:1139-1143, 1168-1173) --
+    also returns a third item: the optional averages of MakeDensity.cpp:
+    1290-1385 per pass, a dict of lists over orders: T_A T_B, rho_binary
+    (MCnucl::calculate_rho_binary: the normalised Gaussians of the pairs,
+    the thickness footprint), T_A and T_B after each pass ("rp_" keys for
+    the recentred pass), and the spectator densities A, B after the turn
+    only."""
     pts = [np.array(xy_a, np.float64).reshape(-1, 2), np.array(xy_b, np.float64).reshape(-1, 2),
            np.array(xy_c, np.float64).reshape(-1, 2)]
+    if spectators is not None:
+        pts += [np.array(spectators[0], np.float64).reshape(-1, 2), np.array(spectators[1], np.float64).reshape(-1, 2)]
+    extra = {k: [] for k in ("tatb", "rho_binary", "ta", "tb", "spec_a", "spec_b",
+                             "rp_tatb", "rp_rho_binary", "rp_ta", "rp_tb")}
 
     def dens():
         return density_glauber(g, w, alpha, pts[0], f_a, pts[1], f_b, pts[2], f_c)[0]
+
+    def record(prefix, with_spec):
+        if spectators is None:
+            return
+        ta, tb = thickness(g, w, pts[0]), thickness(g, w, pts[1])
+        extra[prefix + "tatb"].append(ta * tb)
+        extra[prefix + "rho_binary"].append(thickness(g, w, pts[2]))
+        extra[prefix + "ta"].append(ta)
+        extra[prefix + "tb"].append(tb)
+        if with_spec:
+            extra["spec_a"].append(thickness(g, w, pts[3]))
+            extra["spec_b"].append(thickness(g, w, pts[4]))
 
     out, rp = [], []
     s = dens()
@@ -306,6 +332,7 @@ def rotated_profiles(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c, from_order, t
                 q[:, 1] -= yc
             s = dens()
             rp.append(ff * s)
+            record("rp_", False)
         xc, yc, ang = cm_angle(g, s, n)
         cp, sp = np.cos(ang), np.sin(ang)
         for q in pts:
@@ -314,7 +341,11 @@ def rotated_profiles(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c, from_order, t
             q[:, 1] = 1.0 * sp * x0 + cp * y0
         s = dens()
         out.append(ff * s)
-    return np.array(out), (np.array(rp) if reaction_plane else None)
+        record("", True)
+    res = (np.array(out), (np.array(rp) if reaction_plane else None))
+    if spectators is None:
+        return res
+    return res + ({k: np.array(v) for k, v in extra.items() if v},)
 
 
 def density_trento(g, w, xy_a, f_a, xy_b, f_b):

@@ -114,6 +114,7 @@ SIGNATURES = {
     "smc_run_graph": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
     "smc_fetch_events": (C.c_int, [C.c_void_p, C.c_int, _dp]),
     "smc_average_profiles": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp, _dp]),
+    "smc_average_profiles_extra": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_void_p, _dp, _dp, C.c_void_p]),
     "smc_rho_binary": (C.c_int, [C.c_void_p, _fp]),
     "smc_spectator_density": (C.c_int, [C.c_void_p, _fp, _fp]),
     "smc_kln_dndy": (C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp]),
@@ -375,6 +376,27 @@ class Context:
         rot = np.zeros((self.nev, 4))
         self._check(self.lib.smc_recenter(self.h, n, 1 if turn else 0, _ptr(rot, C.c_double)))
         return rot
+
+    EXTRA_KEYS = ("tatb", "rho_binary", "ta", "tb", "spec_a", "spec_b", "tatb_rp", "rho_binary_rp", "ta_rp", "tb_rp")
+
+    def average_profiles_extra(self, from_order, to_order, accept=None, reaction_plane=True):
+        """smc_average_profiles_extra with every optional average: returns
+        (sums, rp_sums or None, dict of the optional sums)."""
+        nord = to_order - from_order + 1
+        shape = (nord, self.nx, self.ny)
+        sums = np.zeros(shape)
+        rp = np.zeros(shape) if reaction_plane else None
+        keys = [k for k in self.EXTRA_KEYS if reaction_plane or not k.endswith("_rp")]
+        ex = {k: np.zeros(shape) for k in keys}
+        arr = (C.c_void_p * 10)(*[ex[k].ctypes.data if k in ex else None for k in self.EXTRA_KEYS])
+        acc = None
+        if accept is not None:
+            acc = np.ascontiguousarray(np.asarray(accept, dtype=np.uint8))
+        self._check(self.lib.smc_average_profiles_extra(self.h, from_order, to_order,
+                                                        None if acc is None else acc.ctypes.data_as(C.c_void_p),
+                                                        _ptr(sums, C.c_double), _ptr(rp, C.c_double),
+                                                        C.cast(arr, C.c_void_p)))
+        return sums, rp, ex
 
     def average_profiles(self, from_order, to_order, accept=None, reaction_plane=False):
         """Sums over the ctx's accepted events of finalFactor * rho after the

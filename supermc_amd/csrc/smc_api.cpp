@@ -180,6 +180,7 @@ struct smc_ctx {
   unsigned char* avg_acc = nullptr;   // per event: included in the average
   double* avg_sum = nullptr;          // [2][avg_nord][nx][ny] sums (rotated, recentred)
   double* avg_part = nullptr;         // [kAccumSegments][nx][ny] partial sums over event segments
+  double* avg_ext = nullptr;          // [10][avg_nord][nx][ny] sums of the optional averages
   int avg_nord = 0;
   int* nmrect = nullptr;    // [cap]
   double* kln_table = nullptr;
@@ -899,10 +900,27 @@ void ensure_cm(smc_ctx* c) {
   c->cm_nu = moments_cm_units_min();
 }
 
-void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept, double* sums, double* rp_sums) {
+// the optional averages in the order of smc_avg_extra's members
+enum : int { kExTatb = 0, kExRhoBin, kExTa, kExTb, kExSpecA, kExSpecB, kExTatbRp, kExRhoBinRp, kExTaRp, kExTbRp, kExN };
+
+void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept, double* sums, double* rp_sums,
+                      const smc_avg_extra* ex) {
   if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_run first");
   if (from < 1 || to < from || to > 9) throw SmcError(SMC_ERR_ARG, "orders must satisfy 1 <= from <= to <= 9");
   if (!sums) throw SmcError(SMC_ERR_ARG, "sums is NULL");
+  double* want[kExN] = {};
+  if (ex) {
+    double* const m[kExN] = {ex->tatb,   ex->rho_binary, ex->ta,         ex->tb,    ex->spec_a,
+                             ex->spec_b, ex->tatb_rp,    ex->rho_binary_rp, ex->ta_rp, ex->tb_rp};
+    std::copy(m, m + kExN, want);
+  }
+  if (!rp_sums && (want[kExTatbRp] || want[kExRhoBinRp] || want[kExTaRp] || want[kExTbRp]))
+    throw SmcError(SMC_ERR_ARG, "the reaction-plane averages need rp_sums (the recentring pass)");
+  const bool need_t[2] = {want[kExTatbRp] || want[kExTaRp] || want[kExTbRp], want[kExTatb] || want[kExTa] || want[kExTb]};
+  const bool need_rb[2] = {want[kExRhoBinRp] != nullptr, want[kExRhoBin] != nullptr};
+  const bool need_sp = want[kExSpecA] || want[kExSpecB];
+  bool any_ex = false;
+  for (double* w : want) any_ex |= w != nullptr;
   const int nev = c->nev, nord = to - from + 1;
   const size_t N = (size_t)c->d.nx * c->d.ny;
   if (!c->avg_rot) {
@@ -910,10 +928,12 @@ void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept,
     c->avg_acc = dalloc<unsigned char>((size_t)c->cap);
     c->avg_part = dalloc<double>((size_t)kAccumSegments * N);
   }
-  if (c->avg_nord < nord) {
+  if (c->avg_nord < nord || (any_ex && !c->avg_ext)) {
     if (c->avg_sum) HIPCHK(hipFree(c->avg_sum));
-    c->avg_sum = dalloc<double>(2 * (size_t)nord * N);
-    c->avg_nord = nord;
+    if (c->avg_ext) HIPCHK(hipFree(c->avg_ext));
+    c->avg_nord = std::max(nord, c->avg_nord);
+    c->avg_sum = dalloc<double>(2 * (size_t)c->avg_nord * N);
+    c->avg_ext = any_ex ? dalloc<double>((size_t)kExN * c->avg_nord * N) : nullptr;
   }
   std::vector<unsigned char> acc(nev);
   if (accept) {
@@ -926,7 +946,30 @@ void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept,
   }
   HIPCHK(hipMemcpyAsync(c->avg_acc, acc.data(), nev, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemsetAsync(c->avg_sum, 0, 2 * (size_t)nord * N * sizeof(double), c->stream));
+  if (any_ex) HIPCHK(hipMemsetAsync(c->avg_ext, 0, (size_t)kExN * nord * N * sizeof(double), c->stream));
   if (!c->have_density) run_density(c);
+  // the spectators as a source list of their own, from the unmoved nucleons;
+  // they move with the sources below
+  EvBufs sb = c->buf;
+  if (need_sp) {
+    const size_t E = (size_t)c->cap, cap = 2 * kAMax;
+    if (!c->spec_src) {
+      c->spec_src = dalloc<int4>(E * cap);
+      c->spec_swin = dalloc<int4>(E * cap);
+      c->spec_counts = dalloc<int>(E * 8);
+    }
+    sb.src = c->spec_src;
+    sb.swin = c->spec_swin;
+    sb.counts = c->spec_counts;
+    sb.src_cap = (int)cap;
+    check_launch(launch_spectator_src(c->cfg, c->buf, sb, nev, c->stream), "k_spectator_src");
+  }
+  const int nx = c->d.nx, ny = c->d.ny;
+  auto accum = [&](const float* g, const int* counts, double ff, int slot, int o, const float* g2 = nullptr) {
+    check_launch(launch_accum_avg(g, counts, nx, ny, nev, c->avg_acc, ff, c->avg_part,
+                                  c->avg_ext + ((size_t)slot * nord + o) * N, c->stream, g2),
+                 "k_accum_avg");
+  };
   const double ff = c->p.finalFactor;
   for (int o = 0; o < nord; ++o) {
     const int n = from + o;
@@ -935,20 +978,64 @@ void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept,
       check_launch(launch_cm_angle(c->cfg, c->buf, c->grid, c->cm, c->cm_nu, n, pass, nev, c->avg_rot, c->stream),
                    "k_cm_angle");
       check_launch(launch_rotate_src(c->cfg, c->buf, c->avg_rot, nev, c->stream), "k_rotate_src");
+      if (need_sp) check_launch(launch_rotate_src(c->cfg, sb, c->avg_rot, nev, c->stream), "k_rotate_src");
       c->have_density = c->have_thickness = c->have_moments = 0;
       run_density(c);
       double* dst = c->avg_sum + ((size_t)(pass == 0 ? nord : 0) + o) * N;
       check_launch(launch_accum_avg(c->grid, c->buf.counts, c->d.nx, c->d.ny, nev, c->avg_acc, ff, c->avg_part, dst,
                                     c->stream),
                    "k_accum_avg");
+      const bool rp = pass == 0;
+      if (need_t[pass]) {  // MCnucl::calculateThickness of the moved participants
+        SmearSpec ts = thickness_spec(c);
+        ts.sparse = 1;
+        float* ta = grid_buffer(c, &c->ta);
+        float* tb = grid_buffer(c, &c->tb);
+        run_smear(c, ts, ta, tb);
+        if (want[rp ? kExTaRp : kExTa]) accum(ta, c->buf.counts, 1.0, rp ? kExTaRp : kExTa, o);
+        if (want[rp ? kExTbRp : kExTb]) accum(tb, c->buf.counts, 1.0, rp ? kExTbRp : kExTb, o);
+        if (want[rp ? kExTatbRp : kExTatb]) accum(ta, c->buf.counts, 1.0, rp ? kExTatbRp : kExTatb, o, tb);
+      }
+      if (need_rb[pass]) {  // MCnucl::calculate_rho_binary of the moved pairs
+        SmearSpec rs{};
+        rs.enabled[kSrcPair] = 1;
+        rs.win[kSrcPair] = kWin5;
+        rs.scale[kSrcPair] = c->cfg.norm;
+        rs.sparse = 1;
+        float* g = grid_buffer(c, &c->aux0);
+        run_smear(c, rs, g, nullptr);
+        accum(g, c->buf.counts, 1.0, rp ? kExRhoBinRp : kExRhoBin, o);
+      }
+      if (!rp && need_sp) {  // MCnucl::calculate_spectator_density of the moved spectators
+        SmearSpec ss{};
+        for (int k : {kSrcPartA, kSrcPartB}) {
+          ss.enabled[k] = 1;
+          ss.win[k] = kWin5;
+          ss.scale[k] = c->cfg.norm;
+        }
+        ss.acc[kSrcPartB] = 1;
+        ss.combine = 2;
+        ss.sparse = 1;
+        float* ga = grid_buffer(c, &c->aux0);
+        float* gb = grid_buffer(c, &c->aux1);
+        check_launch(launch_tile(c->cfg, sb, ss, nev, ga, gb, c->stream), "k_tile");
+        if (want[kExSpecA]) accum(ga, sb.counts, 1.0, kExSpecA, o);
+        if (want[kExSpecB]) accum(gb, sb.counts, 1.0, kExSpecB, o);
+      }
     }
   }
   std::vector<double> h(2 * (size_t)nord * N);
   HIPCHK(hipMemcpyAsync(h.data(), c->avg_sum, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> hx(any_ex ? (size_t)kExN * nord * N : 0);
+  if (any_ex)
+    HIPCHK(hipMemcpyAsync(hx.data(), c->avg_ext, hx.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   sync(c);
   for (size_t i = 0; i < (size_t)nord * N; ++i) sums[i] += h[i];
   if (rp_sums)
     for (size_t i = 0; i < (size_t)nord * N; ++i) rp_sums[i] += h[(size_t)nord * N + i];
+  for (int k = 0; k < kExN; ++k)
+    if (want[k])
+      for (size_t i = 0; i < (size_t)nord * N; ++i) want[k][i] += hx[(size_t)k * nord * N + i];
 }
 
 // ---------------------------------------------------------------- differentiable batches
@@ -1213,7 +1300,7 @@ void smc_destroy(smc_ctx* c) {
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
-                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->avg_part, c->aux0,
+                  c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->avg_part, c->avg_ext, c->aux0,
                   c->aux1, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
@@ -1537,7 +1624,16 @@ smc_status smc_average_profiles(smc_ctx* c, int from_order, int to_order, const 
                                 double* rp_sums) {
   if (!c) return SMC_ERR_ARG;
   try {
-    average_profiles(c, from_order, to_order, accept, sums, rp_sums);
+    average_profiles(c, from_order, to_order, accept, sums, rp_sums, nullptr);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_average_profiles_extra(smc_ctx* c, int from_order, int to_order, const unsigned char* accept,
+                                      double* sums, double* rp_sums, const smc_avg_extra* extra) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    average_profiles(c, from_order, to_order, accept, sums, rp_sums, extra);
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }
 }

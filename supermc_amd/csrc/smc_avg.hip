@@ -145,8 +145,12 @@ __global__ __launch_bounds__(256) void k_rotate_src(DevCfg cfg, EvBufs buf, cons
 // rectangles of 256 events at a time are staged in LDS (an excluded event
 // gets an empty one), and eight loads are issued before their adds, so each
 // wave keeps eight in flight.
-__global__ __launch_bounds__(256) void k_accum_part(const float* grid, const int* counts, int nx, int ny, int nev,
-                                                    int per_seg, const unsigned char* accept, double* part) {
+// With grid2 the added value is the product of the two grids' cells, in
+// double (T_A T_B of the optional averages).
+template <bool PROD>
+__global__ __launch_bounds__(256) void k_accum_part(const float* grid, const float* grid2, const int* counts, int nx,
+                                                    int ny, int nev, int per_seg, const unsigned char* accept,
+                                                    double* part) {
   __shared__ int2 s_rect[256];
   const int ncell = nx * ny, t = threadIdx.x;
   const int c = blockIdx.x * 256 + t, seg = blockIdx.y;
@@ -154,10 +158,13 @@ __global__ __launch_bounds__(256) void k_accum_part(const float* grid, const int
   const int ix = on ? c / ny : 0, iy = on ? c - ix * ny : 0;
   const int e0 = seg * per_seg, e1 = min(nev, e0 + per_seg);
   const float* g = grid + (on ? c : 0);
-  auto cell = [&](int e, int k) -> float {
+  const float* g2 = PROD ? grid2 + (on ? c : 0) : nullptr;
+  auto cell = [&](int e, int k) -> double {
     const int2 r = s_rect[k];
     const bool in = ix >= (r.x & 0xffff) && ix < (r.x >> 16) && iy >= (r.y & 0xffff) && iy < (r.y >> 16);
-    return in ? g[(size_t)e * ncell] : 0.f;
+    if (!in) return 0.0;
+    if constexpr (PROD) return (double)g[(size_t)e * ncell] * (double)g2[(size_t)e * ncell];
+    return (double)g[(size_t)e * ncell];
   };
   double s = 0.0;
   for (int base = e0; base < e1; base += 256) {
@@ -171,13 +178,13 @@ __global__ __launch_bounds__(256) void k_accum_part(const float* grid, const int
     if (!on) continue;
     int k = 0;
     for (; k + 8 <= n; k += 8) {
-      float v[8];
+      double v[8];
 #pragma unroll
       for (int j = 0; j < 8; ++j) v[j] = cell(base + k + j, k + j);
 #pragma unroll
-      for (int j = 0; j < 8; ++j) s += (double)v[j];
+      for (int j = 0; j < 8; ++j) s += v[j];
     }
-    for (; k < n; ++k) s += (double)cell(base + k, k);
+    for (; k < n; ++k) s += cell(base + k, k);
   }
   if (on) part[(size_t)seg * ncell + c] = s;
 }
@@ -278,14 +285,18 @@ int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int
 }
 
 int launch_accum_avg(const float* grid, const int* counts, int nx, int ny, int nev, const unsigned char* accept,
-                     double ff, double* part, double* sum, void* stream) {
+                     double ff, double* part, double* sum, void* stream, const float* grid2) {
   if (nev <= 0) return 0;
   const int ncell = nx * ny;
   const int per_seg = (nev + kAccumSegments - 1) / kAccumSegments;
   const int nseg = (nev + per_seg - 1) / per_seg;  // <= kAccumSegments: part holds kAccumSegments x ncell
   const unsigned bx = (unsigned)((ncell + 255) / 256);
-  hipLaunchKernelGGL(k_accum_part, dim3(bx, (unsigned)nseg), dim3(256), 0, (hipStream_t)stream, grid, counts, nx, ny,
-                     nev, per_seg, accept, part);
+  if (grid2)
+    hipLaunchKernelGGL(k_accum_part<true>, dim3(bx, (unsigned)nseg), dim3(256), 0, (hipStream_t)stream, grid, grid2,
+                       counts, nx, ny, nev, per_seg, accept, part);
+  else
+    hipLaunchKernelGGL(k_accum_part<false>, dim3(bx, (unsigned)nseg), dim3(256), 0, (hipStream_t)stream, grid,
+                       nullptr, counts, nx, ny, nev, per_seg, accept, part);
   hipLaunchKernelGGL(k_accum_fin, dim3(bx), dim3(256), 0, (hipStream_t)stream, part, nseg, ncell, ff, sum);
   return (int)hipGetLastError();
 }

@@ -188,8 +188,9 @@ int launch_rotate_src(const DevCfg& cfg, const EvBufs& b, const double* rot, int
 int launch_spectator_src(const DevCfg& cfg, const EvBufs& b, const EvBufs& sb, int nev, void* stream);
 // part: kAccumSegments x nx x ny doubles of scratch (segment partials)
 constexpr int kAccumSegments = 32;
+// with grid2, the sum of the cells' products grid * grid2
 int launch_accum_avg(const float* grid, const int* counts, int nx, int ny, int nev, const unsigned char* accept,
-                     double ff, double* part, double* sum, void* stream);
+                     double ff, double* part, double* sum, void* stream, const float* grid2 = nullptr);
 constexpr int kTileVariants = 4;
 int tile_units(const DevCfg& cfg);  // smearing tiles per event of cfg's variant (units of SmearSpec::cm_out)
 int tile_units_max(const DevCfg& cfg);  // the most over all variants (buffer sizing)

@@ -199,19 +199,25 @@ void MakeDensity::generate_profile_average(int nevent) {
   const bool use_sd = rdr_->getVal("use_sd") != 0, use_ed = rdr_->getVal("use_ed") != 0;
   const bool use_block = rdr_->getVal("use_block") != 0, use_4col = rdr_->getVal("use_4col") != 0;
   const bool rp = rdr_->getVal("generate_reaction_plane_avg_profile") != 0;
-  const char* extras[4] = {"output_TATB", "output_rho_binary", "output_TA", "output_spectator_density"};
-  for (const char* k : extras)
-    if (rdr_->getVal(k) != 0) {
-      std::cerr << "generate_profile_average: " << k << " = 1 is not offered by this build" << std::endl;
-      std::exit(1);
-    }
+  // the optional averages (MakeDensity.cpp:749-752)
+  const bool out_tatb = rdr_->getVal("output_TATB") != 0, out_rb = rdr_->getVal("output_rho_binary") != 0;
+  const bool out_ta = rdr_->getVal("output_TA") != 0, out_spec = rdr_->getVal("output_spectator_density") != 0;
   if (from < 1 || to < from || to > 9) {
     std::cerr << "generate_profile_average: average orders must satisfy 1 <= from <= to <= 9" << std::endl;
     std::exit(1);
   }
   const int nord = to - from + 1;
-  const size_t ncell = (size_t)nx_ * ny_;
-  std::vector<double> sums(nord * ncell, 0.0), rp_sums(rp ? nord * ncell : 0, 0.0);
+  const size_t ncell = (size_t)nx_ * ny_, n_all = nord * ncell;
+  std::vector<double> sums(n_all, 0.0), rp_sums(rp ? n_all : 0, 0.0);
+  // one array per optional average, in the order of smc_avg_extra
+  enum { kTatb, kRb, kTa, kTb, kSa, kSb, kTatbRp, kRbRp, kTaRp, kTbRp, kN };
+  const bool on_ex[kN] = {out_tatb, out_rb, out_ta, out_ta, out_spec, out_spec,
+                          out_tatb && rp, out_rb && rp, out_ta && rp, out_ta && rp};
+  std::vector<double> ex[kN];
+  for (int k = 0; k < kN; ++k) ex[k].assign(on_ex[k] ? n_all : 0, 0.0);
+  auto ptr = [&](int k) { return on_ex[k] ? ex[k].data() : nullptr; };
+  const smc_avg_extra extra{ptr(kTatb), ptr(kRb), ptr(kTa),    ptr(kTb),  ptr(kSa),
+                            ptr(kSb),   ptr(kTatbRp), ptr(kRbRp), ptr(kTaRp), ptr(kTbRp)};
   std::vector<unsigned char> acc;
   records_.clear();
   int done = 0;
@@ -230,31 +236,59 @@ void MakeDensity::generate_profile_average(int nevent) {
       records_.insert(records_.end(), rec, rec + SMC_EV_STRIDE);
       ++done;
     }
-    check(ctx_, smc_average_profiles(ctx_, from, to, acc.data(), sums.data(), rp ? rp_sums.data() : nullptr),
+    check(ctx_,
+          smc_average_profiles_extra(ctx_, from, to, acc.data(), sums.data(), rp ? rp_sums.data() : nullptr, &extra),
           "MakeDensity::generate_profile_average");
   }
   // the reference's running average over accepted events
   for (double& v : sums) v /= nevent;
   for (double& v : rp_sums) v /= nevent;
+  for (std::vector<double>& e : ex)
+    for (double& v : e) v /= nevent;
   char buffer[512];
-  const char* kinds[2] = {"sd", "ed"};
+  auto dump = [&](const char* stem, const double* g, int order) {
+    if (use_4col) {
+      std::snprintf(buffer, sizeof(buffer), "data/%s_order_%d_4col.dat", stem, order);
+      dumpDensity4Col(buffer, g, npart_last);
+    }
+    if (use_block) {
+      std::snprintf(buffer, sizeof(buffer), "data/%s_order_%d_block.dat", stem, order);
+      dumpDensityBlock(buffer, g);
+    }
+  };
+  // entropy and energy profiles (setEd equals setSd, MakeDensity.cpp:2524-2533),
+  // then TA*TB, rho_binary, TA / TB (also after the recentring) and the
+  // spectator densities (MakeDensity.cpp:1650-1900 file names)
+  const char* from_tag[2] = {"Sd", "Ed"};
+  const char* avg_tag[2] = {"sd", "ed"};
   const bool on[2] = {use_sd, use_ed};
-  for (int o = 0; o < nord; ++o)
+  char stem[256];
+  for (int o = 0; o < nord; ++o) {
+    const int order = from + o;
     for (int k = 0; k < 2; ++k) {
       if (!on[k]) continue;
       for (int r = 0; r < (rp ? 2 : 1); ++r) {
-        const double* g = (r ? rp_sums.data() : sums.data()) + o * ncell;
-        const char* tag = r ? "_RP" : "";
-        if (use_4col) {
-          std::snprintf(buffer, sizeof(buffer), "data/%sAvg%s_order_%d_4col.dat", kinds[k], tag, from + o);
-          dumpDensity4Col(buffer, g, npart_last);
-        }
-        if (use_block) {
-          std::snprintf(buffer, sizeof(buffer), "data/%sAvg%s_order_%d_block.dat", kinds[k], tag, from + o);
-          dumpDensityBlock(buffer, g);
+        std::snprintf(stem, sizeof(stem), "%sAvg%s", avg_tag[k], r ? "_RP" : "");
+        dump(stem, (r ? rp_sums.data() : sums.data()) + o * ncell, order);
+      }
+      struct { int slot, rp_slot; const char* fmt; } items[] = {
+          {kTatb, kTatbRp, "TATB_from%s%s"},
+          {kRb, kRbRp, "rho_binary_from%s%s"},
+          {kTa, kTaRp, "nuclear_thickness_TA_from%s%s"},
+          {kTb, kTbRp, "nuclear_thickness_TB_from%s%s"},
+          {kSa, -1, "spectator_density_A_from%s%s"},
+          {kSb, -1, "spectator_density_B_from%s%s"}};
+      for (const auto& it : items) {
+        if (!on_ex[it.slot]) continue;
+        std::snprintf(stem, sizeof(stem), it.fmt, from_tag[k], "");
+        dump(stem, ex[it.slot].data() + o * ncell, order);
+        if (it.rp_slot >= 0 && on_ex[it.rp_slot]) {
+          std::snprintf(stem, sizeof(stem), it.fmt, from_tag[k], "_RP");
+          dump(stem, ex[it.rp_slot].data() + o * ncell, order);
         }
       }
     }
+  }
 }
 
 void MakeDensity::generate_profile_ebe_Jet(int nevent) {

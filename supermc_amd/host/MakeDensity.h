@@ -26,11 +26,14 @@
 //                           of the profiles recentred and turned to each
 //                           order's angle, data/sdAvg_order_<n>_block.dat /
 //                           _4col (edAvg_ with use_ed; sdAvg_RP_ with
-//                           generate_reaction_plane_avg_profile).  The
-//                           optional TA*TB, binary-density, TA and
-//                           spectator averages are not offered (exit 1).
-//                           The dS/dy cut is applied once per event, on
-//                           the unturned profile.
+//                           generate_reaction_plane_avg_profile), and with
+//                           output_TATB / output_rho_binary / output_TA /
+//                           output_spectator_density the averages of T_A T_B,
+//                           the binary density, T_A and T_B (also
+//                           _RP) and the spectator densities of the moved
+//                           sources (TATB_fromSd_order_<n>, ...).  The dS/dy
+//                           cut is applied once per event, on the unturned
+//                           profile.
 //
 // Random numbers: Philox stream randomSeed; event i of a run is counter i,
 // so a run is reproducible and independent of gpu_batch_events.

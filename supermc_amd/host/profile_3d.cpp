@@ -1,220 +1,196 @@
-// profile_3d.cpp -- facade of scripts/generate_3d_profiles/profile_3d.cpp:
-// set-up and file output on the host, sampling and smearing through the
-// C-ABI (include/smc_p3d.h) on the GPU.
+// profile_3d.cpp -- the 3D profile facade: set-up and file output on the
+// host, eta / width sampling and the separable 3D smearing through the
+// C-ABI (include/smc_p3d.h) on the GPU.  The methods keep the meaning of
+// scripts/generate_3d_profiles/profile_3d.cpp (cited per method); the code
+// is organised around a library handle and per-list arrays.
 #include "profile_3d.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <ctime>
-#include <fstream>
-#include <iomanip>
 #include <iostream>
 
 namespace {
 
-// events of this path: set_variables is event 0 of the seed's stream, single
-// draws of sample_eta_distribution_from_array take events 2^40 + n
-constexpr uint64_t kSingleDrawEvent0 = 1ull << 40;
+// counter ranges of the seed's stream: the sources of set_variables are
+// event 0, single draws of sample_eta_distribution_from_array start here
+constexpr uint64_t kFirstSingleDraw = 1ull << 40;
+
+// the six per-source numbers the library reads and writes
+void pack(const participant_info& p, double* r) {
+  r[0] = p.x;
+  r[1] = p.y;
+  r[2] = p.eta;
+  r[3] = p.sigma_x;
+  r[4] = p.sigma_y;
+  r[5] = p.sigma_eta;
+}
 
 }  // namespace
 
-void profile_3d::check(int rc, const char* where) {
+void profile_3d::require(int rc, smc_p3d* h, const char* where) const {
   if (rc == SMC_OK) return;
-  cout << "profile_3d::" << where << ": " << smc_p3d_last_error(h) << " (status " << rc << ")" << endl;
-  exit(1);
+  std::printf("profile_3d::%s failed: %s (status %d)\n", where, h ? smc_p3d_last_error(h) : "no handle", rc);
+  std::exit(1);
 }
 
-// profile_3d.cpp:17-110
+// profile_3d.cpp:17-110: grid and beam set-up, the sources' transverse
+// positions (participants keep their nucleus id, collisions get -1), the
+// eta tables, then eta and widths of every source.
 profile_3d::profile_3d(vector<participant_info> part_list_in, vector<participant_info> binary_list_in,
                        int grid_nx_in, int grid_ny_in, int grid_neta_in, double grid_dx_in, double grid_dy_in,
-                       double grid_deta_in, double ecm_in, int random_flag_in)
-    : h(nullptr), h_draw(nullptr) {
-  ecm = ecm_in;
-  grid_nx = grid_nx_in;
-  grid_ny = grid_ny_in;
-  grid_neta = grid_neta_in;
-  grid_dx = grid_dx_in;
-  grid_dy = grid_dy_in;
-  grid_deta = grid_deta_in;
-  random_flag = random_flag_in;
-  smc_p3d_params p{grid_nx, grid_ny, grid_neta, grid_dx, grid_dy, grid_deta, ecm, random_flag};
-  const int rc = smc_p3d_create(&h, &p);
+                       double grid_deta_in, double ecm_in, int random_flag_in) {
+  grid_ = smc_p3d_params{grid_nx_in, grid_ny_in, grid_neta_in, grid_dx_in, grid_dy_in, grid_deta_in, ecm_in,
+                         random_flag_in};
+  const int rc = smc_p3d_create(&dev_, &grid_);
   if (rc != SMC_OK) {
-    cout << "profile_3d: invalid set-up (status " << rc << "): grid sizes, spacings, ecm > 2 GeV and "
-         << "random_flag in 0..3 are required" << endl;
-    exit(1);
+    std::printf("profile_3d: set-up refused (status %d); it needs positive grid sizes and spacings, ecm above "
+                "2 GeV and random_flag 0..3\n", rc);
+    std::exit(1);
   }
-  smc_p3d_derived d{};
-  check(smc_p3d_get_derived(h, &d, nullptr, nullptr, nullptr), "profile_3d");
-  y_beam = d.y_beam;
-  sigma_inelastic = d.sigma_inelastic;
+  require(smc_p3d_get_derived(dev_, &derived_, nullptr, nullptr, nullptr), dev_, "profile_3d");
+  // the reference's first console line (tools parse it)
+  std::cout << "ecm = " << ecm_in << " A GeV, y_beam = " << derived_.y_beam
+            << ", sigmaNN = " << derived_.sigma_inelastic * 10. << " mb. " << std::endl;
 
-  cout << "ecm = " << ecm << " A GeV, "
-       << "y_beam = " << y_beam << ", "
-       << "sigmaNN = " << sigma_inelastic * 10. << " mb. " << endl;
+  for (const participant_info& p : part_list_in) sources_[0].push_back(participant_info{p.id, p.x, p.y, 0, 0, 0, 0});
+  for (const participant_info& p : binary_list_in) sources_[1].push_back(participant_info{-1, p.x, p.y, 0, 0, 0, 0});
+  const size_t cells = (size_t)grid_.grid_neta * grid_.grid_nx * grid_.grid_ny;
+  density_[0].assign(cells, 0.f);
+  density_[1].assign(cells, 0.f);
 
-  for (size_t i = 0; i < part_list_in.size(); i++) {
-    participant_info t{};
-    t.x = part_list_in[i].x;
-    t.y = part_list_in[i].y;
-    t.id = part_list_in[i].id;
-    participant_list.push_back(t);
-  }
-  for (size_t i = 0; i < binary_list_in.size(); i++) {
-    participant_info t{};
-    t.x = binary_list_in[i].x;
-    t.y = binary_list_in[i].y;
-    t.id = -1;
-    binary_collision_list.push_back(t);
-  }
-  rho_part.assign((size_t)grid_neta * grid_nx * grid_ny, 0.f);
-  rho_binary.assign((size_t)grid_neta * grid_nx * grid_ny, 0.f);
-
-  seed = (uint64_t)time(NULL);
-  single_draws = 0;
-  cout << "random seed: " << seed << endl;
+  seed_ = (uint64_t)std::time(nullptr);
+  std::cout << "random seed: " << seed_ << std::endl;
 
   set_eta_distribution();
   set_variables();
 }
 
 profile_3d::~profile_3d() {
-  if (h) smc_p3d_destroy(h);
-  if (h_draw) smc_p3d_destroy(h_draw);
+  if (dev_) smc_p3d_destroy(dev_);
+  if (dev_draw_) smc_p3d_destroy(dev_draw_);
 }
 
-// profile_3d.cpp:133-155 (tabulated by the library at construction)
+// profile_3d.cpp:133-155: the 1000-point eta table and its two columns
+// (tabulated by the library when the handle is made)
 void profile_3d::set_eta_distribution() {
-  eta_distribution_array_eta.assign(SMC_P3D_TABLE, 0.0);
-  eta_distribution_array_projectile.assign(SMC_P3D_TABLE, 0.0);
-  eta_distribution_array_target.assign(SMC_P3D_TABLE, 0.0);
-  smc_p3d_derived d{};
-  check(smc_p3d_get_derived(h, &d, eta_distribution_array_eta.data(), eta_distribution_array_projectile.data(),
-                            eta_distribution_array_target.data()),
-        "set_eta_distribution");
-  eta_distribution_proj_max = d.proj_max;
-  eta_distribution_targ_max = d.targ_max;
+  for (vector<double>& col : table_) col.assign(SMC_P3D_TABLE, 0.0);
+  require(smc_p3d_get_derived(dev_, &derived_, table_[0].data(), table_[1].data(), table_[2].data()), dev_,
+          "set_eta_distribution");
 }
 
-// profile_3d.cpp:157-191: one draw on the device per call, from the table
-// whatever random_flag is (a handle with random_flag 1, made on first use)
+// profile_3d.cpp:157-191: one draw from the table for a source of nucleus
+// participant_id, on the device, whatever the profile's random_flag
 double profile_3d::sample_eta_distribution_from_array(int participant_id) {
-  if (!h_draw) {
-    smc_p3d_params p{grid_nx, grid_ny, grid_neta, grid_dx, grid_dy, grid_deta, ecm, 1};
-    check(smc_p3d_create(&h_draw, &p), "sample_eta_distribution_from_array");
+  if (!dev_draw_) {
+    smc_p3d_params one = grid_;
+    one.random_flag = 1;
+    require(smc_p3d_create(&dev_draw_, &one), dev_draw_, "sample_eta_distribution_from_array");
   }
   const double xy[2] = {0.0, 0.0};
-  const int id[1] = {participant_id};
-  const int off1[2] = {0, 1}, off0[2] = {0, 0};
-  double src[6];
-  const int rc = smc_p3d_sample(h_draw, 1, xy, id, off1, nullptr, off0, seed, kSingleDrawEvent0 + single_draws++,
-                                src, nullptr);
-  if (rc != SMC_OK) {
-    cout << "profile_3d::sample_eta_distribution_from_array: " << smc_p3d_last_error(h_draw) << endl;
-    exit(1);
-  }
-  return src[2];
+  const int id = participant_id;
+  const int one_source[2] = {0, 1}, no_source[2] = {0, 0};
+  double out[6];
+  require(smc_p3d_sample(dev_draw_, 1, xy, &id, one_source, nullptr, no_source, seed_, kFirstSingleDraw + draws_++,
+                         out, nullptr),
+          dev_draw_, "sample_eta_distribution_from_array");
+  return out[2];
 }
 
-// profile_3d.cpp:194-300: eta and widths of every source, on the device
+// profile_3d.cpp:194-300: eta and the three widths of every source, drawn
+// on the device from the seed's stream
 void profile_3d::set_variables() {
-  const size_t np = participant_list.size(), nb = binary_collision_list.size();
-  vector<double> pxy(2 * np), bxy(2 * nb), ps(6 * np), bs(6 * nb);
-  vector<int> pid(np);
-  for (size_t i = 0; i < np; i++) {
-    pxy[2 * i] = participant_list[i].x;
-    pxy[2 * i + 1] = participant_list[i].y;
-    pid[i] = participant_list[i].id;
-  }
-  for (size_t i = 0; i < nb; i++) {
-    bxy[2 * i] = binary_collision_list[i].x;
-    bxy[2 * i + 1] = binary_collision_list[i].y;
-  }
-  const int poff[2] = {0, (int)np}, boff[2] = {0, (int)nb};
-  check(smc_p3d_sample(h, 1, pxy.data(), pid.data(), poff, bxy.data(), boff, seed, 0, ps.data(), bs.data()),
-        "set_variables");
-  auto fill = [](vector<participant_info>& l, const vector<double>& s) {
-    for (size_t i = 0; i < l.size(); i++) {
-      l[i].eta = s[6 * i + 2];
-      l[i].sigma_x = s[6 * i + 3];
-      l[i].sigma_y = s[6 * i + 4];
-      l[i].sigma_eta = s[6 * i + 5];
+  vector<double> xy[2], out[2];
+  vector<int> ids(sources_[0].size());
+  int off[2][2];
+  for (int w = 0; w < 2; ++w) {
+    const vector<participant_info>& l = sources_[w];
+    xy[w].resize(2 * l.size());
+    out[w].resize(6 * l.size());
+    for (size_t i = 0; i < l.size(); ++i) {
+      xy[w][2 * i] = l[i].x;
+      xy[w][2 * i + 1] = l[i].y;
     }
-  };
-  fill(participant_list, ps);
-  fill(binary_collision_list, bs);
+    off[w][0] = 0;
+    off[w][1] = (int)l.size();
+  }
+  for (size_t i = 0; i < ids.size(); ++i) ids[i] = sources_[0][i].id;
+  require(smc_p3d_sample(dev_, 1, xy[0].data(), ids.data(), off[0], xy[1].data(), off[1], seed_, 0, out[0].data(),
+                         out[1].data()),
+          dev_, "set_variables");
+  for (int w = 0; w < 2; ++w)
+    for (size_t i = 0; i < sources_[w].size(); ++i) {
+      const double* r = &out[w][6 * i];
+      sources_[w][i].eta = r[2];
+      sources_[w][i].sigma_x = r[3];
+      sources_[w][i].sigma_y = r[4];
+      sources_[w][i].sigma_eta = r[5];
+    }
 }
 
-void profile_3d::smear(const vector<participant_info>& list, vector<float>& rho) {
-  vector<double> src(6 * list.size());
-  for (size_t i = 0; i < list.size(); i++) {
-    const participant_info& p = list[i];
-    const double r[6] = {p.x, p.y, p.eta, p.sigma_x, p.sigma_y, p.sigma_eta};
-    for (int k = 0; k < 6; k++) src[6 * i + k] = r[k];
-  }
-  const int off[2] = {0, (int)list.size()};
-  vector<float> add(rho.size());
-  check(smc_p3d_smear(h, 1, src.data(), off, add.data()), "generate_3d_profile");
-  for (size_t i = 0; i < rho.size(); i++) rho[i] += add[i];  // the reference accumulates (+=)
+// smears source list `which` and adds it to that list's density (the
+// reference's generate_* add into their arrays)
+void profile_3d::add_smeared(int which) {
+  const vector<participant_info>& l = sources_[which];
+  vector<double> src(6 * l.size());
+  for (size_t i = 0; i < l.size(); ++i) pack(l[i], &src[6 * i]);
+  const int off[2] = {0, (int)l.size()};
+  vector<float> grid(density_[which].size());
+  require(smc_p3d_smear(dev_, 1, src.data(), off, grid.data()), dev_, "generate_3d_profile");
+  std::transform(grid.begin(), grid.end(), density_[which].begin(), density_[which].begin(),
+                 [](float a, float b) { return a + b; });
 }
 
 // profile_3d.cpp:302-344
-void profile_3d::generate_3d_profile() { smear(participant_list, rho_part); }
+void profile_3d::generate_3d_profile() { add_smeared(0); }
 
 // profile_3d.cpp:346-400
-void profile_3d::generate_3d_profile_binary() { smear(binary_collision_list, rho_binary); }
+void profile_3d::generate_3d_profile_binary() { add_smeared(1); }
 
-// profile_3d.cpp:402-444: one line per (eta, x), ny values each
+// The files of profile_3d.cpp:402-434: w_part * participant density +
+// w_bin * collision density per cell, one line per (eta slice, x row) with
+// the row's ny values, each as %18.8e followed by three spaces (the
+// reference's iostream scientific, width 18, precision 8).
+void profile_3d::write_grid(const string& filename, double w_part, double w_bin) const {
+  std::FILE* f = std::fopen(filename.c_str(), "w");
+  if (!f) {
+    std::printf("profile_3d: cannot write %s\n", filename.c_str());
+    std::exit(1);
+  }
+  const size_t ny = (size_t)grid_.grid_ny, rows = (size_t)grid_.grid_neta * grid_.grid_nx;
+  for (size_t r = 0; r < rows; ++r) {
+    for (size_t k = 0; k < ny; ++k) {
+      const size_t c = r * ny + k;
+      const double v = w_part * (double)density_[0][c] + (w_bin != 0.0 ? w_bin * (double)density_[1][c] : 0.0);
+      std::fprintf(f, "%18.8e   ", v);
+    }
+    std::fputc('\n', f);
+  }
+  std::fclose(f);
+}
+
+// entropy density (1 - alpha)/2 participants + alpha collisions
 void profile_3d::output_3d_entropy_profile(string filename, double alpha) {
-  ofstream of(filename.c_str());
-  for (int i = 0; i < grid_neta; i++) {
-    for (int j = 0; j < grid_nx; j++) {
-      for (int k = 0; k < grid_ny; k++) {
-        const size_t c = ((size_t)i * grid_nx + j) * grid_ny + k;
-        double sd_local = ((double)rho_part[c] * (1. - alpha) / 2. + (double)rho_binary[c] * alpha);
-        of << scientific << setw(18) << setprecision(8) << sd_local << "   ";
-      }
-      of << endl;
-    }
-  }
-  of.close();
+  write_grid(filename, (1. - alpha) / 2., alpha);
 }
 
-void profile_3d::output_3d_rhob_profile(string filename) {
-  ofstream of(filename.c_str());
-  for (int i = 0; i < grid_neta; i++) {
-    for (int j = 0; j < grid_nx; j++) {
-      for (int k = 0; k < grid_ny; k++) {
-        double rhob_local = rho_part[((size_t)i * grid_nx + j) * grid_ny + k];
-        of << scientific << setw(18) << setprecision(8) << rhob_local << "   ";
-      }
-      of << endl;
-    }
-  }
-  of.close();
-}
+// net-baryon density: the participants' density
+void profile_3d::output_3d_rhob_profile(string filename) { write_grid(filename, 1.0, 0.0); }
 
-// profile_3d.cpp:402-434 (the reference's helper; the device sampler runs
-// the same search)
+// The table search of profile_3d.cpp:436-470, on a table sorted upwards:
+// the largest index i with A[i] < value (0 when value equals A[0]); a value
+// outside [A[0], A[end]] gives -1 with skip_out_of_range, else the program
+// stops (exit -1).
 long profile_3d::binarySearch(vector<double>* A, double value, bool skip_out_of_range) {
-  int length = A->size();
-  int idx_i = 0, idx_f = length - 1, idx;
-  if (value > (*A)[idx_f]) {
+  const vector<double>& t = *A;
+  if (t.empty() || value < t.front() || value > t.back()) {
     if (skip_out_of_range) return -1;
-    cout << "binarySearch: desired value is too large, exceeding the end of the table." << endl;
-    exit(-1);
+    std::printf("profile_3d::binarySearch: %g lies outside the table [%g, %g]\n", value,
+                t.empty() ? 0.0 : t.front(), t.empty() ? 0.0 : t.back());
+    std::exit(-1);
   }
-  if (value < (*A)[idx_i]) {
-    if (skip_out_of_range) return -1;
-    cout << "binarySearch: desired value is too small, exceeding the beginning of table." << endl;
-    exit(-1);
-  }
-  idx = (int)floor((idx_f + idx_i) / 2.);
-  while ((idx_f - idx_i) > 1) {
-    if ((*A)[idx] < value) idx_i = idx;
-    else idx_f = idx;
-    idx = (int)floor((idx_f + idx_i) / 2.);
-  }
-  return idx_i;
+  const long first_not_below = std::lower_bound(t.begin(), t.end(), value) - t.begin();
+  return std::max(0L, first_not_below - 1);
 }

@@ -1,8 +1,10 @@
-// profile_3d.h -- the reference's profile_3d class
-// (scripts/generate_3d_profiles/profile_3d.h) over the device path of
-// include/smc_p3d.h: same constructor, methods and output files.  The grids
-// are computed on the GPU; GSL's time-seeded stream is replaced by Philox
-// keyed by the same time-derived seed (printed as the reference prints it).
+// profile_3d.h -- the 3D (x, y, eta) profile class of
+// scripts/generate_3d_profiles (profile_3d.h) on the device path of
+// include/smc_p3d.h.  Its public constructor, methods and participant_info
+// record are the reference's interface; the state behind them is this
+// facade's own: a handle on the library, the eta tables it returns, and the
+// two grids.  GSL's time-seeded stream becomes Philox keyed by the same
+// time-derived seed (printed as "random seed: ...").
 #ifndef profile_3d_H
 #define profile_3d_H
 
@@ -21,35 +23,13 @@ struct participant_info {
 };
 
 class profile_3d {
- private:
-  vector<participant_info> participant_list;
-  vector<participant_info> binary_collision_list;
-
-  int grid_nx, grid_ny, grid_neta;
-  double grid_dx, grid_dy, grid_deta;
-  vector<float> rho_part, rho_binary;  // [neta][nx][ny]
-
-  double eta_distribution_proj_max, eta_distribution_targ_max;
-  vector<double> eta_distribution_array_eta;
-  vector<double> eta_distribution_array_projectile;
-  vector<double> eta_distribution_array_target;
-
-  double ecm, y_beam, sigma_inelastic;
-
-  int random_flag;
-  uint64_t seed;
-  uint64_t single_draws;  // draws made by sample_eta_distribution_from_array
-  smc_p3d* h;
-  smc_p3d* h_draw;  // random_flag 1 handle for single table draws
-
-  void check(int rc, const char* where);
-  void smear(const vector<participant_info>& list, vector<float>& rho);
-
  public:
   profile_3d(vector<participant_info> part_list_in, vector<participant_info> binary_list_in, int grid_nx_in,
              int grid_ny_in, int grid_neta_in, double grid_dx_in, double grid_dy_in, double grid_deta_in,
              double ecm_in, int random_flag_in);
   ~profile_3d();
+  profile_3d(const profile_3d&) = delete;
+  profile_3d& operator=(const profile_3d&) = delete;
 
   void set_variables();
   void set_eta_distribution();
@@ -62,9 +42,29 @@ class profile_3d {
   long binarySearch(vector<double>* A, double value, bool skip_out_of_range);
 
   // the sources after set_variables (eta and widths filled in)
-  const vector<participant_info>& participants() const { return participant_list; }
-  const vector<participant_info>& binary_collisions() const { return binary_collision_list; }
-  uint64_t random_seed() const { return seed; }
+  const vector<participant_info>& participants() const { return sources_[0]; }
+  const vector<participant_info>& binary_collisions() const { return sources_[1]; }
+  uint64_t random_seed() const { return seed_; }
+
+ private:
+  // library handles: the profile's own, and one with random_flag 1 for the
+  // single table draws of sample_eta_distribution_from_array
+  smc_p3d* dev_ = nullptr;
+  smc_p3d* dev_draw_ = nullptr;
+  smc_p3d_params grid_{};
+  smc_p3d_derived derived_{};  // y_beam, sigma_inel, table maxima
+  // eta, projectile and target columns of the 1000-point table
+  vector<double> table_[3];
+  // sources: [0] participants, [1] binary collisions
+  vector<participant_info> sources_[2];
+  // densities of both source lists, [neta][nx][ny]
+  vector<float> density_[2];
+  uint64_t seed_ = 0;
+  uint64_t draws_ = 0;  // single draws made so far
+
+  void require(int rc, smc_p3d* h, const char* where) const;
+  void add_smeared(int which);
+  void write_grid(const string& filename, double w_part, double w_bin) const;
 };
 
 #endif

@@ -293,12 +293,39 @@ def test_driver_average_profiles_match_c_abi(tmp_path):
         col = _read_table(tmp_path / "data" / f"sdAvg_order_{n}_4col.dat")
         assert col.shape == (261 * 261, 4)
         np.testing.assert_allclose(col[:, 3], sums[o].ravel() / 10, rtol=1e-10)
-    # an optional average this build does not offer stops the run
-    _write_params(tmp_path, operation=3, nev=2, average_from_order=2, average_to_order=2,
-                  generate_reaction_plane_avg_profile=0, output_TATB=1, output_rho_binary=0, output_TA=0,
-                  output_spectator_density=0)
+
+
+@pytest.mark.gpu
+def test_driver_average_profiles_with_optional_averages(tmp_path):
+    """operation 3 with the reference parameter file's optional averages on
+    (output_TATB, output_rho_binary, output_TA, output_spectator_density,
+    parameters.dat:136-140): every file of MakeDensity.cpp:1650-1900 is
+    written (Sd and Ed, RP where the reference has one) and holds the
+    average of smc_average_profiles_extra's sums."""
+    _need(EXE)
+    _write_params(tmp_path, operation=3, nev=8, average_from_order=2, average_to_order=2,
+                  generate_reaction_plane_avg_profile=1, output_TATB=1, output_rho_binary=1, output_TA=1,
+                  output_spectator_density=1, use_ed=1, use_4col=0, gpu_batch_events=5)
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
-    assert r.returncode == 1 and "output_TATB" in r.stderr
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=8)
+    with Context(p) as ctx:
+        ctx.run(seed=1234, event0=0, nev=8)
+        _, _, ex = ctx.average_profiles_extra(2, 2, reaction_plane=True)
+    names = dict(tatb="TATB_from{k}", rho_binary="rho_binary_from{k}", ta="nuclear_thickness_TA_from{k}",
+                 tb="nuclear_thickness_TB_from{k}", spec_a="spectator_density_A_from{k}",
+                 spec_b="spectator_density_B_from{k}")
+    for key, stem in names.items():
+        for k in ("Sd", "Ed"):
+            blk = _read_table(tmp_path / "data" / (stem.format(k=k) + "_order_2_block.dat"))
+            np.testing.assert_allclose(blk, ex[key][0] / 8, rtol=1e-10, atol=1e-300)
+            rp_file = tmp_path / "data" / (stem.format(k=k) + "_RP_order_2_block.dat")
+            if key.startswith("spec"):
+                assert not rp_file.exists()
+            else:
+                np.testing.assert_allclose(_read_table(rp_file), ex[key + "_rp"][0] / 8, rtol=1e-10, atol=1e-300)
 
 
 @pytest.mark.gpu
