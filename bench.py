@@ -16,6 +16,7 @@ loop, no file output) on a bounded sample, one process per core.
 """
 import argparse
 import json
+import math
 import multiprocessing as mp
 import os
 import sys
@@ -27,8 +28,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK = 157.3e12          # MI355X FP32 (MFMA, and VALU with packed FP32)
-FP32_PEAK_NONPACKED = 78.6e12  # one FP32 FMA per lane per cycle: the code here is built without packed FP32
+FP32_PEAK_NONPACKED = 78.6e12  # one FP32 FMA per lane per 4 cycles per wave64: the code is built without packed FP32
 HBM_PEAK = 8.0e12
+BF16_MFMA_PEAK = 2.5e15       # dense BF16 MFMA
+
+
+def mfma_peak(smear_path):
+    """The matrix-core peak of a smearing path's MFMA precision (FP32 MFMA
+    for path 1, BF16 for the split-precision paths 3 and 4)."""
+    return BF16_MFMA_PEAK if smear_path in (3, 4) else FP32_PEAK
 
 
 SMEAR_PATHS = ["separable VALU tiles", "FP32 MFMA Gx.diag(w).Gy^T + exact-cut pass", "per-cell exponentials",
@@ -45,32 +53,38 @@ def bench_params(nev, smear_path=0):
         dy=0.1, finalFactor=1.0, ecc_from_order=2, ecc_to_order=5, max_events=nev, max_pairs=4096, smear_path=smear_path)
 
 
+CPU_PBPB = dict(A=208, B=208, R_a=6.62, a_a=0.546, R_b=6.62, a_b=0.546, dmin=0.4, siginNN=64.0, w=0.5, alpha=0.14,
+                gamma_theta=1.25, fluct=1, bmin=0.0, bmax=20.0, npmin=2, npmax=500, maxx=13.0, dx=0.1, from_order=2,
+                to_order=5)
+CPU_AUAU = dict(CPU_PBPB, A=197, B=197, R_a=6.38, a_a=0.535, R_b=6.38, a_b=0.535, siginNN=42.0, fluct=0)
+
+
 def _cpu_worker(args):
-    seed, nev = args
+    seed, nev = args[:2]
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    cfg = oracle.OrcEventCfg(A=208, B=208, R_a=6.62, a_a=0.546, R_b=6.62, a_b=0.546, dmin=0.4, siginNN=64.0,
-                             w=0.5, alpha=0.14, gamma_theta=1.25, fluct=1, bmin=0.0, bmax=20.0, npmin=2,
-                             npmax=500, maxx=13.0, dx=0.1, from_order=2, to_order=5)
+    cfg = oracle.OrcEventCfg(**(args[2] if len(args) > 2 else CPU_PBPB))
     t = time.perf_counter()
     n, _ = oracle.run_events(cfg, seed, nev)
     return n, time.perf_counter() - t
 
 
-def cpu_baseline(seconds):
+def cpu_baseline(seconds, cfg=CPU_PBPB, system="Pb+Pb"):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     oracle.lib()  # builds the oracle if needed, before forking
-    cores = max(1, min(16, os.cpu_count() or 1))
-    n1, t1 = _cpu_worker((1, 8))
+    # the host cores this process may use, at most the 16 a one-GPU box
+    # grants its command
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    n1, t1 = _cpu_worker((1, 8, cfg))
     per_core = max(4, int(seconds * n1 / t1))
     t = time.perf_counter()
     with mp.get_context("fork").Pool(cores) as pool:
-        res = pool.map(_cpu_worker, [(1000 + i, per_core) for i in range(cores)])
+        res = pool.map(_cpu_worker, [(1000 + i, per_core, cfg) for i in range(cores)])
     wall = time.perf_counter() - t
     tot = sum(r[0] for r in res)
     return dict(value=tot / wall, unit="events/s", cores=cores, kind="port",
-                sample=f"{tot} Pb+Pb events ({per_core} per core) through the FP64 oracle event loop "
+                sample=f"{tot} {system} events ({per_core} per core) through the FP64 oracle event loop "
                        f"(sample -> collide -> T_A/T_B -> density -> eps_n, no file output), {wall:.1f} s wall")
 
 
@@ -104,8 +118,159 @@ def time_kln(nev, steps, seed):
         # launch (k_tile with the KLN epilogue), timed as smearing
         ker = {name: ctx.timing(kid)[0] / max(ctx.timing(kid)[1], 1)
                for kid, name in ((0, "k_event"), (1, "k_tile(T_A,T_B -> dN/dy table)"), (2, "k_moments"))}
+        fp = footprint_cells(ctx.derived.width, ctx.params.dx, -ctx.params.maxx)
+        table = (ctx.kln_make_table(), ctx.derived.kln_dT)
+    # useful FP32 work of the fused launch: 2 flops per (participant, cell of
+    # its 5w disk) into T_A or T_B; the FP64 table step (about 25 flops per
+    # cell of the rectangle) is not counted
+    work = 2.0 * float(ev[:, 54].sum()) * fp["disk"]
+    tf = work / (ker["k_tile(T_A,T_B -> dN/dy table)"] * 1e-3) / 1e12
     return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, kernels_avg_ms=ker,
-                failed_events=int((ev[:, 57] != 0).sum()), mean_eps2=float(np.hypot(ev[:, 2], ev[:, 12]).mean()))
+                failed_events=int((ev[:, 57] != 0).sum()), mean_eps2=float(np.hypot(ev[:, 2], ev[:, 12]).mean()),
+                roofline_smear=dict(achieved_tflops=tf, frac=tf * 1e12 / FP32_PEAK,
+                                    frac_nonpacked=tf * 1e12 / FP32_PEAK_NONPACKED,
+                                    work="2 flops x Npart x %.1f disk cells" % fp["disk"]),
+                _table=table)
+
+
+def config1_params(nev):
+    """Plain MC-Glauber Au+Au: WS R 6.38 fm, a 0.535 fm, hard core 0.4 fm,
+    A 197, 42 mb, b on 0-20 fm, w 0.5 fm, alpha 0.14, no fluctuations,
+    261^2 cells of 0.1 fm, eps_2..eps_5."""
+    from supermc_amd import default_params
+    return default_params(
+        which_mc_model=5, sub_model=1, shape_of_nucleons=4, gauss_nucl_width=0.5, shape_of_entropy=2,
+        collision_criterion=1, Aproj=197, Atarg=197, proj_ws_R=6.38, proj_ws_a=0.535, targ_ws_R=6.38,
+        targ_ws_a=0.535, nucleon_min_dist=0.4, sigma_nn_mb=42.0, alpha=0.14, cc_fluctuation_model=0, bmin=0.0,
+        bmax=20.0, Npmin=2, Npmax=500, maxx=13.0, maxy=13.0, dx=0.1, dy=0.1, finalFactor=1.0, ecc_from_order=2,
+        ecc_to_order=5, max_events=nev)
+
+
+def time_config1(nev, steps, seed):
+    """1000 Au+Au events from a fixed seed with the grids as output: each
+    step samples, collides, smears and takes eps_n (records to host), then
+    copies every event's entropy grid to host memory, FP32 (smc_density)
+    or FP64 with the reference's double arithmetic (smc_density_f64)."""
+    from supermc_amd import Context
+    out = {}
+    with Context(config1_params(nev)) as ctx:
+        N = ctx.nx * ctx.ny
+        g32 = np.empty((nev, ctx.nx, ctx.ny), np.float32)
+        g64 = np.empty((nev, ctx.nx, ctx.ny))
+        ctx.run(seed, 0, nev)
+        ctx.density(out=g32)
+        ctx.density_f64(out=g64)
+        ctx.synchronize()
+        for name, fetch in (("fp32_grids", lambda: ctx.density(out=g32)), ("fp64_grids", lambda: ctx.density_f64(out=g64))):
+            t0 = time.perf_counter()
+            for k in range(steps):
+                ctx.run(seed, k * nev, nev)
+                fetch()
+            ctx.synchronize()
+            out[name + "_events_per_s"] = nev * steps / (time.perf_counter() - t0)
+        ctx.set_timing(True)
+        ctx.density_f64(out=g64)
+        ms, n = ctx.timing(5)
+        ev = ctx.fetch_events(nev)
+        sq = square_window_cells(ctx.derived.width, ctx.params.dx, -ctx.params.maxx)
+    terms = float(ev[:, 54].sum()) * sq["box"] + float(ev[:, 55].sum()) * sq["win5"]
+    k64 = ms / max(n, 1)
+    out.update(events=nev, steps=steps, grid=[261, 261], mean_npart=float(ev[:, 54].mean()),
+               mean_ncoll=float(ev[:, 55].mean()), k_grid64_ms=k64,
+               k_grid64_cell_terms_per_s=terms / (k64 * 1e-3),
+               k_grid64_write_gbps=nev * N * 8 / (k64 * 1e-3) / 1e9,
+               grid_bytes_per_event={"fp32": N * 4, "fp64": N * 8},
+               note="wall time incl. pageable device-to-host copies of every grid; k_grid64 evaluates exp per "
+                    "(source, window cell) in FP64, FP64 VALU bound")
+    return out
+
+
+def time_kln_direct(nev, steps, seed):
+    """MC-KLN Pb+Pb with dN/dy from the k_T-factorised quadrature in every
+    cell (kln_direct = 1): 32 pT x 32 kT x 48 phi Gauss-Legendre nodes, phi
+    and 2 pi - phi merged, so 32 x 768 node terms per cell with T_A, T_B > 0.
+    A node term is 3 multiplies, 2 minima and one FMA in FP32: counted as
+    5 flops."""
+    from supermc_amd import Context
+    p = kln_params(nev)
+    p.kln_direct = 1
+    with Context(p) as ctx:
+        ctx.run(seed, 0, nev, fetch=False)
+        ctx.synchronize()
+        ctx.set_timing(True)
+        t0 = time.perf_counter()
+        for k in range(steps):
+            ctx.run(seed, (k + 1) * nev, nev, fetch=False)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        ms, n = ctx.timing(3)
+        s, _, _ = ctx.density()
+        active = float((s > 0).sum())
+        nodes = p.kln_n_pt * p.kln_n_kt * (p.kln_n_phi // 2)
+    kms = ms / max(n, 1)
+    tf = 5.0 * active * nodes / (kms * 1e-3) / 1e12
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, k_kln_direct_ms=kms,
+                active_cells_per_event=active / nev, node_terms_per_cell=nodes,
+                roofline=dict(bound="valu", achieved_tflops=tf, frac=tf * 1e12 / FP32_PEAK,
+                              frac_nonpacked=tf * 1e12 / FP32_PEAK_NONPACKED,
+                              work="5 flops per (cell with T_A, T_B > 0, node)"))
+
+
+def _cpu_kln_worker(args):
+    """The oracle's MC-KLN event loop on one core: b, both nuclei, the disk
+    collisions, T_A and T_B, dN/dy from the table (sixPoint2dInterp per
+    cell), eps_2..5; events with Ncoll = 0 or Npart outside [2, 500] are
+    redrawn (MakeDensity.cpp:2143-2162)."""
+    seed, nev, table, dT = args
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    rng = oracle.Rng(seed)
+    g = oracle.grid(13.0, 0.1)
+    w, dsq = 0.5, oracle.lib().orc_dsq(64.0)
+    t = time.perf_counter()
+    done = 0
+    while done < nev:
+        b = 20.0 * math.sqrt(rng.uniform())
+        xa = rng.populate(208, 6.62, 0.546, 0.4, b / 2)
+        xb = rng.populate(208, 6.62, 0.546, 0.4, -b / 2)
+        pa, pb = oracle.sort_nucleus(xa, w), oracle.sort_nucleus(xb, w)
+        col = oracle.collide(xa[pa], xb[pb], w, dsq)
+        npart = col["npart_a"] + col["npart_b"]
+        if col["ncoll"] == 0 or not 2 <= npart <= 500:
+            continue
+        ia, ib = col["order_a"], col["order_b"]
+        ta, tb = oracle.thickness(g, w, xa[pa][ia]), oracle.thickness(g, w, xb[pb][ib])
+        s, _ = oracle.density_kln_table(g, ta, tb, table, dT)
+        boxes = np.array([oracle.particle_box(x, y, w) for x, y in np.concatenate([xa[pa][ia], xb[pb][ib]])])
+        oracle.moments(g, s, boxes, col["ncoll"], 2, 5)
+        done += 1
+    return done, time.perf_counter() - t
+
+
+def cpu_kln_baseline(seconds, table, dT, seed):
+    """MC-KLN Pb+Pb on the host cores through the oracle (one process per
+    core), per-event work only; the one-time dN/dy table (211 x 211 points
+    of the 32 x 32 x 48 quadrature) is priced from a timed sample of points."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    oracle.lib()
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    n1, t1 = _cpu_kln_worker((seed, 2, table, dT))
+    per_core = max(2, int(seconds * n1 / t1))
+    t = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        res = pool.map(_cpu_kln_worker, [(seed + 1 + i, per_core, table, dT) for i in range(cores)])
+    wall = time.perf_counter() - t
+    tot = sum(r[0] for r in res)
+    k = oracle.kln(2760.0, 0.138)
+    t = time.perf_counter()
+    for i in range(3):
+        oracle.kln_dndy_gl(k, 0.5 + i, 1.0 + 0.5 * i, 32, 32, 48)
+    per_point = (time.perf_counter() - t) / 3
+    return dict(value=tot / wall, unit="events/s", cores=cores, kind="port",
+                sample=f"{tot} MC-KLN Pb+Pb events ({per_core} per core) through the FP64 oracle (sample -> "
+                       f"collide -> T_A/T_B -> table dN/dy -> eps_n), {wall:.1f} s wall",
+                table_build_s_on_these_cores=per_point * 211 * 211 / cores)
 
 
 def p3d_lists(nev, seed):
@@ -180,13 +345,18 @@ def mixed_params(nev, A, B, nuclei):
         max_pairs=8192, **nuclei)
 
 
-def smear_roofline(ev, fp, smear_ms, nev):
+def smear_roofline(ev, fp, smear_ms, nev, smear_path=0):
     """Useful FP32 work of the exact-footprint smearing (2 flops per source
     and footprint cell, fp = footprint_cells) over a batch's average
-    smearing time, against MI355X's FP32 figure and the plain-FMA rate."""
+    smearing time, against MI355X's FP32 figure and the plain-FMA rate
+    (VALU tiles), or against the matrix-core peak of the MFMA precision."""
     work = 2.0 * (float(ev[:, 54].sum()) * fp["box"] + float(ev[:, 55].sum()) * fp["disk"]) / len(ev) * nev
     tf = work / (smear_ms * 1e-3) / 1e12
-    return dict(achieved_tflops=tf, frac=tf * 1e12 / FP32_PEAK, frac_nonpacked=tf * 1e12 / FP32_PEAK_NONPACKED)
+    if smear_path in (1, 3, 4):
+        return dict(bound="mfma", achieved_tflops=tf, peak_tflops=mfma_peak(smear_path) / 1e12,
+                    frac=tf * 1e12 / mfma_peak(smear_path))
+    return dict(bound="valu", achieved_tflops=tf, frac=tf * 1e12 / FP32_PEAK,
+                frac_nonpacked=tf * 1e12 / FP32_PEAK_NONPACKED)
 
 
 def time_mixed(nev, total, seed, smear_path):
@@ -216,7 +386,21 @@ def time_mixed(nev, total, seed, smear_path):
     return dict(events_per_s=total / dt, events=total, events_per_batch=nev, grid=[521, 521],
                 smear_path=SMEAR_PATHS[smear_path], systems=[m[0] for m in MIXED], mean_npart=npart,
                 failed_events=int((rec[:, 57] != 0).sum()), kernels_avg_ms_per_batch=ker,
-                roofline_smear=smear_roofline(rec[:nev], fp, ker["k_smear"], nev))
+                roofline_smear=smear_roofline(rec[:nev], fp, ker["k_smear"], nev, smear_path))
+
+
+def time_systems(nev, seed):
+    """k_event per batch of each mixed-workload system alone (521^2), with
+    one and with two attempts per sampling pass (the events are the same)."""
+    from supermc_amd import Context
+    out = {}
+    for name, A, B, nuc in MIXED:
+        with Context(mixed_params(nev, A, B, nuc)) as ctx:
+            ctx.run(seed, 0, nev, fetch=False)
+            _, ms = ctx.autotune_attempts(reps=3)
+        out[name] = {"k_event_ms_1_attempt_per_pass": float(ms[0]), "k_event_ms_2_attempts_per_pass": float(ms[1]),
+                     "us_per_event": float(min(ms)) * 1e3 / nev}
+    return out
 
 
 def time_avg(nev, steps, seed):
@@ -359,7 +543,11 @@ def time_graph(seed):
 
 def time_adjoint(nev, steps, seed):
     """Differentiable Pb+Pb batch: forward (|eps_2|, |eps_3|, dS/dy) plus
-    the adjoint d/d(w, R, a, alpha, k), per event."""
+    the adjoint d/d(w, R, a, alpha, k), per event.  The roofline figures
+    count, per (source, cell of its integer window -- no disk cut on this
+    path): 2 flops in the forward smearing and 10 in the backward gather
+    (the Gaussian product, the cotangent product and the four weighted
+    sums), against the FP32 figures."""
     from supermc_amd import Context
     with Context(bench_params(nev)) as ctx:
         th0 = ctx.diff_setup(seed, 0, nev)
@@ -367,14 +555,31 @@ def time_adjoint(nev, steps, seed):
         ctx.diff_forward(th0)
         ctx.diff_backward(cot)
         ctx.synchronize()
+        ctx.set_timing(True)
         t0 = time.perf_counter()
         for _ in range(steps):
             ctx.diff_forward(th0)
             g = ctx.diff_backward(cot)
         ctx.synchronize()
         dt = time.perf_counter() - t0
-    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps,
-                grad_eps2_plus_eps3=dict(zip(("w", "R", "a", "alpha", "k"), map(float, g))))
+        ker = {name: ctx.timing(kid)[0] / max(ctx.timing(kid)[1], 1)
+               for kid, name in ((0, "k_diff_pos+src"), (1, "k_tile (no cut)"), (2, "k_diff_moments"),
+                                 (4, "k_diff_cot+gather+chain"))}
+        npart = ncoll = 0
+        for e in range(0, nev, max(1, nev // 256)):
+            nuc = ctx.nucleons(e)
+            npart += int((nuc["wound_proj"] >= 0).sum() + (nuc["wound_targ"] >= 0).sum())
+            ncoll += len(ctx.pairs(e)[0])
+        frac_ev = len(range(0, nev, max(1, nev // 256))) / nev
+        sq = square_window_cells(ctx.derived.width, ctx.params.dx, -ctx.params.maxx)
+        cells = (npart * sq["box"] + ncoll * sq["win5"]) / frac_ev
+    fw = 2.0 * cells / (ker["k_tile (no cut)"] * 1e-3) / 1e12
+    bw = 10.0 * cells / (ker["k_diff_cot+gather+chain"] * 1e-3) / 1e12
+    return dict(events_per_s=nev * steps / dt, events_per_step=nev, steps=steps, kernels_avg_ms=ker,
+                grad_eps2_plus_eps3=dict(zip(("w", "R", "a", "alpha", "k"), map(float, g))),
+                roofline=dict(forward_tflops=fw, forward_frac=fw * 1e12 / FP32_PEAK,
+                              backward_tflops=bw, backward_frac=bw * 1e12 / FP32_PEAK,
+                              backward_frac_nonpacked=bw * 1e12 / FP32_PEAK_NONPACKED))
 
 
 def footprint_cells(w, dx, xmin, seed=1, n=4000):
@@ -395,6 +600,21 @@ def footprint_cells(w, dx, xmin, seed=1, n=4000):
     return out
 
 
+def square_window_cells(w, dx, xmin, seed=1, n=4000):
+    """Mean cells of the integer windows without a disk cut: the +-4w
+    participant box and the +-5w window (the differentiable path)."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for kind, h in (("win5", 5 * w), ("box", 4 * w)):
+        tot = 0
+        for x, y in rng.uniform(-3, 3, (n, 2)):
+            nxw = int((x + h - xmin) / dx) - int((x - h - xmin) / dx)
+            nyw = int((y + h - xmin) / dx) - int((y - h - xmin) / dx)
+            tot += nxw * nyw
+        out[kind] = tot / n
+    return out
+
+
 def copy_probe(nbytes=1 << 28):
     """One device-to-device copy of nbytes (a known-size kernel for scaling
     rocprofv3 FETCH_SIZE / WRITE_SIZE readings)."""
@@ -408,20 +628,22 @@ def copy_probe(nbytes=1 << 28):
 
 def eps_error_vs_fp64(ctx, ev, n):
     """Largest |eps_n - eps_n(FP64 oracle)| (n = 2..5, r^2 and r^n weights)
-    over the first n events of the last batch, the oracle fed the same
-    nucleons and Gamma factors."""
+    over n events spread evenly through the last batch, the oracle fed the
+    same nucleons and Gamma factors."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import parity_helpers as H
     worst = {"eps_r2": 0.0, "eps_rn": 0.0}
-    for e in range(min(n, ctx.nev)):
+    picks = np.unique(np.linspace(0, ctx.nev - 1, min(n, ctx.nev)).astype(int))
+    for e in picks:
         o = H.oracle_event(ctx, e)
         m = o["mom"]
         for k in range(2, 6):
             worst["eps_r2"] = max(worst["eps_r2"], abs(ev[e, k] - m["mom_re"][k]), abs(ev[e, 10 + k] - m["mom_im"][k]))
             worst["eps_rn"] = max(worst["eps_rn"], abs(ev[e, 20 + k] - m["momp_re"][k]),
                                   abs(ev[e, 30 + k] - m["momp_im"][k]))
-    worst["events"] = min(n, ctx.nev)
+    worst["events"] = len(picks)
+    worst["spread"] = "events evenly spaced over the batch"
     return worst
 
 
@@ -437,9 +659,13 @@ def main():
     ap.add_argument("--smear-path", type=int, default=0, choices=range(5),
                     help="smearing path, all with the exact footprint: 0 separable VALU tiles, 1 FP32 MFMA "
                          "Gx.diag(w).Gy^T + exact-cut pass, 2 per-cell exponentials, 3 BF16x3 MFMA, 4 BF16x6 MFMA")
-    ap.add_argument("--parity-events", type=int, default=12,
+    ap.add_argument("--parity-events", type=int, default=100,
                     help="events of the last batch re-evaluated by the FP64 oracle for the eps_n error")
     ap.add_argument("--kln-events", type=int, default=20000, help="MC-KLN events per step (0: skip)")
+    ap.add_argument("--kln-direct-events", type=int, default=1000,
+                    help="MC-KLN events per step with the per-cell quadrature (0: skip)")
+    ap.add_argument("--config1-events", type=int, default=1000,
+                    help="Au+Au events with FP32 and FP64 grids as output (0: skip)")
     ap.add_argument("--adjoint-events", type=int, default=4096, help="differentiable batch size (0: skip)")
     ap.add_argument("--p3d-events", type=int, default=16, help="3D profile events per call (0: skip)")
     ap.add_argument("--no-graph", action="store_true", help="skip the eager vs hipGraph-replay comparison")
@@ -453,7 +679,8 @@ def main():
     ap.add_argument("--jet-events", type=int, default=20000, help="operation-2 grid events per batch (0: skip)")
     ap.add_argument("--default-events", type=int, default=100000,
                     help="events per batch with the reference parameter file's defaults (0: skip)")
-    ap.add_argument("--only", choices=("kln", "adjoint", "p3d", "mixed", "avg", "jet", "defaults"), default=None,
+    ap.add_argument("--only", choices=("kln", "kln-direct", "config1", "adjoint", "p3d", "mixed", "systems", "avg",
+                                       "jet", "defaults", "fma"), default=None,
                     help="time only the MC-KLN or the adjoint workload (its own JSON line), e.g. for profiling")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
@@ -470,7 +697,24 @@ def main():
     if args.only:
         if args.only == "kln":
             r = time_kln(args.kln_events, args.steps, args.seed)
+            r.pop("_table")
             metric, wl = "events/s MC-KLN Pb+Pb 2.76 TeV 261^2 grid", "MC-KLN Pb+Pb, dN/dy table, 261x261"
+        elif args.only == "kln-direct":
+            r = time_kln_direct(args.kln_direct_events, args.steps, args.seed)
+            metric, wl = "events/s MC-KLN Pb+Pb 2.76 TeV 261^2, per-cell quadrature", "MC-KLN Pb+Pb, kln_direct"
+        elif args.only == "config1":
+            r = time_config1(args.config1_events, args.steps, args.seed)
+            r["events_per_s"] = r["fp64_grids_events_per_s"]
+            metric, wl = "events/s Au+Au 261^2 with FP64 grids to host", "plain MC-Glauber Au+Au, 1000 events"
+        elif args.only == "fma":
+            from supermc_amd._native import calibrate_fma
+            tf, ms = calibrate_fma()
+            r = dict(events_per_s=None, fma_chain_tflops=tf, ms=ms, peak_nonpacked_tflops=FP32_PEAK_NONPACKED / 1e12)
+            metric, wl = "plain FP32 FMA TFLOP/s (independent v_fma_f32 chains)", "4096 x 256 lanes x 256000 FMAs"
+        elif args.only == "systems":
+            r = time_systems(20000, args.seed)
+            r["events_per_s"] = None
+            metric, wl = "k_event ms per 2x10^4 events, each mixed-workload system alone", "521^2, 42 mb"
         elif args.only == "adjoint":
             r = time_adjoint(args.adjoint_events, args.steps, args.seed)
             metric, wl = "events/s differentiable Pb+Pb 261^2 (forward + adjoint)", "frozen-topology Pb+Pb batch"
@@ -538,16 +782,25 @@ def main():
         # the VALU tiles (and the MFMA contraction's exact-cut pass) issue FP32
         # VALU; frac is against MI355X's FP32 figure, frac_nonpacked against
         # the plain-FMA rate the build can reach
-        roof = dict(bound="valu" if args.smear_path in (0, 2) else "mfma", achieved=achieved / 1e12,
-                    peak=FP32_PEAK / 1e12, unit="TFLOP/s", frac=achieved / FP32_PEAK,
-                    peak_nonpacked=FP32_PEAK_NONPACKED / 1e12, frac_nonpacked=achieved / FP32_PEAK_NONPACKED,
-                    work="2 flops x (Npart x %.1f + Ncoll x %.1f) footprint cells per event" % (fp["box"], fp["disk"]),
+        if args.smear_path in (0, 2):
+            roof = dict(bound="valu", achieved=achieved / 1e12, peak=FP32_PEAK / 1e12, unit="TFLOP/s",
+                        frac=achieved / FP32_PEAK, peak_nonpacked=FP32_PEAK_NONPACKED / 1e12,
+                        frac_nonpacked=achieved / FP32_PEAK_NONPACKED)
+        else:
+            pk = mfma_peak(args.smear_path)
+            roof = dict(bound="mfma", achieved=achieved / 1e12, peak=pk / 1e12, unit="TFLOP/s", frac=achieved / pk)
+        roof.update(work="2 flops x (Npart x %.1f + Ncoll x %.1f) footprint cells per event" % (fp["box"], fp["disk"]),
                     dense_equivalent_tflops=2.0 * N * sum_k / sec / 1e12)
     else:
         work = 4.0 * N * nev  # one FP32 grid per event
         achieved = work / (kernels[dom]["avg_ms"] * 1e-3)
         roof = dict(bound="hbm", achieved=achieved / 1e9, peak=HBM_PEAK / 1e9, unit="GB/s", frac=achieved / HBM_PEAK)
     roof["kernel"] = dom
+    try:
+        from supermc_amd._native import calibrate_fma
+        roof["fma_chain_tflops_measured"] = calibrate_fma()[0]
+    except Exception:  # reported as missing, not guessed
+        roof["fma_chain_tflops_measured"] = None
     # HBM bytes per launch from the committed rocprofv3 FETCH_SIZE /
     # WRITE_SIZE summary (calibrated against a plain copy), scaled to this
     # batch size; null for a smearing path it was not measured on
@@ -571,6 +824,11 @@ def main():
     host_rate = 2 * nev / (time.perf_counter() - th)
 
     kln = time_kln(args.kln_events, 2, args.seed) if args.kln_events > 0 else None
+    kln_table = kln.pop("_table") if kln else None
+    kdir = time_kln_direct(args.kln_direct_events, 2, args.seed) if args.kln_direct_events > 0 else None
+    if kln is not None:
+        kln["per_cell_quadrature"] = kdir
+    cfg1 = time_config1(args.config1_events, 3, args.seed) if args.config1_events > 0 else None
     adj = time_adjoint(args.adjoint_events, 3, args.seed) if args.adjoint_events > 0 else None
     p3d = time_p3d(args.p3d_events, 2, args.seed) if args.p3d_events > 0 else None
     graph = time_graph(args.seed) if not args.no_graph else None
@@ -587,6 +845,10 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_seconds)
+        if kln is not None:
+            kln["cpu_baseline"] = cpu_kln_baseline(args.cpu_seconds, kln_table[0], kln_table[1], args.seed)
+        if cfg1 is not None:
+            cfg1["cpu_baseline"] = cpu_baseline(min(args.cpu_seconds, 10.0), CPU_AUAU, "Au+Au")
         if p3d is not None:
             p3d["cpu_oracle_events_per_s_1core"] = cpu_p3d_baseline(args.seed)
         if avg is not None or jet is not None:
@@ -615,7 +877,8 @@ def main():
                    "smear_path": SMEAR_PATHS[args.smear_path],
                    "eps_err_vs_fp64": eps_err,
                    "kernels_avg_ms": {k: v["avg_ms"] for k, v in kernels.items()},
-                   "mc_kln_pbpb_2760": kln, "adjoint_fwd_bwd_pbpb": adj, "profile_3d_pbpb": p3d,
+                   "mc_kln_pbpb_2760": kln, "auau_1000_grids_fp32_fp64": cfg1, "adjoint_fwd_bwd_pbpb": adj,
+                   "profile_3d_pbpb": p3d,
                    "graph_replay": graph, "tile_autotune": tune, "mixed_systems_521": mixed,
                    "averaged_profiles_op3": avg, "jet_grids_op2": jet, "reference_defaults_301": dflt,
                    "records_to_host_events_per_s": host_rate},

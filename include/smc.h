@@ -216,6 +216,15 @@ smc_status smc_thickness(smc_ctx* ctx, float* TA, float* TB);
  * (sum of s, no finalFactor; NULL allowed). */
 smc_status smc_density(smc_ctx* ctx, float* s, double* dndy);
 
+/* MCnucl::setDensity(iy=0, ipt=-1) in the reference's own precision: the
+ * double grids of MCnucl.cpp:689-760 (which_mc_model 5) and :762-811 (7),
+ * each (source, cell) term evaluated in FP64 with the reference's
+ * expressions (Particle.cpp:122-132, 168-174) and summed in its order.
+ * Events [ev0, ev0 + n) of the current batch; s host [n][nx][ny].
+ * SMC_ERR_UNSUPPORTED for other models, SMC_ERR_ARG for a range outside
+ * the batch. */
+smc_status smc_density_f64(smc_ctx* ctx, int ev0, int n, double* s);
+
 /* MakeDensity::setSd + dumpEccentricities (MakeDensity.cpp:2244-2522):
  * per-event records, SMC_EV_STRIDE doubles each (NULL: device only). */
 smc_status smc_moments(smc_ctx* ctx, double* ev_out);
@@ -363,12 +372,35 @@ smc_status smc_get_event_system(smc_ctx* ctx, int ev, int* system);  /* -1: unmi
 smc_status smc_set_tile_variant(smc_ctx* ctx, int variant);
 smc_status smc_autotune(smc_ctx* ctx, int reps, int* best, double* ms);
 
+/* Sampling passes of the event kernel (no counterpart in the reference).
+ * An event whose projectile is a proton or deuteron and whose target is a
+ * Woods-Saxon nucleus redraws b until a collision happens
+ * (MakeDensity.cpp:2143-2162), about six times for p+Pb at b < 20 fm.
+ * With n = 2 (the default) its workgroup draws the targets of two
+ * consecutive attempts per pass, each from that attempt's own counters, so
+ * the events are bit-for-bit those of n = 1; n = 1 draws one attempt per
+ * pass.  Other systems are not affected. */
+smc_status smc_set_attempts_per_pass(smc_ctx* ctx, int n);
+smc_status smc_get_attempts_per_pass(const smc_ctx* ctx, int* n);
+/* Times the event kernel on the ctx's current sampled batch with n = 1 and
+ * n = 2 (reps launches each after a warm-up), keeps the faster and returns
+ * it, with the per-n times in ms[2] (either pointer may be NULL).  The
+ * batch's events are unchanged. */
+smc_status smc_autotune_attempts(smc_ctx* ctx, int reps, int* best, double* ms);
+
+/* Plain FP32 FMA rate of the current device for this build (no counterpart
+ * in the reference): independent v_fma_f32 chains over 4096 workgroups,
+ * the fastest of three launches; *tflops = 2 x FMAs / time, *ms its time
+ * (may be NULL).  The VALU kernels' rooflines are read against it. */
+smc_status smc_calibrate_fma(double* tflops, double* ms);
+
 /* Timing of the ctx's kernels with HIP events on its stream. */
 #define SMC_K_EVENT 0
 #define SMC_K_SMEAR 1
 #define SMC_K_MOMENTS 2
 #define SMC_K_KLN 3
 #define SMC_K_ADJOINT 4
+#define SMC_K_GRID64 5
 #define SMC_K_COUNT 8
 smc_status smc_set_timing(smc_ctx* ctx, int on);
 smc_status smc_get_timing(smc_ctx* ctx, int kernel, double* total_ms, int* launches);

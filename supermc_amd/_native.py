@@ -109,6 +109,11 @@ SIGNATURES = {
     "smc_run_noalloc": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, C.c_void_p]),
     "smc_thickness": (C.c_int, [C.c_void_p, _fp, _fp]),
     "smc_density": (C.c_int, [C.c_void_p, _fp, _dp]),
+    "smc_density_f64": (C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp]),
+    "smc_set_attempts_per_pass": (C.c_int, [C.c_void_p, C.c_int]),
+    "smc_get_attempts_per_pass": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "smc_autotune_attempts": (C.c_int, [C.c_void_p, C.c_int, _ip, _dp]),
+    "smc_calibrate_fma": (C.c_int, [_dp, _dp]),
     "smc_moments": (C.c_int, [C.c_void_p, _dp]),
     "smc_run": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
     "smc_run_graph": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int, _dp]),
@@ -181,6 +186,16 @@ def default_params(**kw):
             raise KeyError(k)
         setattr(p, k, v)
     return p
+
+
+def calibrate_fma():
+    """smc_calibrate_fma: (TFLOP/s, ms) of a chain of independent FP32 FMAs
+    on the current device -- the plain-FMA rate of this build."""
+    tf, ms = C.c_double(), C.c_double()
+    rc = load().smc_calibrate_fma(C.byref(tf), C.byref(ms))
+    if rc:
+        raise SmcError(rc, "smc_calibrate_fma failed")
+    return tf.value, ms.value
 
 
 def _ptr(a, ctype):
@@ -278,12 +293,24 @@ class Context:
         self._check(self.lib.smc_thickness(self.h, _ptr(ta, C.c_float), _ptr(tb, C.c_float)))
         return ta, tb
 
-    def density(self, want_grid=True):
-        s = np.zeros((self.nev, self.nx, self.ny), np.float32) if want_grid else None
+    def density(self, want_grid=True, out=None):
+        s = out if out is not None else (np.zeros((self.nev, self.nx, self.ny), np.float32) if want_grid else None)
+        if s is not None:
+            assert s.dtype == np.float32 and s.flags.c_contiguous and s.size >= self.nev * self.nx * self.ny
         dndy = np.zeros(self.nev)
         rc = self._check(self.lib.smc_density(self.h, _ptr(s, C.c_float), _ptr(dndy, C.c_double)),
                          allow=(SMC_ERR_DNDY_ZERO,))
         return s, dndy, rc
+
+    def density_f64(self, ev0=0, n=None, out=None):
+        """smc_density_f64: FP64 grids of events [ev0, ev0 + n) with the
+        reference's double arithmetic, [n][nx][ny]."""
+        if n is None:
+            n = self.nev - ev0
+        s = out if out is not None else np.zeros((n, self.nx, self.ny))
+        assert s.dtype == np.float64 and s.flags.c_contiguous and s.size >= n * self.nx * self.ny
+        self._check(self.lib.smc_density_f64(self.h, ev0, n, _ptr(s, C.c_double)))
+        return s
 
     def moments(self):
         ev = np.zeros((self.nev, EV_STRIDE))
@@ -319,6 +346,20 @@ class Context:
         s = C.c_int()
         self._check(self.lib.smc_get_event_system(self.h, ev, C.byref(s)))
         return s.value
+
+    def set_attempts_per_pass(self, n):
+        self._check(self.lib.smc_set_attempts_per_pass(self.h, n))
+
+    def attempts_per_pass(self):
+        n = C.c_int()
+        self._check(self.lib.smc_get_attempts_per_pass(self.h, C.byref(n)))
+        return n.value
+
+    def autotune_attempts(self, reps=3):
+        best = C.c_int()
+        ms = np.zeros(2)
+        self._check(self.lib.smc_autotune_attempts(self.h, reps, C.byref(best), _ptr(ms, C.c_double)))
+        return best.value, ms
 
     def set_tile_variant(self, variant):
         self._check(self.lib.smc_set_tile_variant(self.h, variant))

@@ -150,6 +150,7 @@ struct smc_ctx {
   NucSys* d_mix_sys = nullptr;
   int* d_mix_of = nullptr;
   unsigned long long batch_event0 = 0;
+  unsigned long long batch_seed = 0;  // seed of the current batch's sampled events
   int batch_mode = 0;
   int cap = 0;        // max events
   int nev = 0;        // events currently held
@@ -181,6 +182,8 @@ struct smc_ctx {
   double* avg_sum = nullptr;          // [2][avg_nord][nx][ny] sums (rotated, recentred)
   double* avg_part = nullptr;         // [kAccumSegments][nx][ny] partial sums over event segments
   double* avg_ext = nullptr;          // [10][avg_nord][nx][ny] sums of the optional averages
+  double* grid64 = nullptr;           // FP64 density grids [grid64_cap][nx][ny] (smc_density_f64)
+  int grid64_cap = 0;
   int avg_nord = 0;
   int* nmrect = nullptr;    // [cap]
   double* kln_table = nullptr;
@@ -400,6 +403,7 @@ void validate_and_derive(smc_ctx* c) {
   g.hit_n = d.hit_n_max;
   g.hit_lim = (int)std::floor(std::sqrt((double)d.hit_n_max)) + 1;
   g.crit_gauss = d.criterion == 2;
+  g.spec_attempts = 1;
   g.box_lim = (int)d.box_lim;
   g.gauss_k = (float)(1.4426950408889634 * kQ * kQ / (4.0 * w * w));
   g.gauss_A = (float)(d.sigma_gg / (4.0 * M_PI * w * w));
@@ -839,6 +843,7 @@ static void tag_systems(const smc_ctx* c, double* ev_out, int nev) {
 
 void run_event(smc_ctx* c, int nev, uint64_t seed, uint64_t event0, int fed) {
   c->batch_event0 = event0;
+  c->batch_seed = seed;
   c->batch_mode = fed;
   TimerScope ts(c, SMC_K_EVENT);
   check_launch(launch_event(c->cfg, c->buf, nev, seed, event0, fed, c->stream), "k_event");
@@ -1301,7 +1306,7 @@ void smc_destroy(smc_ctx* c) {
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
                   c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->avg_part, c->avg_ext, c->aux0,
-                  c->aux1, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
+                  c->aux1, c->grid64, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (void* q : c->dif_alloc)
@@ -1576,6 +1581,49 @@ smc_status smc_density(smc_ctx* c, float* s, double* dndy) {
   } catch (const SmcError& e) { return fail(c, e); }
 }
 
+smc_status smc_density_f64(smc_ctx* c, int ev0, int n, double* s) {
+  if (!c || !s || ev0 < 0 || n < 0) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events) throw SmcError(SMC_ERR_STATE, "no events: call smc_sample or smc_set_positions first");
+    if (n > c->nev - ev0) throw SmcError(SMC_ERR_ARG, "event range beyond the current batch");
+    if (c->p.which_mc_model != 5 && c->p.which_mc_model != 7)
+      throw SmcError(SMC_ERR_UNSUPPORTED, "FP64 grids are built for which_mc_model 5 and 7");
+    if (n == 0) return SMC_OK;
+    // the reference's constants, computed as MCnucl.cpp / Particle.cpp write them
+    const double w = c->d.width, wsq = w * w;
+    Grid64Const gc{};
+    gc.norm = 1 / (2 * M_PI * w * w);
+    gc.two_w_w = 2 * w * w;
+    gc.two_wsq = 2 * wsq;
+    gc.r_max = 5 * w;
+    gc.dc_max = 25. * wsq;
+    gc.alpha = c->p.alpha;
+    gc.prefactor = (1.0 - c->p.alpha) / 2.;
+    gc.model = c->p.which_mc_model;
+    gc.use_binary = c->p.which_mc_model == 5 && c->p.alpha > 1e-8;
+    gc.fluct = c->cfg.fluct;
+    const size_t N = (size_t)c->d.nx * c->d.ny;
+    const int chunk = std::min(n, kGrid64Chunk);
+    if (c->grid64_cap < chunk) {
+      if (c->grid64) HIPCHK(hipFree(c->grid64));
+      c->grid64 = nullptr;
+      c->grid64_cap = 0;
+      c->grid64 = dalloc<double>((size_t)chunk * N);
+      c->grid64_cap = chunk;
+    }
+    for (int e = 0; e < n; e += chunk) {
+      const int m = std::min(chunk, n - e);
+      TimerScope ts(c, SMC_K_GRID64);
+      check_launch(launch_grid64(c->cfg, c->buf, gc, ev0 + e, m, c->grid64, c->stream), "k_grid64");
+      ts.done();
+      HIPCHK(hipMemcpyAsync(s + (size_t)e * N, c->grid64, (size_t)m * N * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+    }
+    sync(c);
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
 smc_status smc_moments(smc_ctx* c, double* ev_out) {
   if (!c) return SMC_ERR_ARG;
   try {
@@ -1745,6 +1793,7 @@ smc_status smc_run_noalloc(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, 
     int* nrect = reinterpret_cast<int*>(base + kNoAllocBandBytes + kNoAllocRectBytes + kNoAllocCmBytes);
     check_launch(launch_event(c->cfg, c->buf, nev, seed, event0, 0, c->stream), "k_event");
     c->batch_event0 = event0;
+    c->batch_seed = seed;
     c->batch_mode = 0;
     c->nev = nev;
     c->have_events = 1;
@@ -1853,6 +1902,7 @@ smc_status smc_run_graph(smc_ctx* c, uint64_t seed, uint64_t event0, int nev, do
       HIPCHK(hipGraphExecKernelNodeSetParams(c->graph_exec, c->graph_event_node, &c->graph_event_params));
       HIPCHK(hipGraphLaunch(c->graph_exec, c->stream));
       c->batch_event0 = event0;
+      c->batch_seed = seed;
       c->batch_mode = 0;
       c->nev = nev;
       c->have_events = c->have_density = 1;
@@ -1880,6 +1930,98 @@ smc_status smc_set_tile_variant(smc_ctx* c, int variant) {
   c->have_density = c->have_moments = 0;  // the centre-of-mass partials follow the tile shape
   graph_release(c);
   c->graph_warm = 0;
+  return SMC_OK;
+}
+
+smc_status smc_calibrate_fma(double* tflops, double* ms) {
+  if (!tflops) return SMC_ERR_ARG;
+  const int blocks = 4096, iters = 2000;  // 16 workgroups per CU, about 7 ms at the FMA rate
+  float* out = nullptr;
+  hipStream_t st = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  int rc = SMC_OK;
+  if (hipMalloc(&out, (size_t)blocks * 256 * sizeof(float)) != hipSuccess || hipStreamCreate(&st) != hipSuccess ||
+      hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+    rc = SMC_ERR_HIP;
+  } else {
+    float best = 0.f;
+    for (int r = 0; r < 3 && rc == SMC_OK; ++r) {  // the fastest of three launches
+      if (hipEventRecord(a, st) != hipSuccess || launch_fma_chain(out, blocks, iters, st) != 0 ||
+          hipEventRecord(b, st) != hipSuccess || hipEventSynchronize(b) != hipSuccess) {
+        rc = SMC_ERR_HIP;
+        break;
+      }
+      float f = 0.f;
+      if (hipEventElapsedTime(&f, a, b) != hipSuccess) rc = SMC_ERR_HIP;
+      if (r == 0 || f < best) best = f;
+    }
+    if (rc == SMC_OK) {
+      *tflops = 2.0 * blocks * 256.0 * 128.0 * iters / (best * 1e-3) / 1e12;
+      if (ms) *ms = best;
+    }
+  }
+  if (a) (void)hipEventDestroy(a);
+  if (b) (void)hipEventDestroy(b);
+  if (st) (void)hipStreamDestroy(st);
+  if (out) (void)hipFree(out);
+  return rc;
+}
+
+smc_status smc_set_attempts_per_pass(smc_ctx* c, int n) {
+  if (!c) return SMC_ERR_ARG;
+  if (n != 1 && n != 2) {
+    c->err = "attempts per pass must be 1 or 2";
+    return SMC_ERR_ARG;
+  }
+  c->cfg.spec_attempts = n == 2;
+  graph_release(c);  // a captured event node holds the old configuration
+  c->graph_warm = 0;
+  return SMC_OK;
+}
+
+// Times the event kernel on the ctx's current batch (sampled events, its
+// seed and first event) with one and with two attempts per pass, reps
+// launches each after an untimed one, and keeps the faster.  Both give the
+// same events, so the relaunches leave the batch as it was.
+smc_status smc_autotune_attempts(smc_ctx* c, int reps, int* best, double* ms) {
+  if (!c) return SMC_ERR_ARG;
+  try {
+    if (!c->have_events || c->batch_mode != 0)
+      throw SmcError(SMC_ERR_STATE, "no sampled batch: call smc_sample or smc_run first");
+    if (reps < 1) throw SmcError(SMC_ERR_ARG, "reps must be positive");
+    hipEvent_t a, b;
+    HIPCHK(hipEventCreate(&a));
+    HIPCHK(hipEventCreate(&b));
+    double t[2];
+    for (int v = 0; v < 2; ++v) {
+      c->cfg.spec_attempts = v;
+      check_launch(launch_event(c->cfg, c->buf, c->nev, c->batch_seed, c->batch_event0, 0, c->stream), "k_event");
+      HIPCHK(hipEventRecord(a, c->stream));
+      for (int r = 0; r < reps; ++r)
+        check_launch(launch_event(c->cfg, c->buf, c->nev, c->batch_seed, c->batch_event0, 0, c->stream), "k_event");
+      HIPCHK(hipEventRecord(b, c->stream));
+      HIPCHK(hipEventSynchronize(b));
+      float f = 0.f;
+      HIPCHK(hipEventElapsedTime(&f, a, b));
+      t[v] = f / reps;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    c->cfg.spec_attempts = t[1] < t[0];
+    graph_release(c);
+    c->graph_warm = 0;
+    if (best) *best = c->cfg.spec_attempts ? 2 : 1;
+    if (ms) {
+      ms[0] = t[0];
+      ms[1] = t[1];
+    }
+    return SMC_OK;
+  } catch (const SmcError& e) { return fail(c, e); }
+}
+
+smc_status smc_get_attempts_per_pass(const smc_ctx* c, int* n) {
+  if (!c || !n) return SMC_ERR_ARG;
+  *n = c->cfg.spec_attempts ? 2 : 1;
   return SMC_OK;
 }
 

@@ -61,6 +61,7 @@ struct DevCfg {
   long long hit_n;       // collide iff dxq^2+dyq^2 <= hit_n (quanta^2)
   int hit_lim;           // |dxq| > hit_lim cannot collide
   int crit_gauss;        // collision criterion 2: a draw per pair of overlapping 8w boxes
+  int spec_attempts;     // 1: a proton / deuteron projectile's workgroup draws two attempts' targets per pass
   int box_lim;           // 8w in quanta: boxes overlap iff |dxq|, |dyq| <= box_lim
   float gauss_k;         // log2(e) q^2 / (4 w^2): e^{-b^2/4w^2} = 2^(-n gauss_k), n in quanta^2
   float gauss_A;         // sigma_gg / (4 pi w^2)
@@ -159,6 +160,27 @@ struct SmearSpec {
                     // cell there is zero; readers that need whole grids call
                     // launch_zero_outside first); 0: whole grids are written
 };
+
+// FP64 density grids (smc_grid64.hip): the reference's double expressions,
+// evaluated per (source, cell); constants computed on the host as the
+// reference writes them
+struct Grid64Const {
+  double norm;       // 1 / (2 pi w w)
+  double two_w_w;    // 2 w w   (Particle.cpp:122-132)
+  double two_wsq;    // 2 (w w) (MCnucl.cpp:732-756)
+  double r_max;      // 5 w
+  double dc_max;     // 25 (w w)
+  double alpha, prefactor;  // alpha, (1 - alpha) / 2
+  int model;         // which_mc_model: 5 or 7
+  int use_binary;    // alpha > 1e-8
+  int fluct;         // Gamma factors on
+};
+constexpr int kGrid64Chunk = 2048;  // events per k_grid64 launch (device buffer 8 B x cells each)
+int launch_grid64(const DevCfg& cfg, const EvBufs& b, const Grid64Const& gc, int ev0, int nev, double* out,
+                  void* stream);
+
+// plain FP32 FMA rate (smc_probe.hip): blocks x 256 lanes x 128 x iters FMAs
+int launch_fma_chain(float* out, int blocks, int iters, void* stream);
 
 // Launch wrappers (smc_kernels.hip)
 // k_event's host handle (matches the func of its captured graph node); its

@@ -48,6 +48,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   __shared__ int s_acc[2];
   __shared__ double s_b;
   __shared__ int s_xc;
+  __shared__ int s_xc2;    // b / 2 of the next attempt (two attempts per pass)
+  __shared__ double s_b2;
+  __shared__ int lpx[2], lpy[2];  // the light projectile when slot 0 holds a target
   __shared__ double s_red[2][2];
   __shared__ NucSys s_nuc[2];  // this event's nuclei
 
@@ -63,7 +66,44 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   int attempt = 0, status = kStOk;
   int npa = 0, npb = 0, ncoll = 0;
 
+  // Two attempts per sampling pass for a proton or deuteron on a
+  // Woods-Saxon target (sampled b, cfg.spec_attempts).  The two waves that
+  // serve nucleus 0 would sit idle while the target is drawn; instead they
+  // draw, into slot 0, the target of the next attempt from that attempt's
+  // own counters, and the projectile lives in lpx / lpy.  A rejected
+  // attempt hands slot 0 to the next one, which then needs no sampling
+  // pass.  Every event is the one the attempts taken one by one give.
+  const bool spec = mode == 0 && cfg.spec_attempts && s_nuc[0].kind != kWoodsSaxon &&
+                    s_nuc[1].kind == kWoodsSaxon;
+  const int NS[2] = {spec ? B : A, B};  // nucleons drawn into each slot
+  const int sys0 = spec ? 1 : 0;        // nucleus whose counters slot 0 draws
+  bool pending = false;                 // slot 0 holds this attempt's target
+
+  // proton / deuteron nucleus nu_ of attempt att centred at xcen
+  // (Nucleus.cpp:199-210, 342-359): one lane writes X, Y (and Z) and
+  // returns the nucleon count
+  const auto fill_light = [&](int nu_, unsigned att, int xcen, int* X, int* Y, float* Z) -> int {
+    if (s_nuc[nu_].kind == kProton) {
+      X[0] = xcen; Y[0] = 0; if (Z) Z[0] = 0.f;
+      return 1;
+    }
+    // deuteron: Hulthen separation along the nucleus' orientation
+    // (Nucleus.cpp:193-197)
+    const uint4 ro = draw(key, gev, kStrOrient, nu_, 0, att, 0);
+    const float c = 1.f - 2.f * u01(ro.x), ph = 2.f * kPi * u01(ro.y);
+    const uint4 r = draw(key, gev, kStrDeuteron, nu_, 0, att, 0);
+    const float d = hulthen_inv(u01(r.x));
+    float x = 0.5f * d, y = 0.f, z = 0.f;
+    rotate3(c, ph, x, y, z);
+    const int xq = quant(x), yq = quant(y);
+    X[0] = xq + xcen; Y[0] = yq;
+    X[1] = -xq + xcen; Y[1] = -yq;
+    if (Z) { Z[0] = z * 0x1p24f; Z[1] = -z * 0x1p24f; }
+    return 2;
+  };
+
   for (;; ++attempt) {
+    bool pair_drawn = false;  // this pass drew the next attempt's target too
     if (fed) {
       for (int i = t; i < A; i += 256) {
         px[0][i] = buf.raw[((size_t)e * 2 + 0) * kAMax * 2 + 2 * i];
@@ -74,6 +114,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         py[1][i] = buf.raw[((size_t)e * 2 + 1) * kAMax * 2 + 2 * i + 1];
       }
       __syncthreads();
+    } else if (pending) {
+      // this attempt's target was drawn into slot 0 by the previous pass
+      for (int i = t; i < B; i += 256) {
+        px[1][i] = px[0][i];
+        py[1][i] = py[0][i];
+      }
+      if (t == 0) {
+        s_xc = s_xc2;
+        s_b = s_b2;
+        fill_light(0, attempt, s_xc2, lpx, lpy, nullptr);
+      }
+      __syncthreads();
+      pending = false;
     } else {
       // b = sqrt((bmax^2-bmin^2) u + bmin^2)  (MakeDensity.cpp:2149); the
       // nuclei sit at +-b/2 rounded to the position quantum.
@@ -84,34 +137,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
         s_xc = xc;
         s_b = 2.0 * (double)xc * kQ;
         s_acc[0] = s_acc[1] = 0;
+        if (spec) {
+          const uint4 r2 = draw(key, gev, kStrB, 0, 0, attempt + 1, 0);
+          const int xc2 = (int)llrint(sqrt((cfg.bmax2 - cfg.bmin2) * u53(r2.x, r2.y) + cfg.bmin2) * 0.5 / kQ);
+          s_xc2 = xc2;
+          s_b2 = 2.0 * (double)xc2 * kQ;
+          fill_light(0, attempt, xc, lpx, lpy, nullptr);
+        }
       }
       __syncthreads();
-      // orientation of each nucleus (Nucleus.cpp:193-197)
+      // orientation of the nucleus drawn into each slot (Nucleus.cpp:193-197)
       float ctr[2], phr[2];
       for (int nu = 0; nu < 2; ++nu) {
-        const uint4 r = draw(key, gev, kStrOrient, nu, 0, attempt, 0);
+        const uint4 r = draw(key, gev, kStrOrient, nu ? 1 : sys0, 0, (spec && nu == 0) ? attempt + 1 : attempt, 0);
         ctr[nu] = 1.f - 2.f * u01(r.x);
         phr[nu] = 2.f * kPi * u01(r.y);
       }
-      // proton / deuteron fill directly (Nucleus.cpp:199-210, 342-359)
-      if (wv < 2) {
-        const NucSys ns = s_nuc[wv];
-        const int xcen = (wv == 0) ? s_xc : -s_xc;
-        if (ns.kind == kProton) {
-          if (lane == 0) { px[wv][0] = xcen; py[wv][0] = 0; pz[wv][0] = 0.f; s_acc[wv] = 1; }
-        } else if (ns.kind == kDeuteron) {
-          if (lane == 0) {
-            const uint4 r = draw(key, gev, kStrDeuteron, wv, 0, attempt, 0);
-            const float d = hulthen_inv(u01(r.x));
-            float x = 0.5f * d, y = 0.f, z = 0.f;
-            rotate3(ctr[wv], phr[wv], x, y, z);
-            const int xq = quant(x), yq = quant(y);
-            px[wv][0] = xq + xcen; py[wv][0] = yq; pz[wv][0] = z * 0x1p24f;
-            px[wv][1] = -xq + xcen; py[wv][1] = -yq; pz[wv][1] = -z * 0x1p24f;
-            s_acc[wv] = 2;
-          }
-        }
-      }
+      // proton / deuteron fill directly
+      if (wv < 2 && lane == 0 && s_nuc[wv ? 1 : sys0].kind != kWoodsSaxon)
+        s_acc[wv] = fill_light(wv, attempt, (wv == 0) ? s_xc : -s_xc, px[wv], py[wv], pz[wv]);
       __syncthreads();
       // Woods-Saxon nuclei: sequential acceptance against all earlier
       // nucleons (hard core, Nucleus.cpp:276-294), 64 candidates per round
@@ -124,19 +168,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
       // nucleons accepted so far (split over the two waves) and the
       // sequential acceptance among the round's candidates in list order.
       const int nu = wv & 1, sub = wv >> 1;
+      const unsigned att_nu = (spec && nu == 0) ? attempt + 1 : attempt;
+      bool spec_bad = false;
       for (unsigned round = 0;; ++round) {
-        const bool need0 = s_nuc[0].kind == kWoodsSaxon && s_acc[0] < A;
+        const bool need0 = s_nuc[sys0].kind == kWoodsSaxon && s_acc[0] < NS[0];
         const bool need1 = s_nuc[1].kind == kWoodsSaxon && s_acc[1] < B;
         if (!need0 && !need1) break;
-        if (round > 4096) { status |= kStNoEvent; break; }
+        if (round > 4096) {
+          // this attempt's nuclei unfinished: no event; only the next
+          // attempt's target unfinished: that attempt draws it again
+          if (!spec || need1) status |= kStNoEvent;
+          else spec_bad = true;
+          break;
+        }
         const bool need = nu ? need1 : need0;
-        const NucSys ns = s_nuc[nu];
+        const NucSys ns = s_nuc[nu ? 1 : sys0];
         int nc[2] = {0, 0};  // candidates so far (uniform over the workgroup)
         // candidates wanted this round: the nucleons still missing plus a
         // margin for the hard core, at most 64
         int want[2];
         for (int q = 0; q < 2; ++q) {
-          const int miss = NN[q] - s_acc[q];
+          const int miss = NS[q] - s_acc[q];
           want[q] = min(64, miss + (miss >> 3) + 4);
         }
         for (unsigned it = 0;; ++it) {
@@ -145,8 +197,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
           float z = 0.f;
           if (need && nc[nu] < want[nu]) {
             float x, y;
-            take = ws_proposal(ns, key, gev, nu, round * 128u + sub * 64u + lane, attempt, it, ctr[nu], phr[nu],
-                               x, y, z);
+            take = ws_proposal(ns, key, gev, nu ? 1 : sys0, round * 128u + sub * 64u + lane, att_nu, it, ctr[nu],
+                               phr[nu], x, y, z);
             xq = quant(x);
             yq = quant(y);
           }
@@ -246,7 +298,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
             if (!((((unsigned long long)hi32 << 32) | lo32) & accm)) accm |= 1ull << l;
           }
           // keep the first An - acc0 (later candidates never affect earlier ones)
-          const int An = NN[nu];
+          const int An = NS[nu];
           int nok = __popcll(accm);
           while (nok > An - acc0) {
             accm &= ~(1ull << (63 - __clzll(accm)));
@@ -264,30 +316,35 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
       }
       // CM recentering (Nucleus.cpp:296-309) in FP64, shift rounded to the
       // quantum, then the nucleus centre +-b/2.
-      if (wv < 2 && s_nuc[wv].kind == kWoodsSaxon) {
-        const int An = NN[wv];
+      if (wv < 2 && s_nuc[wv ? 1 : sys0].kind == kWoodsSaxon) {
+        const int An = NS[wv];
         double sxm = 0.0, sym = 0.0;
         for (int i = lane; i < An; i += 64) { sxm += (double)px[wv][i]; sym += (double)py[wv][i]; }
         sxm = wave_sum(sxm);
         sym = wave_sum(sym);
         const int cxq = (int)llrint(sxm / An), cyq = (int)llrint(sym / An);
-        const int xcen = (wv == 0) ? s_xc : -s_xc;
+        const int xcen = (wv == 0) ? (spec ? -s_xc2 : s_xc) : -s_xc;
         for (int i = lane; i < An; i += 64) { px[wv][i] = px[wv][i] - cxq + xcen; py[wv][i] -= cyq; }
       }
       __syncthreads();
+      pair_drawn = spec && !spec_bad;
     }
 
     // ---- sort each nucleus by x, ties by input index (Nucleus.cpp:314):
     // bitonic sort of the keys (x, index)
+    // (with two attempts per pass the projectile is in lpx / lpy)
+    const int* X0 = spec ? lpx : px[0];
+    const int* Y0 = spec ? lpy : py[0];
     for (int nu = 0; nu < 2; ++nu)
-      skey[nu][t] = (t < NN[nu]) ? (((unsigned long long)((unsigned)px[nu][t] ^ 0x80000000u) << 32) | (unsigned)t)
+      skey[nu][t] = (t < NN[nu]) ? (((unsigned long long)((unsigned)(nu ? px[1] : X0)[t] ^ 0x80000000u) << 32) |
+                                    (unsigned)t)
                                  : ~0ull;
     bitonic_sort_2x256(skey);
     for (int nu = 0; nu < 2; ++nu)
       if (t < NN[nu]) {
         const int i = (int)(skey[nu][t] & 0xFFFFFFFFull);
-        sx[nu][t] = px[nu][i];
-        sy[nu][t] = py[nu][i];
+        sx[nu][t] = (nu ? px[1] : X0)[i];
+        sy[nu][t] = (nu ? py[1] : Y0)[i];
       }
     if (t < B) fh[t] = 0x7FFFFFFF;
     __syncthreads();
@@ -375,6 +432,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     const int np = npa + npb;
     if (ncoll > 0 && np >= cfg.npmin && np <= cfg.npmax && !(status & kStNoEvent)) break;
     if (attempt + 1 >= cfg.max_attempts || (status & kStNoEvent)) { status |= kStNoEvent; break; }
+    pending = pair_drawn;
     __syncthreads();
   }
 
