@@ -1603,7 +1603,9 @@ smc_status smc_density_f64(smc_ctx* c, int ev0, int n, double* s) {
     gc.use_binary = c->p.which_mc_model == 5 && c->p.alpha > 1e-8;
     gc.fluct = c->cfg.fluct;
     const size_t N = (size_t)c->d.nx * c->d.ny;
-    const int chunk = std::min(n, kGrid64Chunk);
+    // events per launch: at most kGrid64Chunk and about 1 GiB of device grids
+    const int fit = (int)std::max<size_t>(1, ((size_t)1 << 30) / (N * sizeof(double)));
+    const int chunk = std::min(n, std::min(kGrid64Chunk, fit));
     if (c->grid64_cap < chunk) {
       if (c->grid64) HIPCHK(hipFree(c->grid64));
       c->grid64 = nullptr;

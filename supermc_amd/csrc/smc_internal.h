@@ -175,7 +175,7 @@ struct Grid64Const {
   int use_binary;    // alpha > 1e-8
   int fluct;         // Gamma factors on
 };
-constexpr int kGrid64Chunk = 2048;  // events per k_grid64 launch (device buffer 8 B x cells each)
+constexpr int kGrid64Chunk = 2048;  // most events per k_grid64 launch (device buffer 8 B x cells each)
 int launch_grid64(const DevCfg& cfg, const EvBufs& b, const Grid64Const& gc, int ev0, int nev, double* out,
                   void* stream);
 

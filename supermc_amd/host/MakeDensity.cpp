@@ -155,16 +155,21 @@ void MakeDensity::generate_profile_ebe(int nevent) {
   const bool use_sd = rdr_->getVal("use_sd") != 0, use_ed = rdr_->getVal("use_ed") != 0;
   const bool use_block = rdr_->getVal("use_block") != 0, use_4col = rdr_->getVal("use_4col") != 0;
   const size_t ncell = (size_t)nx_ * ny_;
+  // the reference's grids are doubles: for the two-component and
+  // sqrt(T_A T_B) models the files are written from the library's FP64
+  // grids (its double expressions), MC-KLN from the FP32 grid
+  const bool fp64 = p_.which_mc_model == 5 || p_.which_mc_model == 7;
   std::vector<float> grid;
+  std::vector<double> grid64(fp64 ? ncell : 0);
   std::vector<double> dndy;
   records_.clear();
   int event = 1;
   char buffer[512];
   while (event <= nevent) {
     const int n = next_batch(nevent - event + 1);
-    grid.resize((size_t)n * ncell);
+    grid.resize(fp64 ? 0 : (size_t)n * ncell);
     dndy.resize(n);
-    const smc_status st = smc_density(ctx_, grid.data(), dndy.data());
+    const smc_status st = smc_density(ctx_, fp64 ? nullptr : grid.data(), dndy.data());
     if (st != SMC_ERR_DNDY_ZERO) check(ctx_, st, "MakeDensity::generate_profile_ebe");
     for (int e = 0; e < n && event <= nevent; ++e) {
       const double* rec = &batch_rec_[(size_t)e * SMC_EV_STRIDE];
@@ -174,18 +179,22 @@ void MakeDensity::generate_profile_ebe(int nevent) {
       }
       // MakeDensity.cpp:597: the collision tables of every event
       smc_host::dump_binary_table(ctx_, p_, e, "data/binary.dat");
-      const float* s = &grid[(size_t)e * ncell];
+      if (fp64) check(ctx_, smc_density_f64(ctx_, e, 1, grid64.data()), "MakeDensity::generate_profile_ebe");
+      const float* s = fp64 ? nullptr : &grid[(size_t)e * ncell];
       const char* kinds[2] = {"sd", "ed"};
       const bool on[2] = {use_sd, use_ed};
       for (int k = 0; k < 2; ++k) {
         if (!on[k]) continue;
         if (use_4col) {
           std::snprintf(buffer, sizeof(buffer), "data/%s_event_%d_4col.dat", kinds[k], event);
-          dumpDensity4Col(buffer, s, p_.finalFactor, rec[SMC_EV_NPART]);
+          if (fp64) write_4col(buffer, grid64.data(), p_.finalFactor, rec[SMC_EV_NPART], nx_, ny_, xmin_, ymin_,
+                               p_.dx, p_.dy);
+          else dumpDensity4Col(buffer, s, p_.finalFactor, rec[SMC_EV_NPART]);
         }
         if (use_block) {
           std::snprintf(buffer, sizeof(buffer), "data/%s_event_%d_block.dat", kinds[k], event);
-          dumpDensityBlock(buffer, s, p_.finalFactor);
+          if (fp64) write_block(buffer, grid64.data(), p_.finalFactor, nx_, ny_);
+          else dumpDensityBlock(buffer, s, p_.finalFactor);
         }
       }
       records_.insert(records_.end(), rec, rec + SMC_EV_STRIDE);

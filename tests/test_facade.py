@@ -198,7 +198,10 @@ def test_driver_runs_reference_default_parameters(tmp_path):
 
 @pytest.mark.gpu
 def test_driver_profiles_match_c_abi(tmp_path):
-    """operation 1: data/sd_event_<i>_block.dat = finalFactor * s."""
+    """operation 1: data/sd_event_<i>_block.dat = finalFactor * s, written
+    (like the reference's double grids) from the FP64 grids: the printed
+    12 digits equal finalFactor times the oracle's double density of the
+    same event, and the FP32 grid agrees to its own bar."""
     _need(EXE)
     _write_params(tmp_path, operation=1, nev=3, finalFactor=2.5, use_4col=1)
     r = subprocess.run([EXE], cwd=tmp_path, capture_output=True, text=True, timeout=300)
@@ -209,13 +212,17 @@ def test_driver_profiles_match_c_abi(tmp_path):
     with Context(p) as ctx:
         ev = ctx.run(seed=1234, event0=0, nev=3)
         s, _, _ = ctx.density()
+        s64 = ctx.density_f64()
+        orc = [H.oracle_event(ctx, e)["s"] for e in range(3)]
     for e in range(3):
         blk = _read_table(tmp_path / "data" / f"sd_event_{e + 1}_block.dat")
         assert blk.shape == (261, 261)
-        np.testing.assert_allclose(blk, 2.5 * s[e].astype(np.float64), rtol=1e-11, atol=1e-300)
+        np.testing.assert_allclose(blk, 2.5 * s64[e], rtol=1e-11, atol=1e-300)
+        np.testing.assert_allclose(blk, 2.5 * orc[e], rtol=1e-11, atol=1e-300)
+        assert H.rel_linf(blk, 2.5 * s[e].astype(np.float64)) <= 1e-5
         col = _read_table(tmp_path / "data" / f"sd_event_{e + 1}_4col.dat")
         assert col.shape == (261 * 261, 4)
-        np.testing.assert_allclose(col[:, 3], 2.5 * s[e].ravel().astype(np.float64), rtol=1e-11)
+        np.testing.assert_allclose(col[:, 3], 2.5 * s64[e].ravel(), rtol=1e-11)
         assert col[1, 2] == pytest.approx(-12.9) and col[261, 1] == pytest.approx(-12.9)
         header = open(tmp_path / "data" / f"sd_event_{e + 1}_4col.dat").readline()
         assert header.split()[:2] == ["#", "<npart>="] and float(header.split()[2]) == ev[e, 54]

@@ -58,8 +58,9 @@ def test_fp64_grids_pbpb_fluct_fed_positions(model):
 
 
 def test_fp64_grid_ranges_and_chunks_are_event_local():
-    """Ranges across the 2048-event launch chunk give each event the grid
-    it has alone; out-of-batch ranges and other models are refused."""
+    """Ranges across a launch chunk (2048 events on small grids, about 1 GiB
+    of grids on large ones) give each event the grid it has alone;
+    out-of-batch ranges and other models are refused."""
     from supermc_amd import Context, SmcError
     from supermc_amd._native import SMC_ERR_ARG, SMC_ERR_UNSUPPORTED
     n = 2100
@@ -76,6 +77,16 @@ def test_fp64_grid_ranges_and_chunks_are_event_local():
             with pytest.raises(SmcError) as ei:
                 ctx.density_f64(ev0, m, out=np.zeros((max(m, 1), ctx.nx, ctx.ny)))
             assert ei.value.code == SMC_ERR_ARG
+    # 521^2 grids: about 1 GiB of FP64 grids per launch, 494 events
+    n = 500
+    p = H.glauber_params("AuAu", max_events=n, dx=0.05, dy=0.05)
+    with Context(p) as ctx:
+        ctx.run(seed=12, event0=0, nev=n, fetch=False)
+        full = ctx.density_f64()
+        for e in (0, 493, 494, n - 1):
+            assert np.array_equal(ctx.density_f64(e, 1)[0], full[e]), e
+        o = H.oracle_event(ctx, 494)
+        assert H.rel_linf(full[494], o["s"]) <= TOL64
     q = H.glauber_params("AuAu", max_events=2, which_mc_model=1, sub_model=7, ecm=2760.0, lambda_=0.138, tmax=71,
                          tmax_subdivision=3, kln_n_pt=8, kln_n_kt=8, kln_n_phi=8)
     xa, xb, bs = H.cpu_configs(q, 2, seed=3, bvals=np.array([0.0, 5.0]))
