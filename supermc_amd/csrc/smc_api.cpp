@@ -1273,6 +1273,16 @@ smc_status smc_params_default(smc_params* p) {
 size_t smc_params_sizeof(void) { return sizeof(smc_params); }
 size_t smc_derived_sizeof(void) { return sizeof(smc_derived); }
 
+// the event kernel instance: the second sampling slot only when it is on
+// and some system of the context has a proton or deuteron projectile on a
+// Woods-Saxon target
+static void update_spec(smc_ctx* c) {
+  bool light = c->cfg.nuc[0].kind != kWoodsSaxon && c->cfg.nuc[1].kind == kWoodsSaxon;
+  for (size_t i = 0; i + 1 < c->mix_sys.size(); i += 2)
+    light = light || (c->mix_sys[i].kind != kWoodsSaxon && c->mix_sys[i + 1].kind == kWoodsSaxon);
+  c->cfg.spec_live = c->cfg.spec_attempts && light;
+}
+
 smc_status smc_create(smc_ctx** out, const smc_params* p) {
   if (!out || !p) return SMC_ERR_ARG;
   *out = nullptr;
@@ -1281,6 +1291,7 @@ smc_status smc_create(smc_ctx** out, const smc_params* p) {
     c->p = *p;
     c->cap = p->max_events > 0 ? p->max_events : 1024;
     validate_and_derive(c);
+    update_spec(c);
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     build_tables(c);
     build_kln(c);
@@ -1976,6 +1987,7 @@ smc_status smc_set_attempts_per_pass(smc_ctx* c, int n) {
     return SMC_ERR_ARG;
   }
   c->cfg.spec_attempts = n == 2;
+  update_spec(c);
   graph_release(c);  // a captured event node holds the old configuration
   c->graph_warm = 0;
   return SMC_OK;
@@ -1997,6 +2009,7 @@ smc_status smc_autotune_attempts(smc_ctx* c, int reps, int* best, double* ms) {
     double t[2];
     for (int v = 0; v < 2; ++v) {
       c->cfg.spec_attempts = v;
+      update_spec(c);
       check_launch(launch_event(c->cfg, c->buf, c->nev, c->batch_seed, c->batch_event0, 0, c->stream), "k_event");
       HIPCHK(hipEventRecord(a, c->stream));
       for (int r = 0; r < reps; ++r)
@@ -2010,6 +2023,7 @@ smc_status smc_autotune_attempts(smc_ctx* c, int reps, int* best, double* ms) {
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     c->cfg.spec_attempts = t[1] < t[0];
+    update_spec(c);
     graph_release(c);
     c->graph_warm = 0;
     if (best) *best = c->cfg.spec_attempts ? 2 : 1;
@@ -2130,6 +2144,7 @@ smc_status smc_set_systems(smc_ctx* c, int nsys, const smc_params* systems, int 
     }
     c->mix_sys = sys;
     c->mix_of = of;
+    update_spec(c);
     c->have_events = c->have_density = c->have_thickness = c->have_moments = 0;
     return SMC_OK;
   } catch (const SmcError& e) { return fail(c, e); }

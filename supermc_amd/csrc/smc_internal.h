@@ -62,6 +62,8 @@ struct DevCfg {
   int hit_lim;           // |dxq| > hit_lim cannot collide
   int crit_gauss;        // collision criterion 2: a draw per pair of overlapping 8w boxes
   int spec_attempts;     // 1: a proton / deuteron projectile's workgroup draws two attempts' targets per pass
+  int spec_live;         // spec_attempts and some system of the context has such a projectile: the
+                         // event kernel instance that carries the second slot
   int box_lim;           // 8w in quanta: boxes overlap iff |dxq|, |dyq| <= box_lim
   float gauss_k;         // log2(e) q^2 / (4 w^2): e^{-b^2/4w^2} = 2^(-n gauss_k), n in quanta^2
   float gauss_A;         // sigma_gg / (4 pi w^2)

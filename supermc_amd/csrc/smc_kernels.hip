@@ -20,8 +20,10 @@ namespace smc {
 
 // ------------------------------------------------------------- k_event
 // GAUSS: collision criterion 2 (the disk criterion's instance carries none
-// of its code)
-template <bool GAUSS>
+// of its code).  SPEC: the instance with the second sampling slot for
+// proton / deuteron projectiles (launched only when a system of the batch
+// has one; the other instance keeps the registers that slot costs).
+template <bool GAUSS, bool SPEC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) void k_event(DevCfg cfg, EvBufs buf, uint2 key,
                                                unsigned long long event0, int mode) {
   // mode 0: impact parameter sampled, events accepted as MakeDensity.cpp:2147-2162
@@ -67,14 +69,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
   int npa = 0, npb = 0, ncoll = 0;
 
   // Two attempts per sampling pass for a proton or deuteron on a
-  // Woods-Saxon target (sampled b, cfg.spec_attempts).  The two waves that
+  // Woods-Saxon target (sampled b, the SPEC instance).  The two waves that
   // serve nucleus 0 would sit idle while the target is drawn; instead they
   // draw, into slot 0, the target of the next attempt from that attempt's
   // own counters, and the projectile lives in lpx / lpy.  A rejected
   // attempt hands slot 0 to the next one, which then needs no sampling
   // pass.  Every event is the one the attempts taken one by one give.
-  const bool spec = mode == 0 && cfg.spec_attempts && s_nuc[0].kind != kWoodsSaxon &&
-                    s_nuc[1].kind == kWoodsSaxon;
+  const bool spec = SPEC && mode == 0 && s_nuc[0].kind != kWoodsSaxon && s_nuc[1].kind == kWoodsSaxon;
   const int NS[2] = {spec ? B : A, B};  // nucleons drawn into each slot
   const int sys0 = spec ? 1 : 0;        // nucleus whose counters slot 0 draws
   bool pending = false;                 // slot 0 holds this attempt's target
@@ -688,15 +689,26 @@ __global__ __launch_bounds__(256) void k_smear(DevCfg cfg, EvBufs buf, SmearSpec
 static hipStream_t S(void* s) { return (hipStream_t)s; }
 
 const void* event_kernel_symbol(const DevCfg& cfg) {
-  return cfg.crit_gauss ? reinterpret_cast<const void*>(&k_event<true>) : reinterpret_cast<const void*>(&k_event<false>);
+  if (cfg.spec_live)
+    return cfg.crit_gauss ? reinterpret_cast<const void*>(&k_event<true, true>)
+                          : reinterpret_cast<const void*>(&k_event<false, true>);
+  return cfg.crit_gauss ? reinterpret_cast<const void*>(&k_event<true, false>)
+                        : reinterpret_cast<const void*>(&k_event<false, false>);
 }
 
 int launch_event(const DevCfg& cfg, const EvBufs& b, int nev, unsigned long long seed,
                  unsigned long long event0, int mode, void* stream) {
   if (nev <= 0) return 0;
   const uint2 key = make_uint2((unsigned)seed, (unsigned)(seed >> 32));
-  if (cfg.crit_gauss) hipLaunchKernelGGL(k_event<true>, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
-  else hipLaunchKernelGGL(k_event<false>, dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
+  const void* fn = event_kernel_symbol(cfg);
+  if (fn == reinterpret_cast<const void*>(&k_event<true, true>))
+    hipLaunchKernelGGL((k_event<true, true>), dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
+  else if (fn == reinterpret_cast<const void*>(&k_event<false, true>))
+    hipLaunchKernelGGL((k_event<false, true>), dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
+  else if (fn == reinterpret_cast<const void*>(&k_event<true, false>))
+    hipLaunchKernelGGL((k_event<true, false>), dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
+  else
+    hipLaunchKernelGGL((k_event<false, false>), dim3(nev), dim3(256), 0, S(stream), cfg, b, key, event0, mode);
   return (int)hipGetLastError();
 }
 
