@@ -11,7 +11,7 @@
 //  k_smear    2D Gaussian smearing of a source list onto the grid, one
 //             workgroup per (event, 16x64 tile), exact reference footprints
 //             via integer threshold tables.  (MCnucl.cpp:398-478, 626-866)
-//  (moments: smc_grid.hip)
+//  (moments: smc_moments.hip; the default smearing: smc_tile.hip)
 #include <hip/hip_runtime.h>
 
 #include "smc_device.h"
