@@ -336,6 +336,39 @@ def test_driver_average_profiles_with_optional_averages(tmp_path):
 
 
 @pytest.mark.gpu
+def test_driver_operation_3_with_reference_parameter_file(tmp_path):
+    """superMC.e with the reference's own parameters.dat values
+    (tests/golden/reference_parameters.json: sqrt(T_A T_B), Gaussian
+    nucleons and collisions at 5020 GeV, 301^2 cells, use_sd and use_ed,
+    every optional average on, reaction-plane profiles on), run as
+    `superMC.e operation=3 nev=12 randomSeed=5` -- command-line overrides,
+    as the reference takes them: it ends normally and writes each averaged
+    profile of orders 2 and 3, Sd and Ed, with their reaction-plane forms,
+    as 301 x 301 blocks."""
+    import json
+    _need(EXE)
+    vals = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "reference_parameters.json")))["values"]
+    open(tmp_path / "parameters.dat", "w").write("".join(f"{k} = {v}\n" for k, v in vals.items()))
+    os.makedirs(tmp_path / "data", exist_ok=True)
+    r = subprocess.run([EXE, "operation=3", "nev=12", "randomSeed=5"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    stems = ["sdAvg", "sdAvg_RP", "edAvg", "edAvg_RP"]
+    for k in ("Sd", "Ed"):
+        for base in ("TATB_from", "rho_binary_from", "nuclear_thickness_TA_from", "nuclear_thickness_TB_from"):
+            stems += [f"{base}{k}", f"{base}{k}_RP"]
+        stems += [f"spectator_density_A_from{k}", f"spectator_density_B_from{k}"]
+    for n in (2, 3):
+        for stem in stems:
+            blk = _read_table(tmp_path / "data" / f"{stem}_order_{n}_block.dat")
+            assert blk.shape == (301, 301), (stem, n)
+            assert np.isfinite(blk).all() and (blk >= 0).all() and blk.max() > 0, (stem, n)
+    # the averaged entropy profile integrates to finalFactor x <dS/dy> / (dx dy) x dx dy
+    sd = _read_table(tmp_path / "data" / "sdAvg_order_2_block.dat")
+    assert sd.sum() * 0.01 > 1.0
+
+
+@pytest.mark.gpu
 def test_driver_jet_operation(tmp_path):
     """operation 2 (MakeDensity.cpp:148-500): per-event tables, eccentricity
     lines in sn_ecc_eccp_<n>_event_<i>.dat, and the rho_binary, T_A / T_B,
