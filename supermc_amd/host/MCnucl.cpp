@@ -2,6 +2,8 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <fstream>
+#include <iomanip>
 #include <iostream>
 
 #include "smc_host.h"
@@ -78,6 +80,34 @@ void MCnucl::setDensity(int iy, int ipt) {
 
 // MCnucl.cpp:911-960 (built on the device, kept in the context)
 void MCnucl::makeTable() { check(ctx_, smc_kln_make_table(ctx_, nullptr), "MCnucl::makeTable"); }
+
+namespace {
+[[noreturn]] void off_path(const char* what) {
+  std::cout << what << ": pT-resolved densities and tables are not built in this library." << std::endl;
+  std::exit(1);
+}
+}  // namespace
+
+double MCnucl::getRho(int i, int x, int y, int pt) {
+  if (pt >= 0) off_path("MCnucl::getRho(i, x, y, pt)");
+  return getRho(i, x, y);
+}
+
+void MCnucl::makeTable(double, double, int) { off_path("MCnucl::makeTable(double, double, int)"); }
+
+void MCnucl::dumpdNdydptTable5Col(std::string, double****, const int) { off_path("MCnucl::dumpdNdydptTable5Col"); }
+
+void MCnucl::dumpdNdyTable4Col(std::string filename, double*** dNdyTable, const int iy) {
+  smc_derived d{};
+  check(ctx_, smc_get_derived(ctx_, &d), "MCnucl::dumpdNdyTable4Col");
+  std::ofstream of(filename.c_str(), std::ios_base::app);
+  const double y = 0.0;  // the one rapidity bin at y = 0 (ny = 1, ymax = 0)
+  for (int i = 1; i < d.kln_tmax; ++i)
+    for (int j = 1; j < d.kln_tmax; ++j)
+      of << std::fixed << std::setprecision(3) << std::setw(10) << y << std::fixed << std::setprecision(3)
+         << std::setw(10) << d.kln_dT * i << std::fixed << std::setprecision(3) << std::setw(10) << d.kln_dT * j
+         << std::setprecision(12) << std::setw(22) << dNdyTable[iy][i][j] << std::endl;
+}
 
 // MCnucl.cpp:1177-1211 / 1213-1220
 void MCnucl::dumpBinaryTable(std::string filename) { smc_host::dump_binary_table(ctx_, p_, 0, filename.c_str()); }

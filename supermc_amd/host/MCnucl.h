@@ -35,6 +35,9 @@ class MCnucl {
   double getTA1(int x, int y) { return ta_[(size_t)x * ny_ + y]; }
   double getTA2(int x, int y) { return tb_[(size_t)x * ny_ + y]; }
   double getRho(int i, int x, int y) { (void)i; return rho_[(size_t)x * ny_ + y]; }
+  // pt < 0: the pT-integrated density; pT-resolved densities are not built
+  // (the program stops with a message, as for the other options off this path)
+  double getRho(int i, int x, int y, int pt);
   void setRho(int i, int x, int y, double val) { (void)i; rho_[(size_t)x * ny_ + y] = val; }
   int getNcoll() { return ncoll_; }
   int getNpart1() { return npart1_; }
@@ -49,6 +52,14 @@ class MCnucl {
   int CentralityCut();
   void setCentralityCut(int Nmin, int Nmax) { npmax_ = Nmax; npmin_ = Nmin; }
   void makeTable();
+  // MCnucl.cpp:967-1025, the dN/dy d2pT table: not built (stops with a message)
+  void makeTable(double ptmin, double dpt, int iPtmax);
+  // MCnucl.cpp:1027-1048: rows (y, T_A, T_B, dNdyTable[iy][i][j]) for
+  // i, j >= 1 on this context's table grid (tmax points of dT), appended
+  // to filename; dNdyTable is the caller's [iy][tmax][tmax] array
+  void dumpdNdyTable4Col(std::string filename, double*** dNdyTable, const int iy);
+  // MCnucl.cpp:1050-1076, the pT-resolved table: not built (stops with a message)
+  void dumpdNdydptTable5Col(std::string filename, double**** dNdydptTable, const int iy);
   // MCnucl.cpp:1303-1323: the participant bounding boxes (projectile, then
   // target, in wound order) and the collision pairs' boxes (a CollisionPair
   // never sets its box: all edges 0), and their union

@@ -3,6 +3,7 @@
 //   facade_check mcnucl <params-file> b0 b1 ...         one event per b through MCnucl
 //   facade_check hotspots <params-file> b0 b1 ...       MCnucl::getHotSpots, then after rotateGrid
 //   facade_check kln <ecm> <lambda> ta0 tb0 ta1 tb1 ... KLNModel::getdNdy per point
+//   facade_check dndytable <params-file> <out>          MCnucl::makeTable, dumpdNdyTable4Col
 //   facade_check p3d <parts> <bins> nx ny neta dx deta ecm random_flag alpha <out>
 //                                                       profile_3d: both grids, entropy file
 #include <cstdio>
@@ -102,6 +103,22 @@ int main(int argc, char* argv[]) {
         std::printf("U%s %.17g %.17g %.17g %.17g\n", tag, u.getXL(), u.getXR(), u.getYL(), u.getYR());
       }
     }
+    return 0;
+  }
+  if (!std::strcmp(mode, "dndytable") && argc == 4) {  // MCnucl::makeTable + dumpdNdyTable4Col
+    ParameterReader r;
+    r.readFromFile(argv[2]);
+    MCnucl mc(&r);
+    mc.makeTable();
+    smc_derived d{};
+    if (smc_get_derived(mc.context(), &d) != SMC_OK) return 3;
+    const int n = d.kln_tmax;
+    std::vector<double> flat((size_t)n * n);
+    if (smc_kln_make_table(mc.context(), flat.data()) != SMC_OK) return 3;
+    std::vector<double*> rows(n);
+    for (int i = 0; i < n; ++i) rows[i] = &flat[(size_t)i * n];
+    double** one = rows.data();
+    mc.dumpdNdyTable4Col(argv[3], &one, 0);
     return 0;
   }
   if (!std::strcmp(mode, "kln")) {

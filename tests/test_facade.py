@@ -506,6 +506,34 @@ def test_klnmodel_facade_matches_oracle():
         assert g == pytest.approx(ref, rel=1e-11, abs=0.0)
 
 
+@pytest.mark.gpu
+def test_mcnucl_dndy_table_4col(tmp_path):
+    """MCnucl::makeTable and dumpdNdyTable4Col (MCnucl.cpp:1027-1048): one
+    row per (T_A, T_B) grid point off the zero row and column, T = dT i in
+    fixed 3-digit columns, the table value with 12 decimals -- the values of
+    smc_kln_make_table."""
+    _need(CHECK)
+    params = _write_params(tmp_path, which_mc_model=1, sub_model=7, kln_n_pt=8, kln_n_kt=8, kln_n_phi=8)
+    out = tmp_path / "dndy_table.dat"
+    r = subprocess.run([CHECK, "dndytable", params, str(out)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    from supermc_amd import Context
+    import parity_helpers as H
+    p = H.glauber_params("PbPb", ecm=2760.0, which_mc_model=1, sub_model=7, lambda_=0.138, tmax=71,
+                         tmax_subdivision=3, kln_n_pt=8, kln_n_kt=8, kln_n_phi=8, max_events=2)
+    with Context(p) as ctx:
+        tab = ctx.kln_make_table()
+        dT = ctx.derived.kln_dT
+    n = tab.shape[0]
+    rows = _read_table(out)
+    assert rows.shape == ((n - 1) ** 2, 4)
+    i, j = np.meshgrid(np.arange(1, n), np.arange(1, n), indexing="ij")
+    np.testing.assert_allclose(rows[:, 1], np.round(dT * i.ravel(), 3), atol=1e-9)
+    np.testing.assert_allclose(rows[:, 2], np.round(dT * j.ravel(), 3), atol=1e-9)
+    np.testing.assert_allclose(rows[:, 3], tab[1:, 1:].ravel(), rtol=1e-11, atol=1e-12)
+    assert (rows[:, 0] == 0).all()
+
+
 # ---- scripts/generate_3d_profiles: profile_3d and generate_profile_3d.e ----
 EXE3D = os.path.join(ROOT, "supermc_amd", "bin", "generate_profile_3d.e")
 
