@@ -477,14 +477,20 @@ __global__ __launch_bounds__(256) void k_diff_gather(DevCfg cfg, EvBufs buf, Sme
     const int xw = (spec.win[kind] == kWinBox) ? wv.x : wv.z;
     const int yw = (spec.win[kind] == kWinBox) ? wv.y : wv.w;
     const int xlo = xw & 0xffff, xhi = xw >> 16, ylo = yw & 0xffff, yhi = yw >> 16;
+    // The Gaussian factorises, G(dx) G(dy), and dx depends on the row only,
+    // dy on the column only: each lane (a column) sums over the rows
+    //   P = sum_r Ex_r g_rc,  Q = sum_r Ex_r dx_r g_rc,  R = sum_r Ex_r dx_r^2 g_rc
+    // and folds them with its own Ey and dy once per column block:
+    //   e0 = Ey P,  e1 = Ey Q,  e2 = Ey dy P,  e3 = Ey (R + dy^2 P).
     float e0 = 0.f, e1 = 0.f, e2 = 0.f, e3 = 0.f;
     for (int r0 = xlo; r0 < xhi; r0 += 64) {
       const int nr = min(64, xhi - r0);
-      float dxl, exl;  // row r0 + lane: offset from the source and its Gaussian
+      float dxl, exl, liml;  // row r0 + lane: offset from the source, its Gaussian, cut2 - dx^2
       {
         const int rowh = (int)llrint((cfg.xmin + (r0 + lane) * cfg.dx) / kH);
         dxl = (float)(rowh - s.x) * kHf;
         exl = __builtin_amdgcn_exp2f(-(dxl * dxl) * c2);
+        liml = cut2 - dxl * dxl;
       }
       for (int c0 = ylo; c0 < yhi; c0 += 64) {
         const int col = c0 + lane;
@@ -492,36 +498,32 @@ __global__ __launch_bounds__(256) void k_diff_gather(DevCfg cfg, EvBufs buf, Sme
         const int ygh = (int)llrint((cfg.ymin + col * cfg.dy) / kH);
         const float dy = (float)(ygh - s.y) * kHf;
         const float dy2 = dy * dy;
-        const float Ey = __builtin_amdgcn_exp2f(-dy2 * c2);
         const float* gr = g + (size_t)r0 * ny + (on ? col : ylo);
-        auto cell = [&](int j, float& gv, float& dx, float& dx2) {
-          dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dxl), j));
+        float P = 0.f, Q = 0.f, R = 0.f;
+        auto row = [&](int j, float gv) {
+          const float dx = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(dxl), j));
           const float Ex = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(exl), j));
-          dx2 = dx * dx;
-          const bool in = on && dy2 <= cut2 - dx2;
-          gv = in ? gr[(size_t)j * ny] * (Ex * Ey) : 0.f;
+          const float lim = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(liml), j));
+          const float t = (on && dy2 <= lim) ? gv * Ex : 0.f;
+          P += t;
+          const float tx = t * dx;
+          Q += tx;
+          R = fmaf(tx, dx, R);
         };
         int j = 0;
         for (; j + 4 <= nr; j += 4) {  // four rows' loads in flight, added in row order
-          float gv[4], dx[4], dx2[4];
+          float gv[4];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) cell(j + q, gv[q], dx[q], dx2[q]);
+          for (int q = 0; q < 4; ++q) gv[q] = gr[(size_t)(j + q) * ny];
 #pragma unroll
-          for (int q = 0; q < 4; ++q) {
-            e0 += gv[q];
-            e1 = fmaf(gv[q], dx[q], e1);  // dx, dy: cell minus source
-            e2 = fmaf(gv[q], dy, e2);
-            e3 = fmaf(gv[q], dx2[q] + dy2, e3);
-          }
+          for (int q = 0; q < 4; ++q) row(j + q, gv[q]);
         }
-        for (; j < nr; ++j) {
-          float gv, dx, dx2;
-          cell(j, gv, dx, dx2);
-          e0 += gv;
-          e1 = fmaf(gv, dx, e1);
-          e2 = fmaf(gv, dy, e2);
-          e3 = fmaf(gv, dx2 + dy2, e3);
-        }
+        for (; j < nr; ++j) row(j, gr[(size_t)j * ny]);
+        const float Ey = __builtin_amdgcn_exp2f(-dy2 * c2);
+        e0 = fmaf(Ey, P, e0);
+        e1 = fmaf(Ey, Q, e1);
+        e2 = fmaf(Ey * dy, P, e2);
+        e3 = fmaf(Ey, fmaf(dy2, P, R), e3);
       }
     }
     const double s0 = wave_sum((double)e0), s1 = wave_sum((double)e1), s2 = wave_sum((double)e2),
