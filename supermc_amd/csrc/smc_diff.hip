@@ -563,33 +563,49 @@ __global__ __launch_bounds__(256) void k_diff_chain(DevCfg cfg, EvBufs buf, Diff
       gk += o.x * 0.5 * (1.0 - alpha) * fs * 0.5 * (1.0 - alpha + 1e-8);
     }
   }
-  // positions: participants carry their source, pairs half to each nucleon
-  double gR = 0.0, gA = 0.0;
+  // positions: participants carry their source, pairs half to each nucleon;
+  // the pair list goes through LDS 256 entries at a time and every thread
+  // (nucleon t of each nucleus) adds its pairs in pair order
+  __shared__ unsigned lp[256];
   const unsigned* pairs = buf.pairs + (size_t)e * cfg.max_pairs;
+  double gx[2] = {0.0, 0.0}, gy[2] = {0.0, 0.0};
+  int wnd[2];
   for (int nu = 0; nu < 2; ++nu) {
-    if (t < cfg.nuc[nu].A) {
-      double gx = 0.0, gy = 0.0;
-      const int wnd = buf.wound[(size_t)e * 2 * kAMax + nu * kAMax + t];
-      if (wnd >= 0) {
-        const double4 o = gs[nu ? npa + wnd : wnd];
-        gx += o.y;
-        gy += o.z;
-      }
-      if (wnd >= 0) {
-        const int np = min(ncoll, cfg.max_pairs);
-        for (int q = 0; q < np; ++q) {
-          const unsigned pr = pairs[q];
-          const int i = nu ? (int)(pr >> 16) : (int)(pr & 0xFFFFu);
-          if (i == t) {
-            const double4 o = gs[npa + npb + q];
-            gx += 0.5 * o.y;
-            gy += 0.5 * o.z;
-          }
+    wnd[nu] = (t < cfg.nuc[nu].A) ? buf.wound[(size_t)e * 2 * kAMax + nu * kAMax + t] : -1;
+    if (wnd[nu] >= 0) {
+      const double4 o = gs[nu ? npa + wnd[nu] : wnd[nu]];
+      gx[nu] += o.y;
+      gy[nu] += o.z;
+    }
+  }
+  const int np = min(ncoll, cfg.max_pairs);
+  for (int q0 = 0; q0 < np; q0 += 256) {
+    __syncthreads();
+    if (q0 + t < np) lp[t] = pairs[q0 + t];
+    __syncthreads();
+    const int m = min(256, np - q0);
+    for (int q = 0; q < m; ++q) {
+      const unsigned pr = lp[q];
+      const bool a = wnd[0] >= 0 && (int)(pr & 0xFFFFu) == t, b = wnd[1] >= 0 && (int)(pr >> 16) == t;
+      if (a || b) {
+        const double4 o = gs[npa + npb + q0 + q];
+        if (a) {
+          gx[0] += 0.5 * o.y;
+          gy[0] += 0.5 * o.z;
+        }
+        if (b) {
+          gx[1] += 0.5 * o.y;
+          gy[1] += 0.5 * o.z;
         }
       }
+    }
+  }
+  double gR = 0.0, gA = 0.0;
+  for (int nu = 0; nu < 2; ++nu) {
+    if (t < cfg.nuc[nu].A) {
       const double* p = d.dpos + (((size_t)e * 2 + nu) * kAMax + t) * 6;
-      gR += gx * p[2] + gy * p[3];
-      gA += gx * p[4] + gy * p[5];
+      gR += gx[nu] * p[2] + gy[nu] * p[3];
+      gA += gx[nu] * p[4] + gy[nu] * p[5];
     }
   }
   const double v[5] = {gw, gR, gA, ga, gk};
