@@ -1084,6 +1084,7 @@ void diff_setup(smc_ctx* c, uint64_t seed, uint64_t event0, int nev) {
   d.gsrc = dif_alloc<double4>(c, 10, E * SC);
   d.gev = dif_alloc<double>(c, 11, E * 5);
   d.wstab = dif_alloc<double>(c, 13, kWsTabLen);
+  d.cotc = dif_alloc<double>(c, 14, E * kCotStride);
   std::vector<double> x, w;
   gauss_legendre01(64, x, w);
   std::vector<double> glh(128);

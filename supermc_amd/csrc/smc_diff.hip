@@ -16,7 +16,8 @@
 //             dG^{-1}/ds = -(dP/ds)/pdf.
 //  density    the exact-footprint tile kernel with w in the Gaussian and the
 //             frozen windows.
-//  adjoint    cell cotangents G = dL/drho (k_diff_cot), one wavefront per
+//  adjoint    cell cotangents G = dL/drho inside each event's source
+//             rectangle (k_diff_cot_coef, k_diff_cot), one wavefront per
 //             source gathering G against the recomputed Gaussians
 //             (k_diff_gather), per-event chain rule to theta (k_diff_chain).
 #include "smc_device.h"
@@ -389,67 +390,83 @@ __global__ __launch_bounds__(256) void k_diff_src(DevCfg cfg, EvBufs buf, DiffBu
 // ------------------------------------------------------------- backward
 // Cell cotangents G = dL/drho for L = sum_e lam2 |eps2| + lam3 |eps3| +
 // mu dS/dy, from the per-event scalars of k_diff_moments (layout kDs*).
-__global__ __launch_bounds__(256) void k_diff_cot(DevCfg cfg, DiffBufs d, const double* cot, double ffdxdy,
-                                                  float* G) {
-  // per event and n = 2, 3: lam cx / S, lam cy / S (the centre's dependence),
-  // lam Ar / (Am N), lam Ai / (Am N), lam Am / N^2 (the masked g_n terms)
-  __shared__ double cf[2][5];
-  __shared__ double c0;
+// Per event: the coefficients of the cell cotangents -- lam cx / S,
+// lam cy / S (the centre's dependence), lam Ar / (Am N), lam Ai / (Am N),
+// lam Am / N^2 (the masked g_n terms) for n = 2, 3, and c0 (dS/dy).
+__global__ __launch_bounds__(64) void k_diff_cot_coef(DiffBufs d, const double* cot, double ffdxdy, int nev) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nev) return;
+  const double* sc = d.dsc + (size_t)e * kDsStride;
+  double* out = d.cotc + (size_t)e * kCotStride;
+  const double S = sc[kDsS], N = sc[kDsN] + 1e-15;
+  const double dNx = -2.0 * sc[kDsMx], dNy = -2.0 * sc[kDsMy];
+  out[0] = cot[(size_t)e * 3 + 2] * ffdxdy;
+  for (int q = 0; q < 2; ++q) {
+    const double lam = cot[(size_t)e * 3 + q];
+    const double Ar = sc[kDsA2r + 2 * q], Ai = sc[kDsA2i + 2 * q];
+    const double Am = sqrt(Ar * Ar + Ai * Ai);
+    double* f = out + 1 + 5 * q;
+    f[0] = f[1] = f[2] = f[3] = f[4] = 0.0;
+    if (lam == 0.0 || !(Am > 0.0)) continue;
+    double dAxr, dAxi, dAyr, dAyi;
+    if (q == 0) {  // d(z^2)/dx = 2z, d(z^2)/dy = 2iz
+      dAxr = -2.0 * sc[kDsMx];
+      dAxi = -2.0 * sc[kDsMy];
+      dAyr = 2.0 * sc[kDsMy];
+      dAyi = -2.0 * sc[kDsMx];
+    } else {
+      dAxr = -sc[kDsB3xr];
+      dAxi = -sc[kDsB3xi];
+      dAyr = -sc[kDsB3yr];
+      dAyi = -sc[kDsB3yi];
+    }
+    const double cx = (Ar * dAxr + Ai * dAxi) / (Am * N) - Am * dNx / (N * N);
+    const double cy = (Ar * dAyr + Ai * dAyi) / (Am * N) - Am * dNy / (N * N);
+    f[0] = lam * cx / S;
+    f[1] = lam * cy / S;
+    f[2] = lam * Ar / (Am * N);
+    f[3] = lam * Ai / (Am * N);
+    f[4] = lam * Am / (N * N);
+  }
+}
+
+// Cell cotangents G = dL/drho inside each event's source rectangle -- the
+// cells the gather reads (every source window lies in it); kCotBlocks
+// workgroups per event stride over the rectangle's cells.
+constexpr int kCotBlocks = 64;
+__global__ __launch_bounds__(256) void k_diff_cot(DevCfg cfg, EvBufs buf, DiffBufs d, float* G) {
+  __shared__ double cf[kCotStride];
   const int e = blockIdx.y;
   const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
   const double* sc = d.dsc + (size_t)e * kDsStride;
-  if (threadIdx.x == 0) {
-    const double S = sc[kDsS], N = sc[kDsN] + 1e-15;
-    const double dNx = -2.0 * sc[kDsMx], dNy = -2.0 * sc[kDsMy];
-    c0 = cot[(size_t)e * 3 + 2] * ffdxdy;
-    for (int q = 0; q < 2; ++q) {
-      const double lam = cot[(size_t)e * 3 + q];
-      const double Ar = sc[kDsA2r + 2 * q], Ai = sc[kDsA2i + 2 * q];
-      const double Am = sqrt(Ar * Ar + Ai * Ai);
-      double* f = cf[q];
-      f[0] = f[1] = f[2] = f[3] = f[4] = 0.0;
-      if (lam == 0.0 || !(Am > 0.0)) continue;
-      double dAxr, dAxi, dAyr, dAyi;
-      if (q == 0) {  // d(z^2)/dx = 2z, d(z^2)/dy = 2iz
-        dAxr = -2.0 * sc[kDsMx];
-        dAxi = -2.0 * sc[kDsMy];
-        dAyr = 2.0 * sc[kDsMy];
-        dAyi = -2.0 * sc[kDsMx];
-      } else {
-        dAxr = -sc[kDsB3xr];
-        dAxi = -sc[kDsB3xi];
-        dAyr = -sc[kDsB3yr];
-        dAyi = -sc[kDsB3yi];
-      }
-      const double cx = (Ar * dAxr + Ai * dAxi) / (Am * N) - Am * dNx / (N * N);
-      const double cy = (Ar * dAyr + Ai * dAyi) / (Am * N) - Am * dNy / (N * N);
-      f[0] = lam * cx / S;
-      f[1] = lam * cy / S;
-      f[2] = lam * Ar / (Am * N);
-      f[3] = lam * Ai / (Am * N);
-      f[4] = lam * Am / (N * N);
-    }
-  }
+  if (threadIdx.x < kCotStride) cf[threadIdx.x] = d.cotc[(size_t)e * kCotStride + threadIdx.x];
   __syncthreads();
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= (size_t)nx * ny) return;
-  const int ix = (int)(i / ny), iy = (int)(i % ny);
-  const double x = cfg.xmin + ix * cfg.dx - sc[kDsXc], y = cfg.ymin + iy * cfg.dy - sc[kDsYc];
-  const double r2 = x * x + y * y;
-  const bool m = (d.mask[(size_t)e * nx * nwy + (size_t)ix * nwy + (iy >> 5)] >> (iy & 31)) & 1u;
-  double g = c0;
-  g += (cf[0][0] + cf[1][0]) * x + (cf[0][1] + cf[1][1]) * y;
-  if (m && r2 > 0.0) {
-    // g_n(z) = z^n r^(2-n)
-    const double gr = x * x - y * y, gi = 2.0 * x * y;
-    g += cf[0][2] * gr + cf[0][3] * gi - cf[0][4] * r2;
-    if (cf[1][2] != 0.0 || cf[1][3] != 0.0 || cf[1][4] != 0.0) {
-      const double ir = 1.0 / sqrt(r2);
-      const double tr = (gr * x - gi * y) * ir, ti = (gr * y + gi * x) * ir;
-      g += cf[1][2] * tr + cf[1][3] * ti - cf[1][4] * r2;
+  const int rx = buf.counts[(size_t)e * 8 + 6], ry = buf.counts[(size_t)e * 8 + 7];
+  const int x0 = rx & 0xffff, x1 = rx >> 16, y0 = ry & 0xffff, y1 = ry >> 16;
+  const int wy = y1 - y0;
+  const size_t ncr = (x1 > x0 && wy > 0) ? (size_t)(x1 - x0) * wy : 0;
+  const double c0 = cf[0], ax = cf[1] + cf[6], ay = cf[2] + cf[7];
+  const double xc = sc[kDsXc], yc = sc[kDsYc];
+  const bool third = cf[8] != 0.0 || cf[9] != 0.0 || cf[10] != 0.0;
+  for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < ncr; k += (size_t)kCotBlocks * blockDim.x) {
+    const int ix = x0 + (int)(k / wy), iy = y0 + (int)(k % wy);
+    const double x = cfg.xmin + ix * cfg.dx - xc, y = cfg.ymin + iy * cfg.dy - yc;
+    const double r2 = x * x + y * y;
+    const bool m = (d.mask[(size_t)e * nx * nwy + (size_t)ix * nwy + (iy >> 5)] >> (iy & 31)) & 1u;
+    double g = c0;
+    g += ax * x + ay * y;
+    if (m && r2 > 0.0) {
+      // g_n(z) = z^n r^(2-n)
+      const double gr = x * x - y * y, gi = 2.0 * x * y;
+      g += cf[3] * gr + cf[4] * gi - cf[5] * r2;
+      if (third) {
+        const double ir = 1.0 / sqrt(r2);
+        const double tr = (gr * x - gi * y) * ir, ti = (gr * y + gi * x) * ir;
+        g += cf[8] * tr + cf[9] * ti - cf[10] * r2;
+      }
     }
+    G[(size_t)e * nx * ny + (size_t)ix * ny + iy] = (float)g;
   }
-  G[(size_t)e * nx * ny + i] = (float)g;
 }
 
 // One wavefront per source: sums of G E, G E dx, G E dy, G E d^2 over the
@@ -639,9 +656,8 @@ int launch_diff_forward_pre(const DevCfg& cfg, const EvBufs& b, const DiffBufs& 
 int launch_diff_backward(const DevCfg& cfg, const EvBufs& b, const SmearSpec& spec, const DiffBufs& d, int nev,
                          const double* cot_dev, double ffdxdy, float* G, const double* theta, void* stream) {
   hipStream_t st = (hipStream_t)stream;
-  const size_t ncell = (size_t)cfg.nx * cfg.ny;
-  hipLaunchKernelGGL(k_diff_cot, dim3((unsigned)((ncell + 255) / 256), nev), dim3(256), 0, st, cfg, d, cot_dev,
-                     ffdxdy, G);
+  hipLaunchKernelGGL(k_diff_cot_coef, dim3((unsigned)((nev + 63) / 64)), dim3(64), 0, st, d, cot_dev, ffdxdy, nev);
+  hipLaunchKernelGGL(k_diff_cot, dim3(kCotBlocks, nev), dim3(256), 0, st, cfg, b, d, G);
   const unsigned nev8 = (unsigned)((nev + 7) / 8 * 8);  // k_diff_gather: whole groups of 8 events (one per XCD)
   hipLaunchKernelGGL(k_diff_gather, dim3(nev8 * kGatherBlocks), dim3(256), 0, st, cfg, b, spec, d, (const float*)G,
                      theta[0], theta[3], nev);

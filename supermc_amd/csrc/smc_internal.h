@@ -131,7 +131,9 @@ struct DiffBufs {
   const double2* gl;  // Gauss-Legendre nodes/weights on [0, 1]
   int ngl;
   double* wstab;      // [kWsTabLen] radial quantiles of the current (R, a) (Newton starts), then Z, Z_R, Z_a
+  double* cotc;       // [ev][kCotStride] coefficients of the cell cotangents (k_diff_cot_coef)
 };
+constexpr int kCotStride = 12;  // c0, then (lam cx/S, lam cy/S, lam Ar/(Am N), lam Ai/(Am N), lam Am/N^2) for n = 2, 3
 constexpr int kWsTab = 2049;              // radial quantile nodes, uniform in u^(1/3)
 constexpr int kWsTabLen = kWsTab + 3;     // + the normalisation and its R, a derivatives
 // per-event scalars of the differentiable moments
