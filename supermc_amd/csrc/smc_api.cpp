@@ -1183,8 +1183,11 @@ void diff_forward(smc_ctx* c, const double* th) {
   t0.done();
   float* grid = grid_buffer(c, &c->grid);
   TimerScope t1(c, SMC_K_SMEAR);
-  check_launch(launch_tile(diff_cfg(c, th), c->buf, diff_spec(c, th), nev, grid, nullptr, c->stream), "k_tile");
+  SmearSpec spec = diff_spec(c, th);
+  spec.sparse = 1;  // k_diff_moments reads each event's source rectangle only
+  check_launch(launch_tile(diff_cfg(c, th), c->buf, spec, nev, grid, nullptr, c->stream), "k_tile");
   c->cm_nu = 0;
+  c->grid_sparse = 1;
   t1.done();
   TimerScope t2(c, SMC_K_MOMENTS);
   const double ffdxdy = c->p.finalFactor * c->p.dx * c->p.dy;

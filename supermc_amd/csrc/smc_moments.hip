@@ -578,10 +578,18 @@ __global__ __launch_bounds__(kGT) void k_diff_moments(DevCfg cfg, EvBufs buf, Di
   const int t = threadIdx.x, lane = t & 63, wv = t >> 6;
   const int nx = cfg.nx, ny = cfg.ny, nwy = (ny + 31) >> 5;
   const float* g = grid + (size_t)e * nx * ny;
+  // the density is zero outside the event's source rectangle (and not
+  // stored there): each thread keeps its cells of the whole grid's stride
+  // pattern inside the rectangle only, so every sum adds the same non-zero
+  // terms in the same order
+  const int rx = buf.counts[(size_t)e * 8 + 6], ry = buf.counts[(size_t)e * 8 + 7];
+  const int x0 = rx & 0xffff, x1 = min(rx >> 16, nx), y0 = ry & 0xffff, y1 = min(ry >> 16, ny);
+  const int ix0 = wv + (x0 > wv ? (x0 - wv + kGW - 1) / kGW * kGW : 0);
+  const int iy0 = lane + (y0 > lane ? (y0 - lane + 63) / 64 * 64 : 0);
   double s1[3] = {0.0, 0.0, 0.0};
-  for (int ix = wv; ix < nx; ix += kGW) {
+  for (int ix = ix0; ix < x1; ix += kGW) {
     const double xg = cfg.xmin + ix * cfg.dx;
-    for (int iy = lane; iy < ny; iy += 64) {
+    for (int iy = iy0; iy < y1; iy += 64) {
       const double v = g[(size_t)ix * ny + iy];
       s1[0] += v;
       s1[1] += xg * v;
@@ -594,10 +602,10 @@ __global__ __launch_bounds__(kGT) void k_diff_moments(DevCfg cfg, EvBufs buf, Di
   double a[11];
 #pragma unroll
   for (int i = 0; i < 11; ++i) a[i] = 0.0;
-  for (int ix = wv; ix < nx; ix += kGW) {
+  for (int ix = ix0; ix < x1; ix += kGW) {
     const double x = cfg.xmin + ix * cfg.dx - xc;
     const unsigned* row = dyn_mask + (size_t)ix * nwy;
-    for (int iy = lane; iy < ny; iy += 64) {
+    for (int iy = iy0; iy < y1; iy += 64) {
       const double v = g[(size_t)ix * ny + iy];
       if (v == 0.0 || !((row[iy >> 5] >> (iy & 31)) & 1u)) continue;
       const double y = cfg.ymin + iy * cfg.dy - yc;
