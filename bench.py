@@ -146,6 +146,37 @@ def config1_params(nev):
         ecc_to_order=5, max_events=nev)
 
 
+class PinnedHost:
+    """Page-locked host arrays from hipHostMalloc of the HIP runtime that
+    libsmc is linked against (loaded by its soname, so the same instance);
+    freed by close()."""
+
+    def __init__(self):
+        import ctypes
+        self._c = ctypes
+        self._hip = ctypes.CDLL("libamdhip64.so.7")
+        self._hip.hipHostMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t, ctypes.c_uint]
+        self._hip.hipHostFree.argtypes = [ctypes.c_void_p]
+        self._ptrs = []
+
+    def array(self, shape, dtype):
+        c = self._c
+        dt = np.dtype(dtype)
+        n = int(np.prod(shape))
+        p = c.c_void_p()
+        rc = self._hip.hipHostMalloc(c.byref(p), max(n * dt.itemsize, 1), 0)
+        if rc != 0 or not p.value:
+            raise RuntimeError("hipHostMalloc of %d bytes failed (%d)" % (n * dt.itemsize, rc))
+        self._ptrs.append(p.value)
+        buf = (c.c_char * (n * dt.itemsize)).from_address(p.value)
+        return np.frombuffer(buf, dtype=dt).reshape(shape)
+
+    def close(self):
+        for p in self._ptrs:
+            self._hip.hipHostFree(p)
+        self._ptrs = []
+
+
 def time_config1(nev, steps, seed):
     """1000 Au+Au events from a fixed seed with the grids as output: each
     step samples, collides, smears and takes eps_n (records to host), then
@@ -155,11 +186,11 @@ def time_config1(nev, steps, seed):
     out = {}
     with Context(config1_params(nev)) as ctx:
         N = ctx.nx * ctx.ny
-        # page-locked host buffers (torch allocates them through HIP), so the
-        # copies run at the link's rate rather than through a staging copy
-        import torch
-        g32 = torch.empty((nev, ctx.nx, ctx.ny), dtype=torch.float32, pin_memory=True).numpy()
-        g64 = torch.empty((nev, ctx.nx, ctx.ny), dtype=torch.float64, pin_memory=True).numpy()
+        # page-locked host buffers from the HIP runtime libsmc runs on, so
+        # the copies run at the link's rate rather than through a staging copy
+        pin = PinnedHost()
+        g32 = pin.array((nev, ctx.nx, ctx.ny), np.float32)
+        g64 = pin.array((nev, ctx.nx, ctx.ny), np.float64)
         ctx.run(seed, 0, nev)
         ctx.density(out=g32)
         ctx.density_f64(out=g64)
@@ -182,6 +213,8 @@ def time_config1(nev, steps, seed):
                for kid, name in ((0, "k_event"), (1, "k_tile"), (2, "k_moments"), (5, "k_grid64"))}
         ev = ctx.fetch_events(nev)
         sq = square_window_cells(ctx.derived.width, ctx.params.dx, -ctx.params.maxx)
+        del g32, g64
+        pin.close()
     terms = float(ev[:, 54].sum()) * sq["box"] + float(ev[:, 55].sum()) * sq["win5"]
     k64 = ker["k_grid64"]
     dev32 = nev / ((ker["k_event"] + ker["k_tile"] + ker["k_moments"]) * 1e-3)
@@ -766,7 +799,7 @@ def main():
         # tile shape chosen on the warm-up batch (smc_autotune; grids do not depend on it)
         best, ms = ctx.autotune(reps=2)
         tune = dict(variant=best, ms_per_variant=[round(float(m), 3) for m in ms],
-                    variants=["32x64 chunk 8", "16x64 chunk 8", "32x64 chunk 4", "64x64 chunk 8"])
+                    variants=["32x64 chunk 8", "16x64 chunk 8", "32x64 chunk 4", "64x64 chunk 8", "64x32 rows interleaved"])
     ctx.set_timing(True)
     t0 = time.perf_counter()
     for k in range(args.steps):
