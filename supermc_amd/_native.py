@@ -31,6 +31,8 @@ EV_EPS_RE, EV_EPS_IM, EV_EPSP_RE, EV_EPSP_IM, EV_RN = 0, 10, 20, 30, 40
 EV_XC, EV_YC, EV_TOTAL, EV_DSDY, EV_NPART, EV_NCOLL, EV_B, EV_STATUS = 50, 51, 52, 53, 54, 55, 56, 57
 EV_SYSTEM = 58
 
+TILE_VARIANTS = 4  # SMC_TILE_VARIANTS
+
 K_EVENT, K_SMEAR, K_MOMENTS, K_KLN = 0, 1, 2, 3
 
 
@@ -368,7 +370,7 @@ class Context:
         """Times every tile shape on the current events and keeps the
         fastest: (best, ms per shape)."""
         best = C.c_int()
-        ms = np.zeros(4)
+        ms = np.zeros(TILE_VARIANTS)
         self._check(self.lib.smc_autotune(self.h, reps, C.byref(best), _ptr(ms, C.c_double)))
         return best.value, ms
 

@@ -31,6 +31,17 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) == set(_native.SIGNATURES), "binding and header disagree"
 
 
+def test_binding_constants_match_header():
+    """Array sizes the binding passes to the library follow the header."""
+    from supermc_amd import _native
+    txt = open(os.path.join(ROOT, "include", "smc.h")).read()
+    want = {"SMC_TILE_VARIANTS": _native.TILE_VARIANTS, "SMC_EV_STRIDE": _native.EV_STRIDE,
+            "SMC_MAX_A": _native.MAX_A}
+    for name, val in want.items():
+        m = re.search(r"#define\s+%s\s+(\d+)" % name, txt)
+        assert m and int(m.group(1)) == val, name
+
+
 def test_struct_layouts():
     from supermc_amd import _native
     lib = _native.load()
