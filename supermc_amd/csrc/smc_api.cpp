@@ -1056,6 +1056,9 @@ void diff_check_params(const smc_ctx* c) {
     throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches share R and a between the nuclei");
   if (!(p.cc_fluctuation_model == 0 || p.cc_fluctuation_model == 6))
     throw SmcError(SMC_ERR_UNSUPPORTED, "cc_fluctuation_model must be 0 or 6");
+  // k_diff_moments holds the whole hot-spot mask of an event in LDS
+  if ((size_t)d.nx * ((d.ny + 31) / 32) * sizeof(unsigned) > 65536)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need nx * ceil(ny / 32) <= 16384 (the LDS mask)");
 }
 
 template <class T>

@@ -12,7 +12,6 @@ namespace smc {
 
 constexpr int kGW = 8;            // waves per workgroup
 constexpr int kGT = kGW * 64;     // threads per workgroup
-constexpr int kGMaxN = 272;       // widest grid of the one-pass tile kernels
 
 // LDS of the moment phase
 struct MomLds {
@@ -234,29 +233,39 @@ __device__ __forceinline__ void event_cm(const DevCfg& cfg, const double* cmp, i
   yc = s2 / s0;
 }
 
-// Grid-sum units of k_mom_cm: one wavefront per (row band, event).
-template <int MAXC>
+// Grid-sum units of k_mom_cm: one wavefront per (row band, event).  SEG:
+// the rectangle is taken in column segments of 64 MAXC (grids wider than
+// one segment); otherwise a single pass over its first 64 MAXC columns.
+template <int MAXC, int SEG>
 __global__ __launch_bounds__(64) void k_mom_cm(DevCfg cfg, EvBufs buf, const float* grid, double* cmp) {
   const int b = blockIdx.x, e = blockIdx.y, lane = threadIdx.x;
   const float* g = grid + (size_t)e * cfg.nx * cfg.ny;
   const Rect R = event_rect(buf, e);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-  float cur[MAXC], nxt[MAXC];
-  load_row<MAXC>(g, R.x0 + b, R, cfg.ny, cur);
-  for (int ix = R.x0 + b; ix < R.x1; ix += kNB) {
-    load_row<MAXC>(g, ix + kNB, R, cfg.ny, nxt);
-    double rs = 0.0, ys = 0.0;
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j) {
-      const double d = (double)cur[j];
-      rs += d;
-      ys += (cfg.ymin + (R.y0 + lane + 64 * j) * cfg.dy) * d;
+  for (int cs = R.y0;; cs += 64 * MAXC) {
+    Rect Rs = R;
+    if (SEG) {
+      Rs.y0 = cs;
+      Rs.y1 = min(R.y1, cs + 64 * MAXC);
     }
-    s0 += rs;
-    s1 += (cfg.xmin + ix * cfg.dx) * rs;
-    s2 += ys;
+    float cur[MAXC], nxt[MAXC];
+    load_row<MAXC>(g, Rs.x0 + b, Rs, cfg.ny, cur);
+    for (int ix = Rs.x0 + b; ix < Rs.x1; ix += kNB) {
+      load_row<MAXC>(g, ix + kNB, Rs, cfg.ny, nxt);
+      double rs = 0.0, ys = 0.0;
 #pragma unroll
-    for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+      for (int j = 0; j < MAXC; ++j) {
+        const double d = (double)cur[j];
+        rs += d;
+        ys += (cfg.ymin + (Rs.y0 + lane + 64 * j) * cfg.dy) * d;
+      }
+      s0 += rs;
+      s1 += (cfg.xmin + ix * cfg.dx) * rs;
+      s2 += ys;
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    }
+    if (!SEG || Rs.y1 >= R.y1) break;
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
@@ -420,8 +429,8 @@ struct BandLayout {
   static constexpr int NV = FIX ? 11 + 5 * NORD : 10 + 6 * NORD;
 };
 
-// Pass over one row band of one event.
-template <int NORD, int MAXC, int FIX>
+// Pass over one row band of one event; SEG as in k_mom_cm.
+template <int NORD, int MAXC, int FIX, int SEG>
 __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX ? (NORD > 4 ? 2 : (MAXC <= 5 ? 4 : 3)) : 1, 8))) void k_mom_band(
     DevCfg cfg, EvBufs buf, const float* grid, const double* cmp, int nu, const uint2* rects, const int* nrect,
     double* band) {
@@ -449,29 +458,37 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(FIX ? (NORD 
     for (int o = 0; o < NORD; ++o) M.mre[o] = M.mim[o] = M.mno[o] = M.pre[o] = M.pim[o] = M.pno[o] = 0.0;
   }
   // cell values enter without finalFactor; the sums are scaled once below
-  double yv[MAXC];
-#pragma unroll
-  for (int j = 0; j < MAXC; ++j) yv[j] = cfg.ymin + (R.y0 + lane + 64 * j) * cfg.dy - yc;
-  float cur[MAXC], nxt[MAXC];
-  load_row<MAXC>(g, R.x0 + b, R, ny, cur);
-  for (int lr = 0; lr < nrow; ++lr) {
-    const int ix = R.x0 + b + kNB * lr;
-    load_row<MAXC>(g, ix + kNB, R, ny, nxt);
-    const double x = cfg.xmin + ix * cfg.dx - xc;
-    const unsigned* row = bm + lr * nwy;
-#pragma unroll
-    for (int j = 0; j < MAXC; ++j) {
-      if (cur[j] != 0.f) {
-        const int iy = R.y0 + lane + 64 * j;
-        const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
-        if constexpr (FIX != 0)
-          cell_fixed<FIX, FIX + NORD - 1>(a, x, yv[j], (double)cur[j], masked);
-        else
-          moment_cell<NORD>(M, from, to, (double)cur[j], x, yv[j], masked);
-      }
+  for (int cs = R.y0;; cs += 64 * MAXC) {
+    Rect Rs = R;
+    if (SEG) {
+      Rs.y0 = cs;
+      Rs.y1 = min(R.y1, cs + 64 * MAXC);
     }
+    double yv[MAXC];
 #pragma unroll
-    for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    for (int j = 0; j < MAXC; ++j) yv[j] = cfg.ymin + (Rs.y0 + lane + 64 * j) * cfg.dy - yc;
+    float cur[MAXC], nxt[MAXC];
+    load_row<MAXC>(g, Rs.x0 + b, Rs, ny, cur);
+    for (int lr = 0; lr < nrow; ++lr) {
+      const int ix = Rs.x0 + b + kNB * lr;
+      load_row<MAXC>(g, ix + kNB, Rs, ny, nxt);
+      const double x = cfg.xmin + ix * cfg.dx - xc;
+      const unsigned* row = bm + lr * nwy;
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) {
+        if (cur[j] != 0.f) {
+          const int iy = Rs.y0 + lane + 64 * j;
+          const bool masked = (row[iy >> 5] >> (iy & 31)) & 1u;
+          if constexpr (FIX != 0)
+            cell_fixed<FIX, FIX + NORD - 1>(a, x, yv[j], (double)cur[j], masked);
+          else
+            moment_cell<NORD>(M, from, to, (double)cur[j], x, yv[j], masked);
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < MAXC; ++j) cur[j] = nxt[j];
+    }
+    if (!SEG || Rs.y1 >= R.y1) break;
   }
   if constexpr (!FIX) {
 #pragma unroll
@@ -671,16 +688,17 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
                    const double* cm_units, int nu, double* cm_scratch, double* band, void* rects, int* nrect,
                    void* stream) {
   if (nev <= 0) return 0;
-  if (cfg.nx > kGMaxN * 2 || cfg.ny > kGMaxN * 2) return (int)hipErrorInvalidValue;
   hipStream_t st = (hipStream_t)stream;
   if (mode != 0) {
     hipLaunchKernelGGL(k_grid_sum, dim3(nev), dim3(kGT), 0, st, cfg, b, grid, ev_out);
     return (int)hipGetLastError();
   }
-  const bool narrow = cfg.ny <= 320;
+  // rectangles up to 320 columns in one pass of 5 columns per lane, up to
+  // 576 in one pass of 9, wider ones in segments of 576
+  const int width = cfg.ny <= 320 ? 0 : (cfg.ny <= 64 * 9 ? 1 : 2);
   if (!cm_units) {  // centre-of-mass partials from the grid itself
-    if (narrow) hipLaunchKernelGGL(k_mom_cm<5>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm_scratch);
-    else hipLaunchKernelGGL(k_mom_cm<9>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm_scratch);
+    const int err = launch_cm_partials(cfg, b, grid, nev, cm_scratch, stream);
+    if (err) return err;
     cm_units = cm_scratch;
     nu = kNB;
   }
@@ -689,31 +707,42 @@ int launch_moments(const DevCfg& cfg, const EvBufs& b, const float* grid, int ne
   const int nord = cfg.to_order - cfg.from_order + 1;
   const size_t shm = (size_t)((cfg.nx + kNB - 1) / kNB) * ((cfg.ny + 31) / 32) * sizeof(unsigned);
   const dim3 gb(kNB, nev);
+#define SMC_BAND(NORD_, FIX_)                                                                                       \
+  do {                                                                                                              \
+    if (width == 0)                                                                                                 \
+      hipLaunchKernelGGL((k_mom_band<NORD_, 5, FIX_, 0>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc,      \
+                         nrect, band);                                                                              \
+    else if (width == 1)                                                                                            \
+      hipLaunchKernelGGL((k_mom_band<NORD_, 9, FIX_, 0>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc,      \
+                         nrect, band);                                                                              \
+    else                                                                                                            \
+      hipLaunchKernelGGL((k_mom_band<NORD_, 9, FIX_, 1>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc,      \
+                         nrect, band);                                                                              \
+  } while (0)
   // fixed-order forms for orders 2..5 (the eccentricity table) and 1..9
   // (the parameter file's default, and every range of more than four
   // orders, whose records keep only the requested ones); ranges of up to
   // four other orders take the general form
   if (cfg.from_order == 2 && cfg.to_order == 5) {
-    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, 2>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    else hipLaunchKernelGGL((k_mom_band<4, 9, 2>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    SMC_BAND(4, 2);
     hipLaunchKernelGGL((k_mom_final<4, 2>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
   } else if (nord > 4) {
-    if (narrow) hipLaunchKernelGGL((k_mom_band<9, 5, 1>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    else hipLaunchKernelGGL((k_mom_band<9, 9, 1>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    SMC_BAND(9, 1);
     hipLaunchKernelGGL((k_mom_final<9, 1>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
   } else {
-    if (narrow) hipLaunchKernelGGL((k_mom_band<4, 5, 0>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
-    else hipLaunchKernelGGL((k_mom_band<4, 9, 0>), gb, dim3(64), shm, st, cfg, b, grid, cm_units, nu, rc, nrect, band);
+    SMC_BAND(4, 0);
     hipLaunchKernelGGL((k_mom_final<4, 0>), dim3(nev), dim3(64), 0, st, cfg, b, cm_units, nu, band, ev_out);
   }
+#undef SMC_BAND
   return (int)hipGetLastError();
 }
 
 int launch_cm_partials(const DevCfg& cfg, const EvBufs& b, const float* grid, int nev, double* cm, void* stream) {
   if (nev <= 0) return 0;
   hipStream_t st = (hipStream_t)stream;
-  if (cfg.ny <= 320) hipLaunchKernelGGL(k_mom_cm<5>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
-  else hipLaunchKernelGGL(k_mom_cm<9>, dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
+  if (cfg.ny <= 320) hipLaunchKernelGGL((k_mom_cm<5, 0>), dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
+  else if (cfg.ny <= 64 * 9) hipLaunchKernelGGL((k_mom_cm<9, 0>), dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
+  else hipLaunchKernelGGL((k_mom_cm<9, 1>), dim3(kNB, nev), dim3(64), 0, st, cfg, b, grid, cm);
   return (int)hipGetLastError();
 }
 

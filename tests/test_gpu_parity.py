@@ -140,6 +140,33 @@ def test_order_ranges_match_oracle(orders, dx):
                 assert (rec[:, [n, 10 + n, 20 + n, 30 + n]] == 0).all()
 
 
+@pytest.mark.parametrize("maxx,dx,orders", [(30.0, 0.1, (2, 5)), (13.0, 0.02, (2, 5)), (13.0, 0.02, (1, 9))])
+def test_wide_grids_match_oracle(maxx, dx, orders):
+    """Grids wider than 576 columns, where the moment kernels take each
+    event's rectangle in column segments of 576: 601^2 at 0.1 fm (one
+    segment) and 1301^2 at 0.02 fm (rectangles over 576 columns wide), for
+    orders 2..5 and 1..9, against the oracle on fed positions."""
+    p = H.glauber_params("PbPb", max_events=3, maxx=maxx, maxy=maxx, dx=dx, dy=dx, ecc_from_order=orders[0],
+                         ecc_to_order=orders[1])
+    bvals = np.array([0.0, 8.0, 13.0])
+    xa, xb, bs = H.cpu_configs(p, len(bvals), seed=91, bvals=bvals)
+    with _ctx(p) as ctx:
+        assert ctx.ny > 576
+        ctx.set_positions(xa, xb, bs, fluct_seed=3)
+        ctx.collide(allow=(4,))
+        s, _, _ = ctx.density()
+        rec = ctx.moments()
+        spans = []
+        for e in range(len(bvals)):
+            o = H.oracle_event(ctx, e)
+            assert H.rel_linf(s[e], o["s"]) <= _grid_tol(p), H.rel_linf(s[e], o["s"])
+            _check_event_record(rec[e], o, p)
+            cols = np.nonzero(s[e].any(axis=0))[0]
+            spans.append(cols[-1] - cols[0] + 1)
+        if dx < 0.05:
+            assert max(spans) > 576, spans
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_sampled_events_match_oracle(path):
     p = H.glauber_params("PbPb", max_events=48, smear_path=path)
