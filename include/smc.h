@@ -148,7 +148,8 @@ typedef struct smc_ctx smc_ctx;
 
 /* MCnucl::MCnucl (MCnucl.cpp:22-155) + MakeDensity::MakeDensity set-up
  * (MakeDensity.cpp:21-84): validates parameters, derives sigma/width/dsq,
- * builds the exact footprint tables, allocates the batch on the device. */
+ * builds the exact footprint tables, allocates the batch on the device.
+ * Grids of up to 2048 x 2048 cells (SMC_ERR_ARG above). */
 smc_status smc_create(smc_ctx** ctx, const smc_params* p);
 void smc_destroy(smc_ctx* ctx);
 const char* smc_last_error(const smc_ctx* ctx);

@@ -167,6 +167,24 @@ def test_wide_grids_match_oracle(maxx, dx, orders):
             assert max(spans) > 576, spans
 
 
+def test_largest_grid_runs():
+    """2048 x 2048 cells, the largest grid a context accepts (the band mask
+    of the moment kernel then fills 64 KB of LDS): sampled events run, and
+    their records agree with the same events' records at 0.1 fm, to the
+    resolution difference."""
+    p = H.glauber_params("PbPb", max_events=4, maxx=20.47, maxy=20.47, dx=0.02, dy=0.02)
+    q = H.glauber_params("PbPb", max_events=4, maxx=20.5, maxy=20.5, dx=0.1, dy=0.1)
+    with _ctx(p) as ctx:
+        assert ctx.nx == 2048 and ctx.ny == 2048
+        fine = ctx.run(seed=12, event0=0, nev=4)
+    with _ctx(q) as ctx:
+        coarse = ctx.run(seed=12, event0=0, nev=4)
+    assert (fine[:, 57] == 0).all()
+    assert np.array_equal(fine[:, 54:57], coarse[:, 54:57])  # Npart, Ncoll, b: the same events
+    assert np.all(np.isfinite(fine[:, :53]))
+    np.testing.assert_allclose(fine[:, 2], coarse[:, 2], atol=2e-3)  # eps_2 (real part), grid resolution
+
+
 @pytest.mark.parametrize("path", PATHS)
 def test_sampled_events_match_oracle(path):
     p = H.glauber_params("PbPb", max_events=48, smear_path=path)
