@@ -50,7 +50,8 @@ typedef int smc_status;
 typedef struct smc_params {
   /* model (parameters.dat:6-21) */
   int which_mc_model;          /* 1 MC-KLN, 5 MC-Glauber, 7 sqrt(TA TB) */
-  int sub_model;               /* 1 (Glauber classical), 7 (KLN) */
+  int sub_model;               /* Glauber: 1 (classical), 2 ("Ulrich": pairs only,
+                                  MCnucl.cpp:713-717); 7 (KLN) */
   double lambda;               /* KLN saturation exponent */
   int tmax, tmax_subdivision;  /* KLN table size (MCnucl.cpp:915-919) */
   double alpha;                /* two-component mixing */

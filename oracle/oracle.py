@@ -66,9 +66,9 @@ def lib():
             "orc_collide_bruteforce": (C.c_int, [C.c_int, _dp, _dp, C.c_int, _dp, _dp, C.c_double, _ip, _ip]),
             "orc_thickness": (None, [C.POINTER(OrcGrid), C.c_double, C.c_int, _dp, _dp, _dp]),
             "orc_add_density": (None, [C.POINTER(OrcGrid), C.c_double, C.c_int, _dp, _dp, _dp, _dp]),
-            "orc_density_glauber": (C.c_double, [C.POINTER(OrcGrid), C.c_double, C.c_double,
+            "orc_density_glauber": (C.c_double, [C.POINTER(OrcGrid), C.c_double, C.c_double, C.c_int,
                                                  C.c_int, _dp, _dp, _dp, C.c_int, _dp, _dp, _dp,
-                                                 C.c_int, _dp, _dp, _dp, _dp]),
+                                                 C.c_int, _dp, _dp, _dp, _dp, _dp]),
             "orc_density_trento": (C.c_double, [C.POINTER(OrcGrid), C.c_double, C.c_int, _dp, _dp, _dp,
                                                 C.c_int, _dp, _dp, _dp, _dp]),
             "orc_moments": (C.c_int, [C.POINTER(OrcGrid), _dp, C.c_int, _dp, C.c_int, C.c_int, C.c_int,
@@ -248,7 +248,22 @@ def thickness(g, w, xy):
     return T
 
 
-def density_glauber(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c):
+def uli_weights(pair_a, pair_b, na, nb):
+    """MCnucl.cpp:345-348 (which_mc_model 5, sub_model 2): each pair's
+    additional_weight += 1/n for its projectile and its target nucleon, n
+    that nucleon's number of collisions, in the reference's int arithmetic
+    (Particle.h:64 returns an int), so a nucleon adds 1 exactly when it
+    collided once."""
+    pair_a = np.asarray(pair_a, np.int64)
+    pair_b = np.asarray(pair_b, np.int64)
+    na_coll = np.bincount(pair_a, minlength=na)
+    nb_coll = np.bincount(pair_b, minlength=nb)
+    return (1 // na_coll[pair_a] + 1 // nb_coll[pair_b]).astype(np.float64)
+
+
+def density_glauber(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c, sub_model=1, aw=None):
+    """MCnucl::setDensity, which_mc_model 5 (MCnucl.cpp:689-760): sub_model 1
+    or 2, aw the pairs' additional weights (uli_weights) or None for 0."""
     def cols(xy):
         xy = np.ascontiguousarray(xy, np.float64).reshape(-1, 2)
         return len(xy), np.ascontiguousarray(xy[:, 0]), np.ascontiguousarray(xy[:, 1])
@@ -258,9 +273,10 @@ def density_glauber(g, w, alpha, xy_a, f_a, xy_b, f_b, xy_c, f_c):
     fa = np.ascontiguousarray(f_a, np.float64)
     fb = np.ascontiguousarray(f_b, np.float64)
     fc = np.ascontiguousarray(f_c, np.float64)
+    awp = None if aw is None else np.ascontiguousarray(aw, np.float64)
     s = np.zeros((g.nx, g.ny))
-    dndy = lib().orc_density_glauber(C.byref(g), w, alpha, na, _p(xa), _p(ya), _p(fa), nb, _p(xb), _p(yb), _p(fb),
-                                     nc, _p(xc), _p(yc), _p(fc), _p(s))
+    dndy = lib().orc_density_glauber(C.byref(g), w, alpha, int(sub_model), na, _p(xa), _p(ya), _p(fa), nb, _p(xb),
+                                     _p(yb), _p(fb), nc, _p(xc), _p(yc), _p(fc), None if awp is None else _p(awp), _p(s))
     return s, dndy
 
 

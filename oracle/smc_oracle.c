@@ -256,20 +256,25 @@ void orc_add_density(const orc_grid* g, double w, int n, const double* x,
   }
 }
 
-double orc_density_glauber(const orc_grid* g, double w, double alpha,
+double orc_density_glauber(const orc_grid* g, double w, double alpha, int sub_model,
                            int npa, const double* xa, const double* ya, const double* fa,
                            int npb, const double* xb, const double* yb, const double* fb,
                            int ncoll, const double* xc, const double* yc, const double* fc,
-                           double* s)
+                           const double* aw, double* s)
 {
-  /* MCnucl.cpp:689-760, which_mc_model 5, sub_model 1, shape_of_entropy 2 */
+  /* MCnucl.cpp:689-760, which_mc_model 5, shape_of_entropy 2; sub_model 1
+   * ("classical": wounded term times (1-alpha)/2, :704-712) or 2 ("Ulrich":
+   * no wounded term, :713-717); aw: each pair's additional_weight
+   * (MCnucl.cpp:345-348), NULL for 0 */
   size_t N = (size_t)g->nx * g->ny;
   double* rhop = (double*)calloc(N, sizeof(double));
   double* tab = (double*)calloc(N, sizeof(double));
-  orc_add_density(g, w, npa, xa, ya, fa, rhop);
-  orc_add_density(g, w, npb, xb, yb, fb, rhop);
-  double prefactor = (1.0 - alpha) / 2.;
-  for (size_t c = 0; c < N; c++) rhop[c] = rhop[c] * prefactor;
+  if (sub_model == 1) {
+    orc_add_density(g, w, npa, xa, ya, fa, rhop);
+    orc_add_density(g, w, npb, xb, yb, fb, rhop);
+    double prefactor = (1.0 - alpha) / 2.;
+    for (size_t c = 0; c < N; c++) rhop[c] = rhop[c] * prefactor;
+  }
   double wsq = w * w;
   double dc_sq_max_gaussian = 25. * wsq;
   double d_max = 5. * w;
@@ -277,6 +282,7 @@ double orc_density_glauber(const orc_grid* g, double w, double alpha,
     for (int icoll = 0; icoll < ncoll; icoll++) {
       double x = xc[icoll], y = yc[icoll];
       double fluctfactor = fc ? fc[icoll] : 1.0;
+      double additional_weight = aw ? aw[icoll] : 0.;
       int x_idx_left = (int)((x - d_max - g->xmin) / g->dx);
       int x_idx_right = (int)((x + d_max - g->xmin) / g->dx);
       int y_idx_left = (int)((y - d_max - g->ymin) / g->dy);
@@ -292,7 +298,7 @@ double orc_density_glauber(const orc_grid* g, double w, double alpha,
           double dc = (x - xg) * (x - xg) + (y - yg) * (y - yg);
           if (dc <= dc_sq_max_gaussian)
             tab[(size_t)ir * g->ny + jr] += fluctfactor * (1 / (2 * M_PI * w * w)) *
-                                            exp(-dc / (2 * wsq)) * (alpha + (1. - alpha) * 0.);
+                                            exp(-dc / (2 * wsq)) * (alpha + (1. - alpha) * additional_weight);
         }
       }
     }
@@ -839,8 +845,8 @@ int orc_run_events(const orc_event_cfg* c, uint64_t seed, int nev, double* eps2_
     memset(TB, 0, N * sizeof(double));
     orc_thickness(&g, c->w, npa, pxa, pya, TA);
     orc_thickness(&g, c->w, npb, pxb, pyb, TB);
-    orc_density_glauber(&g, c->w, c->alpha, npa, pxa, pya, pfa, npb, pxb, pyb, pfb,
-                        ncoll, cx, cy, cf, s);
+    orc_density_glauber(&g, c->w, c->alpha, 1, npa, pxa, pya, pfa, npb, pxb, pyb, pfb,
+                        ncoll, cx, cy, cf, NULL, s);
     double mr[10], mi[10], mpr[10], mpi[10], rn[10], cm[3];
     orc_moments(&g, s, npa + npb, boxes, ncoll, c->from_order, c->to_order, mr, mi, mpr, mpi, rn, cm);
     if (eps2_out) {

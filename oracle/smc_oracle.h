@@ -71,11 +71,11 @@ void orc_add_density(const orc_grid* g, double w, int n, const double* x,
 /* MCnucl.cpp:626-820, which_mc_model 5 / sub_model 1, shape_of_entropy 2.
  * Participants of A and B (any order) with fluct factors, pair midpoints
  * with fluct factors.  s is written (not accumulated).  Returns dndy = sum s. */
-double orc_density_glauber(const orc_grid* g, double w, double alpha,
+double orc_density_glauber(const orc_grid* g, double w, double alpha, int sub_model,
                            int npa, const double* xa, const double* ya, const double* fa,
                            int npb, const double* xb, const double* yb, const double* fb,
                            int ncoll, const double* xc, const double* yc, const double* fc,
-                           double* s);
+                           const double* aw, double* s);
 
 /* MCnucl.cpp:762-811, which_mc_model 7: s = sqrt(rho_A rho_B). */
 double orc_density_trento(const orc_grid* g, double w,

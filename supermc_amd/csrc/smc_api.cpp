@@ -273,8 +273,8 @@ void validate_and_derive(smc_ctx* c) {
   smc_derived& d = c->d;
   if (!(p.which_mc_model == 5 || p.which_mc_model == 7 || p.which_mc_model == 1))
     throw SmcError(SMC_ERR_UNSUPPORTED, "which_mc_model must be 1, 5 or 7");
-  if (p.which_mc_model == 5 && p.sub_model != 1)
-    throw SmcError(SMC_ERR_UNSUPPORTED, "MC-Glauber sub_model 2 is not on this path");
+  if (p.which_mc_model == 5 && p.sub_model != 1 && p.sub_model != 2)
+    throw SmcError(SMC_ERR_ARG, "MC-Glauber sub_model must be 1 or 2 (MCnucl.cpp:704-720)");
   if (p.which_mc_model == 1 && p.sub_model != 7)
     throw SmcError(SMC_ERR_UNSUPPORTED, "MC-KLN requires sub_model 7 (KLN uGD)");
   if (!(p.shape_of_nucleons >= 2 && p.shape_of_nucleons <= 4))
@@ -433,6 +433,7 @@ void validate_and_derive(smc_ctx* c) {
   g.alpha = (float)p.alpha;
   g.pref_part = (float)((1.0 - p.alpha) / 2.);
   g.use_binary = p.alpha > 1e-8;
+  g.uli = p.which_mc_model == 5 && p.sub_model == 2;
   g.final_factor = p.finalFactor;
   g.from_order = p.ecc_from_order;
   g.to_order = p.ecc_to_order;
@@ -582,6 +583,7 @@ void alloc_buffers(smc_ctx* c) {
   b.wound = dalloc<int>(E * 2 * kAMax);
   b.pairs = dalloc<unsigned>(E * MP);
   b.pfluct = dalloc<float>(E * MP);
+  b.paw = c->cfg.uli ? dalloc<unsigned char>(E * MP) : nullptr;
   b.counts = dalloc<int>(E * 8);
   b.bval = dalloc<double>(E);
   b.raw = dalloc<int>(E * 2 * kAMax * 2);
@@ -606,7 +608,16 @@ void sync(smc_ctx* c) { HIPCHK(hipStreamSynchronize(c->stream)); }
 SmearSpec density_spec(const smc_ctx* c) {
   const DevCfg& g = c->cfg;
   SmearSpec s{};
-  if (c->p.which_mc_model == 5) {
+  if (c->p.which_mc_model == 5 && g.uli) {
+    // MCnucl.cpp:713-717, 724-760 ("Ulrich" Glauber, sub_model 2): no
+    // participant term; pair k adds f_k (alpha + (1 - alpha) aw_k) G, the
+    // product held in the pair's source weight (k_event)
+    s.enabled[kSrcPair] = g.use_binary;
+    s.win[kSrcPair] = kWin5;
+    s.scale[kSrcPair] = g.norm;
+    s.use_fluct[kSrcPair] = 1;
+    s.combine = 0;
+  } else if (c->p.which_mc_model == 5) {
     // MCnucl.cpp:689-760: (1-alpha)/2 sum_part f T_n + alpha sum_coll f G
     s.enabled[kSrcPartA] = s.enabled[kSrcPartB] = 1;
     s.win[kSrcPartA] = s.win[kSrcPartB] = kWinBox;
@@ -1047,7 +1058,8 @@ void average_profiles(smc_ctx* c, int from, int to, const unsigned char* accept,
 void diff_check_params(const smc_ctx* c) {
   const smc_params& p = c->p;
   const smc_derived& d = c->d;
-  if (p.which_mc_model != 5) throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need which_mc_model 5");
+  if (p.which_mc_model != 5 || p.sub_model != 1)
+    throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need which_mc_model 5, sub_model 1");
   if (d.criterion != 1) throw SmcError(SMC_ERR_UNSUPPORTED, "differentiable batches need the disk criterion (1)");
   for (int nu = 0; nu < 2; ++nu)
     if (c->cfg.nuc[nu].kind != kWoodsSaxon || c->cfg.nuc[nu].deformed)
@@ -1321,7 +1333,7 @@ void smc_destroy(smc_ctx* c) {
   if (c->d_mix_sys) (void)hipFree(c->d_mix_sys);
   if (c->d_mix_of) (void)hipFree(c->d_mix_of);
   for (auto& pe : c->pending) { (void)hipEventDestroy(pe.a); (void)hipEventDestroy(pe.b); }
-  void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.counts,
+  void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.paw, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
                   c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->avg_part, c->avg_ext, c->aux0,
                   c->aux1, c->grid64, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
@@ -1619,6 +1631,7 @@ smc_status smc_density_f64(smc_ctx* c, int ev0, int n, double* s) {
     gc.prefactor = (1.0 - c->p.alpha) / 2.;
     gc.model = c->p.which_mc_model;
     gc.use_binary = c->p.which_mc_model == 5 && c->p.alpha > 1e-8;
+    gc.uli = c->cfg.uli;
     gc.fluct = c->cfg.fluct;
     const size_t N = (size_t)c->d.nx * c->d.ny;
     // events per launch: at most kGrid64Chunk and about 1 GiB of device grids

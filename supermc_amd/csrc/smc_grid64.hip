@@ -9,7 +9,9 @@
 //   participant  r = sqrt((xg-x)^2 + (yg-y)^2); r > 5w -> 0, else
 //                (1/(2 pi w^2)) exp(-r r / (2 w w)), times f
 //   collision    dc = (x-xg)^2 + (y-yg)^2 <= 25 w^2:
-//                f (1/(2 pi w^2)) exp(-dc / (2 w^2)) alpha
+//                f (1/(2 pi w^2)) exp(-dc / (2 w^2)) (alpha + (1 - alpha) aw)
+//                with aw = 0, or for sub_model 2 (MCnucl.cpp:713-717: no
+//                participant term) the pair's weight of MCnucl.cpp:345-348
 // summed per cell in source order (participants of A and B in wound order,
 // then the pairs in pair order), the participant sum scaled by (1-alpha)/2
 // at the end.  The file is built with -ffp-contract=off so that no product
@@ -39,6 +41,7 @@ __global__ __launch_bounds__(256) void k_grid64(DevCfg cfg, EvBufs buf, Grid64Co
   __shared__ double2 lpos[256];
   __shared__ int4 lwin[256];  // x lo, x hi, y lo, y hi
   __shared__ double lf[256];
+  __shared__ double lw[256];  // pair weight: alpha, or alpha + (1 - alpha) aw for sub_model 2
   __shared__ int lacc[256];   // 0: participant sum, 1: collision sum / nucleus B
   __shared__ int lpair[256];  // 1: collision term
   __shared__ int wcount[4];
@@ -54,6 +57,8 @@ __global__ __launch_bounds__(256) void k_grid64(DevCfg cfg, EvBufs buf, Grid64Co
 
   double a0[4] = {0.0, 0.0, 0.0, 0.0}, a1[4] = {0.0, 0.0, 0.0, 0.0};
   const int K = live ? buf.counts[(size_t)e * 8 + 5] : 0;  // uniform over the workgroup
+  const int nsrc_part = buf.counts[(size_t)e * 8] + buf.counts[(size_t)e * 8 + 1];
+  const size_t pbase = (size_t)e * cfg.max_pairs;
   const int4* src = buf.src + (size_t)e * buf.src_cap;
   const int4* swin = buf.swin + (size_t)e * buf.src_cap;
 
@@ -62,7 +67,7 @@ __global__ __launch_bounds__(256) void k_grid64(DevCfg cfg, EvBufs buf, Grid64Co
     bool take = false;
     double2 pos = make_double2(0.0, 0.0);
     int4 win = make_int4(0, 0, 0, 0);
-    double f = 1.0;
+    double f = 1.0, wgt = gc.alpha;
     int acc = 0, pair = 0;
     if (k < K) {
       const int4 s = src[k];
@@ -70,10 +75,15 @@ __global__ __launch_bounds__(256) void k_grid64(DevCfg cfg, EvBufs buf, Grid64Co
       pair = s.w == kSrcPair;
       const int wx = pair ? w4.z : w4.x, wy = pair ? w4.w : w4.y;  // 5w windows for collisions, boxes otherwise
       win = make_int4(wx & 0xffff, wx >> 16, wy & 0xffff, wy >> 16);
-      take = (pair ? gc.use_binary : 1) && win.x < win.y && win.z < win.w && win.x < tx0 + kGX && win.y > tx0 &&
-             win.z < ty0 + kGY && win.w > ty0;
+      take = (pair ? gc.use_binary : !gc.uli) && win.x < win.y && win.z < win.w && win.x < tx0 + kGX &&
+             win.y > tx0 && win.z < ty0 + kGY && win.w > ty0;
       pos = make_double2((double)s.x * kH, (double)s.y * kH);
       f = gc.fluct ? (double)__int_as_float(s.z) : 1.0;
+      if (pair && gc.uli) {  // the pair's Gamma factor and (Alpha + (1 - Alpha) aw) of MCnucl.cpp:755
+        const size_t pk = pbase + (size_t)(k - nsrc_part);
+        f = gc.fluct ? (double)buf.pfluct[pk] : 1.0;
+        wgt = gc.alpha + (1. - gc.alpha) * (double)buf.paw[pk];
+      }
       acc = gc.model == 7 ? (s.w == kSrcPartB) : pair;
     }
     // ordered compaction: the tile's sources in list order
@@ -88,6 +98,7 @@ __global__ __launch_bounds__(256) void k_grid64(DevCfg cfg, EvBufs buf, Grid64Co
       lpos[woff + before] = pos;
       lwin[woff + before] = win;
       lf[woff + before] = f;
+      lw[woff + before] = wgt;
       lacc[woff + before] = acc;
       lpair[woff + before] = pair;
     }
@@ -107,7 +118,7 @@ __global__ __launch_bounds__(256) void k_grid64(DevCfg cfg, EvBufs buf, Grid64Co
           const double ddx = p.x - xg[r], ddy = p.y - yg;
           const double dc = ddx * ddx + ddy * ddy;
           if (dc <= gc.dc_max) {
-            const double v = fq * gc.norm * exp(-dc / gc.two_wsq) * gc.alpha;
+            const double v = fq * gc.norm * exp(-dc / gc.two_wsq) * lw[q];
             if (to1) a1[r] += v; else a0[r] += v;
           }
         }

@@ -87,6 +87,8 @@ struct DevCfg {
   int hb_h;              // box half-width 4*max(w,qw) in half-quanta (rounded)
   float alpha, pref_part;
   int use_binary;        // alpha > 1e-8 (MCnucl.cpp:724)
+  int uli;               // MC-Glauber sub_model 2 (MCnucl.cpp:713-717): pairs only, pair k weighted
+                         // alpha + (1 - alpha) aw_k (MCnucl.cpp:345-348, 750-755)
   double final_factor;
   int from_order, to_order;
   StepTable tab[kNumTables];
@@ -99,6 +101,8 @@ struct EvBufs {
   int* wound;      // [ev][2][kAMax] participant rank (wound order) or -1
   unsigned* pairs; // [ev][max_pairs] proj index | targ index << 16
   float* pfluct;   // [ev][max_pairs]
+  unsigned char* paw;  // [ev][max_pairs] sub_model 2 only: aw = [n_a == 1] + [n_b == 1], n = a nucleon's
+                       // collisions (the reference's integer 1/n, MCnucl.cpp:346-347); null otherwise
   int* counts;     // [ev][8]: npartA, npartB, ncoll, status, attempts, nsrc,
                    //   source rectangle x0 | x1 << 16, y0 | y1 << 16 (cells)
   double* bval;    // [ev]
@@ -178,6 +182,7 @@ struct Grid64Const {
   int model;         // which_mc_model: 5 or 7
   int use_binary;    // alpha > 1e-8
   int fluct;         // Gamma factors on
+  int uli;           // sub_model 2: no participant term, pair weights alpha + (1 - alpha) aw
 };
 constexpr int kGrid64Chunk = 2048;  // most events per k_grid64 launch (device buffer 8 B x cells each)
 int launch_grid64(const DevCfg& cfg, const EvBufs& b, const Grid64Const& gc, int ev0, int nev, double* out,

@@ -453,6 +453,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     if (t < B && fh[t] >= 0) woundB = wrank[t];
   }
   const int off = block_excl_scan(t < A ? rowcnt[t] : 0, scan_tmp);
+  if (cfg.uli) {
+    // sub_model 2: the collisions of each target nucleon (its column of the
+    // hit bitmap), into wrank, which the wound order no longer needs
+    __syncthreads();
+    int n = 0;
+    if (t < B)
+      for (int i = 0; i < A; ++i) n += (hits[i][t >> 5] >> (t & 31)) & 1u;
+    wrank[t] = n;
+    __syncthreads();
+  }
   float fA = 1.f, fB = 1.f;
   if (cfg.fluct) {
     if (woundA >= 0) fA = gamma_draw(cfg.gk_part, cfg.gs_part, key, gev, kStrFluctA, (unsigned)t);
@@ -539,9 +549,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5, 8))) voi
     float fp = 1.f;
     if (cfg.fluct) fp = gamma_draw(cfg.gk_bin, cfg.gs_bin, key, gev, kStrFluctPair, (unsigned)k);
     buf.pfluct[pbase + k] = fp;
+    float wsrc = fp;  // the pair source's weight
+    if (cfg.uli) {
+      // MCnucl.cpp:345-348: additional_weight += 1/n for both nucleons, in
+      // int arithmetic (n >= 1 collisions), so aw = [n_a == 1] + [n_b == 1];
+      // the density weighs the pair by Alpha + (1 - Alpha) aw (:750-755)
+      const int aw = (rowcnt[i] == 1) + (wrank[j] == 1);
+      buf.paw[pbase + k] = (unsigned char)aw;
+      wsrc = fp * (cfg.alpha + (1.f - cfg.alpha) * (float)aw);
+    }
     if (src_ok) {
       const int xh = sx[0][i] + sx[1][j], yh = sy[0][i] + sy[1][j];
-      src[nsrc_part + k] = make_int4(xh, yh, __float_as_int(fp), kSrcPair);
+      src[nsrc_part + k] = make_int4(xh, yh, __float_as_int(wsrc), kSrcPair);
       const int4 w = src_windows(cfg, xh, yh, false);
       swin[nsrc_part + k] = w;
       grow(w, false);

@@ -77,7 +77,10 @@ def oracle_event(ctx, ev, kln_table=None, sizes=None):
     out["TA"] = oracle.thickness(g, w, xa[ia])
     out["TB"] = oracle.thickness(g, w, xb[ib])
     if p.which_mc_model == 5:
-        s, dndy = oracle.density_glauber(g, w, p.alpha, xa[ia], fa, xb[ib], fb, mid, pf.astype(np.float64))
+        aw = oracle.uli_weights(pa, pb, len(xa), len(xb)) if p.sub_model == 2 else None
+        s, dndy = oracle.density_glauber(g, w, p.alpha, xa[ia], fa, xb[ib], fb, mid, pf.astype(np.float64),
+                                         sub_model=p.sub_model, aw=aw)
+        out["aw"] = aw
     elif p.which_mc_model == 7:
         s, dndy = oracle.density_trento(g, w, xa[ia], fa, xb[ib], fb)
     else:

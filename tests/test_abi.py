@@ -62,7 +62,7 @@ def test_defaults_are_reference_parameters_dat():
 
 @pytest.mark.parametrize("override,code", [
     (dict(which_mc_model=3), 7),
-    (dict(which_mc_model=5, sub_model=2, collision_criterion=1), 7),
+    (dict(which_mc_model=5, sub_model=3, collision_criterion=1), 1),
     (dict(which_mc_model=5, sub_model=1, collision_criterion=3), 7),
     (dict(which_mc_model=5, sub_model=1, collision_criterion=4), 7),
     (dict(which_mc_model=5, sub_model=1, collision_criterion=1, Aproj=16), 7),

@@ -117,6 +117,42 @@ def test_fed_positions_full_chain(system, path):
             _check_event_record(rec[e], o, p)
 
 
+@pytest.mark.parametrize("path", PATHS)
+@pytest.mark.parametrize("system", ["AuAu", "PbPb"])
+def test_ulrich_glauber_matches_oracle(system, path):
+    """which_mc_model 5, sub_model 2 (MCnucl.cpp:713-717): no participant
+    term, pair k weighted alpha + (1 - alpha) aw_k (MCnucl.cpp:345-348,
+    750-755).  Pair weights exactly the oracle's rule on the reported pair
+    lists; density grids on every smearing path, FP64 grids and eps_n
+    against the oracle on fed positions."""
+    p = H.glauber_params(system, max_events=8, smear_path=path, sub_model=2)
+    bvals = np.array([0.0, 3.0, 6.0, 9.0, 11.0, 12.5, 13.5, 14.5])
+    xa, xb, bs = H.cpu_configs(p, len(bvals), seed=4321 + p.Aproj, bvals=bvals)
+    with _ctx(p) as ctx:
+        ctx.set_positions(xa, xb, bs, fluct_seed=17)
+        npa, npb, nc, rc = ctx.collide(allow=(4,))
+        s, dndy, _ = ctx.density()
+        rec = ctx.moments()
+        s64 = ctx.density_f64() if path == 0 else None
+        seen = set()
+        for e in range(len(bvals)):
+            if nc[e] == 0:
+                continue
+            o = H.oracle_event(ctx, e)
+            _check_collision(ctx, e, o)
+            _, kind, wsrc = ctx.sources(e)
+            pa, pb, pf = o["gpu_pairs"]
+            want = (pf.astype(np.float64) * (p.alpha + (1 - p.alpha) * o["aw"])).astype(np.float32)
+            np.testing.assert_allclose(wsrc[kind == 2], want, rtol=2e-7)
+            seen.update(o["aw"].tolist())
+            assert H.rel_linf(s[e], o["s"]) <= _grid_tol(p), H.rel_linf(s[e], o["s"])
+            assert dndy[e] == pytest.approx(o["dndy"], rel=_sum_tol(p))
+            _check_event_record(rec[e], o, p)
+            if s64 is not None:
+                assert H.rel_linf(s64[e], o["s"]) <= 1e-12, H.rel_linf(s64[e], o["s"])
+        assert seen == {0.0, 1.0, 2.0}  # central and peripheral events hold every weight
+
+
 @pytest.mark.parametrize("orders,dx", [((1, 9), 0.1), ((1, 9), 0.05), ((3, 7), 0.1), ((1, 3), 0.1), ((3, 5), 0.05)])
 def test_order_ranges_match_oracle(orders, dx):
     """eps_1..eps_9 (the parameter file's default range, with n = 1's r^3

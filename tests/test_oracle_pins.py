@@ -195,6 +195,45 @@ def test_glauber_density_prefactors(orc):
     assert s.sum() == pytest.approx(dndy, rel=1e-12)
 
 
+def test_ulrich_glauber_pair_weights(orc):
+    """sub_model 2 (MCnucl.cpp:713-717, 345-348, 750-755): no participant
+    term, and pair k weighs alpha + (1 - alpha) aw_k with aw_k = 1/n_a + 1/n_b
+    in int arithmetic.  Pinned by what the rule fixes: an isolated pair
+    (both nucleons collide once) weighs 2 - alpha; the two pairs of a
+    nucleon hit twice, whose partners collide once, weigh 1; three pairs
+    among nucleons that all collide at least twice weigh alpha; at alpha = 1
+    both sub_models give the pair density alone."""
+    g = orc.grid(13.0, 0.1)
+    w, alpha = 0.5, 0.14
+    none = np.zeros((0, 2))
+    part = np.array([[0.3, 0.1], [-0.4, 0.2]])
+    # projectile nucleons a0..a2, target nucleons b0..b3
+    pair_a = np.array([0, 1, 1, 2, 2])
+    pair_b = np.array([0, 1, 2, 3, 3])
+    aw = orc.uli_weights(pair_a, pair_b, 3, 4)
+    assert aw.tolist() == [2.0, 1.0, 1.0, 0.0, 0.0]  # a0-b0 isolated; a1 hit twice; a2-b3 twice each
+    assert orc.uli_weights([0, 0, 1, 1], [0, 1, 0, 1], 2, 2).tolist() == [0.0] * 4
+    pc = np.array([[-3.0, 0.0], [2.0, 1.0], [2.5, -1.0], [-1.0, 4.0], [-1.2, 4.1]])
+    fc = np.array([1.5, 0.25, 1.0, 2.0, 0.5])
+    one = np.ones(len(part))
+    s2, d2 = orc.density_glauber(g, w, alpha, part, one, part, one, pc, fc, sub_model=2, aw=aw)
+    # each pair alone through the sub_model-1 form at alpha = 1 (weight f G)
+    for k in range(len(pc)):
+        gk, _ = orc.density_glauber(g, w, 1.0, none, np.zeros(0), none, np.zeros(0), pc[k:k + 1], fc[k:k + 1])
+        sk, _ = orc.density_glauber(g, w, alpha, part, one, part, one, pc[k:k + 1], fc[k:k + 1], sub_model=2,
+                                    aw=aw[k:k + 1])
+        np.testing.assert_allclose(sk, (alpha + (1 - alpha) * aw[k]) * gk, rtol=1e-13, atol=1e-300)
+    assert aw[0] == 2.0 and (alpha + (1 - alpha) * aw[0]) == pytest.approx(2 - alpha, rel=1e-15)
+    # no participant term: the participants' positions do not enter
+    s2b, _ = orc.density_glauber(g, w, alpha, part + 1.0, one, part - 1.0, one, pc, fc, sub_model=2, aw=aw)
+    assert np.array_equal(s2, s2b)
+    # alpha = 1: both sub_models are the pair density
+    s1a, _ = orc.density_glauber(g, w, 1.0, part, one, part, one, pc, fc)
+    s2a, _ = orc.density_glauber(g, w, 1.0, part, one, part, one, pc, fc, sub_model=2, aw=aw)
+    np.testing.assert_allclose(s2a, s1a, rtol=1e-14, atol=1e-300)
+    assert s2.sum() == pytest.approx(d2, rel=1e-12)
+
+
 def _moments_of(orc, g, pts, w=0.5):
     s, _ = orc.density_glauber(g, w, 0.0, pts, np.ones(len(pts)), np.zeros((0, 2)), np.zeros(0),
                                np.zeros((0, 2)), np.zeros(0))
