@@ -61,8 +61,10 @@ typedef struct smc_params {
   /* collision_criterion (MCnucl.cpp:357-385): 1 disk b^2 <= dsq; 2 Gaussian
    * probability 1 - exp(-sigma_gg e^{-b^2/4w^2} / (4 pi w^2)) per pair whose
    * 8w boxes overlap (GaussianNucleonsCal.cpp:59-67); other values follow
-   * shape_of_entropy (2 -> Gaussian).  3 and 4 (quark / density collisions)
-   * are not on this path. */
+   * shape_of_entropy (2 -> Gaussian, 4..9 -> disk).  3 and 4 (quark /
+   * density collisions) are not on this path.  shape_of_entropy: 2 or 4..9
+   * (Gaussian entropy, MCnucl.cpp:752-756); 1 (disk) and 3 (quarks) are not
+   * on this path. */
   int shape_of_nucleons, collision_criterion, shape_of_entropy;  /* shape 2, 3 or 4 (Gaussian nucleons) */
   double quark_width, gauss_nucl_width;
   /* collision (parameters.dat:64-68) */

@@ -281,11 +281,15 @@ void validate_and_derive(smc_ctx* c) {
     throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_nucleons must be 2, 3 or 4 (Gaussian nucleons)");
   if (p.shape_of_nucleons == 3 && !(p.gaussian_lambda > 0))
     throw SmcError(SMC_ERR_ARG, "shape_of_nucleons 3 needs gaussian_lambda > 0");
-  if (p.shape_of_entropy != 2)
-    throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_entropy must be 2 (Gaussian)");
+  // Gaussian entropy: 2..9 share the Gaussian density (MCnucl.cpp:752-756,
+  // 853-859) and the fluctuation law (:1278-1281, :314); 1 (disk) and 3
+  // (quark fluctuations) are not on this path
+  if (!(p.shape_of_entropy == 2 || (p.shape_of_entropy >= 4 && p.shape_of_entropy <= 9)))
+    throw SmcError(SMC_ERR_UNSUPPORTED, "shape_of_entropy must be 2 or 4..9 (Gaussian entropy)");
   // MCnucl.cpp:363-383: 1 disk, 2 Gaussian; other values follow
-  // shape_of_entropy (2 here, so the Gaussian test)
-  d.criterion = (p.collision_criterion >= 1 && p.collision_criterion <= 4) ? p.collision_criterion : 2;
+  // shape_of_entropy: 2 the Gaussian test, 4..9 the disk
+  d.criterion = (p.collision_criterion >= 1 && p.collision_criterion <= 4) ? p.collision_criterion
+                                                                           : (p.shape_of_entropy == 2 ? 2 : 1);
   if (d.criterion > 2)
     throw SmcError(SMC_ERR_UNSUPPORTED, "collision_criterion 3 and 4 (quark / density collisions) are not on this path");
   if (!(p.cc_fluctuation_model == 0 || p.cc_fluctuation_model == 6))

@@ -5,7 +5,7 @@ The reference draws drand48 per tested pair; the device draws Philox, so
 parity is in law (the draws cannot match):
 - sigma_gg, the tested box and the criterion in effect equal the oracle's
   (sig_eff to 1e-10 relative; 0 selects the Gaussian test through
-  shape_of_entropy 2; 3 and 4 are rejected);
+  shape_of_entropy 2 and the disk through 4..9; 3 and 4 are rejected);
 - on fed Pb+Pb positions, every collision is a pair the reference's sweep
   tests (oracle.gauss_tested_pairs), and the hit counts per impact-parameter
   bin over 64 replicas of each configuration agree with sum P(b) within 5
@@ -48,6 +48,27 @@ def test_derived_sigma_gg_box_and_criterion():
             Context(_params(max_events=4, collision_criterion=bad))
     with Context(default_params(max_events=8)) as ctx:
         assert ctx.derived.criterion == 2
+    # shape_of_entropy 4..9: the Gaussian entropy of 2, and the disk test
+    # when no criterion is forced (MCnucl.cpp:376-383 falls to disk)
+    with Context(_params(max_events=4, collision_criterion=0, shape_of_entropy=5)) as ctx:
+        assert ctx.derived.criterion == 1
+    for bad in (1, 3, 10):
+        with pytest.raises(SmcError):
+            Context(_params(max_events=4, shape_of_entropy=bad))
+
+
+def test_gaussian_entropy_shapes_give_the_same_events():
+    """shape_of_entropy 2 and 4..9 share the density and the fluctuation law
+    (MCnucl.cpp:752-756, 853-859, 1278-1281): with the disk criterion forced,
+    or chosen by the fallback for 4..9, the events are the same bit for bit."""
+    from supermc_amd import Context
+    recs = []
+    for kw in (dict(shape_of_entropy=2, collision_criterion=1), dict(shape_of_entropy=6, collision_criterion=1),
+               dict(shape_of_entropy=9, collision_criterion=0)):
+        with Context(H.glauber_params("PbPb", max_events=64, **kw)) as ctx:
+            recs.append(ctx.run(seed=8, event0=0, nev=64))
+    assert np.array_equal(recs[0], recs[1], equal_nan=True)
+    assert np.array_equal(recs[0], recs[2], equal_nan=True)
 
 
 @pytest.mark.parametrize("lam", [4.14, 2.0])
