@@ -128,16 +128,18 @@ def _read_table(path):
 
 
 @pytest.mark.gpu
-def test_driver_ecc_table_matches_c_abi(tmp_path):
+@pytest.mark.parametrize("sub_model", [1, 2])
+def test_driver_ecc_table_matches_c_abi(tmp_path, sub_model):
     """operation 9: data/sn_ecc_eccp_<n>.dat and _10.dat carry the records of
-    events 0..nev-1 of stream randomSeed, in order, whatever the batch size."""
+    events 0..nev-1 of stream randomSeed, in order, whatever the batch size;
+    MC-Glauber sub_model 1 and 2 (the parameter file's key)."""
     _need(EXE)
-    _write_params(tmp_path)
+    _write_params(tmp_path, sub_model=sub_model)
     r = subprocess.run([EXE, "nev=20"], cwd=tmp_path, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
     from supermc_amd import Context
     import parity_helpers as H
-    p = H.glauber_params("PbPb", ecm=2760.0, max_events=20)
+    p = H.glauber_params("PbPb", ecm=2760.0, max_events=20, sub_model=sub_model)
     with Context(p) as ctx:
         ev = ctx.run(seed=1234, event0=0, nev=20)
     for n in range(2, 6):
