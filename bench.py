@@ -239,8 +239,10 @@ def time_kln_direct(nev, steps, seed):
     """MC-KLN Pb+Pb with dN/dy from the k_T-factorised quadrature in every
     cell (kln_direct = 1): 32 pT x 32 kT x 48 phi Gauss-Legendre nodes, phi
     and 2 pi - phi merged, so 32 x 768 node terms per cell with T_A, T_B > 0.
-    A node term is 3 multiplies, 2 minima and one FMA in FP32: counted as
-    5 flops."""
+    The work unit is the integrand's node term c min(1, a u) min(1, b v)
+    added to the sum, 5 flops as written (four multiplies and the add); the
+    kernel evaluates it as c min(1/a, u) min(1/b, v) with a b taken out per
+    pT node (two minima, a multiply and an FMA)."""
     from supermc_amd import Context
     p = kln_params(nev)
     p.kln_direct = 1

@@ -19,13 +19,16 @@ ARCH = os.environ.get("SMC_OFFLOAD_ARCH", "gfx950")
 DEVICE_FLAGS = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 COMMON = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}", "-I", os.path.join(ROOT, "include")]
 
-KERNEL_SOURCES = ["smc_kernels.hip", "smc_moments.hip", "smc_mtile.hip", "smc_tile.hip", "smc_kln.hip", "smc_diff.hip",
-                  "smc_avg.hip", "smc_p3d.hip", "smc_noalloc.hip", "smc_grid64.hip", "smc_probe.hip"]
+KERNEL_SOURCES = ["smc_kernels.hip", "smc_moments.hip", "smc_mtile.hip", "smc_tile.hip", "smc_kln.hip",
+                  "smc_kln_direct.hip", "smc_diff.hip", "smc_avg.hip", "smc_p3d.hip", "smc_noalloc.hip", "smc_grid64.hip",
+                  "smc_probe.hip"]
 # smc_tile.hip's disk cut relies on v_mul_f32 flushing denormal products
 # (kernel MODE register: f32 denormals flushed on input and output).
 # smc_grid64.hip rounds every FP64 product and sum on its own, as the
-# reference's host doubles do (no fused multiply-adds).
-SOURCE_FLAGS = {"smc_tile.hip": ["-fgpu-flush-denormals-to-zero"], "smc_grid64.hip": ["-ffp-contract=off"]}
+# reference's host doubles do (no fused multiply-adds).  smc_kln_direct.hip
+# takes minima of node values without NaN quieting (finite inputs only).
+SOURCE_FLAGS = {"smc_tile.hip": ["-fgpu-flush-denormals-to-zero"], "smc_grid64.hip": ["-ffp-contract=off"],
+                "smc_kln_direct.hip": ["-fno-honor-nans"]}
 HOST_SOURCES = ["smc_api.cpp", "smc_p3d_api.cpp"]
 
 

@@ -192,6 +192,7 @@ struct smc_ctx {
   void* kln_nodes_f = nullptr;
   void* kln_ptd = nullptr;
   void* kln_nodes_d = nullptr;
+  unsigned* kln_list = nullptr;   // active cells of the per-cell quadrature (an event chunk), then its count
   int kln_npt = 0, kln_nn = 0;
   KlnConst kln_kc{};
   // differentiable batches
@@ -533,7 +534,7 @@ void build_kln(smc_ctx* c) {
   for (int j = 0; j < nphi / 2; ++j) { uf.push_back(xf[j]); ww.push_back(2.0 * wf[j]); }
   if (nphi % 2) { uf.push_back(xf[nphi / 2]); ww.push_back(wf[nphi / 2]); }
   int nn = nkt * (int)uf.size();
-  const int nn_pad = (nn + 1) & ~1;  // the FP32 kernel walks nodes in pairs
+  const int nn_pad = (nn + 15) & ~15;  // the FP32 kernel walks nodes 16 at a time (zero nodes add 0)
   std::vector<double> ptd(2 * npt), nd(4 * (size_t)npt * nn_pad, 0.0);
   std::vector<float> ptf(2 * npt), nf(4 * (size_t)npt * nn_pad, 0.f);
   for (int a = 0; a < npt; ++a) {
@@ -806,9 +807,17 @@ void run_kln_density(smc_ctx* c) {
   const size_t ncell_ev = (size_t)c->d.nx * c->d.ny;
   TimerScope ts(c, SMC_K_KLN);
   if (c->p.kln_direct) {
-    check_launch(launch_kln_direct(a, b, s, ncell_ev * c->nev, c->kln_ptf, c->kln_nodes_f, c->kln_npt, c->kln_nn,
-                                   c->kln_kc, c->stream),
-                 "k_kln_direct");
+    // events in chunks of at most 2^28 cells: the active-cell list (32-bit
+    // cell indices) stays within 1 GiB
+    const int chunk = (int)std::max<size_t>(1, std::min<size_t>((size_t)c->cap, ((size_t)1 << 28) / ncell_ev));
+    const size_t cells = (size_t)chunk * ncell_ev;
+    if (!c->kln_list) c->kln_list = dalloc<unsigned>(cells + 1);
+    for (int e0 = 0; e0 < c->nev; e0 += chunk) {
+      const size_t o = (size_t)e0 * ncell_ev, n = (size_t)std::min(chunk, c->nev - e0) * ncell_ev;
+      check_launch(launch_kln_direct(a + o, b + o, s + o, n, c->kln_ptf, c->kln_nodes_f, c->kln_npt, c->kln_nn,
+                                     c->kln_kc, c->kln_list, c->kln_list + cells, c->stream),
+                   "k_kln_direct");
+    }
   } else {
     check_launch(launch_kln_table(a, b, s, (int)ncell_ev, c->nev, c->d.ny, c->kln_table, c->d.kln_tmax, c->d.kln_dT,
                                   c->buf.counts, c->stream),
@@ -1340,7 +1349,7 @@ void smc_destroy(smc_ctx* c) {
   void* ptrs[] = {c->buf.pos, c->buf.fluct, c->buf.wound, c->buf.pairs, c->buf.pfluct, c->buf.paw, c->buf.counts,
                   c->buf.bval, c->buf.raw, c->buf.src, c->buf.swin, c->dtab, c->grid, c->ta, c->tb, c->tmp, c->evout,
                   c->cm, c->band, c->mrects, c->nmrect, c->avg_rot, c->avg_acc, c->avg_sum, c->avg_part, c->avg_ext, c->aux0,
-                  c->aux1, c->grid64, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d};
+                  c->aux1, c->grid64, c->spec_src, c->spec_swin, c->spec_counts, c->kln_table, c->kln_ptf, c->kln_nodes_f, c->kln_ptd, c->kln_nodes_d, c->kln_list};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (void* q : c->dif_alloc)

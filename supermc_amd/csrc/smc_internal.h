@@ -285,7 +285,8 @@ int launch_diff_moments(const DevCfg& cfg, const EvBufs& b, const DiffBufs& d, c
 int launch_diff_backward(const DevCfg& cfg, const EvBufs& b, const SmearSpec& spec, const DiffBufs& d, int nev,
                          const double* cot_dev, double ffdxdy, float* G, const double* theta, void* stream);
 int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell, const void* ptf,
-                      const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
+                      const void* nodes, int npt, int nn, const KlnConst& kc, unsigned* list, unsigned* count,
+                      void* stream);  // list: ncell entries, count: one (device scratch of the caller)
 int launch_kln_points(const double* ta, const double* tb, double* out, int n, const void* ptd,
                       const void* nodes, int npt, int nn, const KlnConst& kc, void* stream);
 int launch_kln_table(const float* ta, const float* tb, float* out, int ncell_ev, int nev, int ny,

@@ -12,8 +12,7 @@
 // (phi and 2 pi - phi merged) and per pT node P and g.  A cell then costs
 //   sum_pT P / (alpha(a) alpha(b)) sum_n c_n min(1, a u_n) min(1, b v_n).
 //
-//  k_kln_direct  FP32, one lane per cell, node data read through the scalar
-//                unit (uniform addresses), FP64 only for the per-pT sum.
+//  k_kln_direct  FP32 per cell (smc_kln_direct.hip).
 //  k_kln_points  FP64 for arbitrary (T_A, T_B): KLNModel::getdNdy and the
 //                dN/dy table of MCnucl::makeTable (MCnucl.cpp:911-960).
 //  k_kln_table   MCnucl::setDensity for which_mc_model 1 (MCnucl.cpp:654-687):
@@ -22,44 +21,12 @@
 
 namespace smc {
 
-__device__ __forceinline__ float alpha_s_f(float q2, float lq2, float amax, float inv_beta0) {
-  if (q2 <= lq2) return amax;
-  return fminf(amax, inv_beta0 / logf(q2 / lq2));
-}
 __device__ __forceinline__ double alpha_s_d(double q2, double lq2, double amax, double inv_beta0) {
   if (q2 <= lq2) return amax;
   return fmin(amax, inv_beta0 / log(q2 / lq2));
 }
 
-// node layout: ptd[i] = {P, g} (double), nodes[i * nn + n] = {c, u, v} (float / double)
-__global__ __launch_bounds__(256) void k_kln_direct(const float* ta, const float* tb, float* out, size_t ncell,
-                                                    const float2* ptf, const float4* nodes, int npt, int nn,
-                                                    KlnConst kc) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= ncell) return;
-  const float A = ta[i], B = tb[i];
-  if (!(A > 0.f && B > 0.f)) {
-    out[i] = 0.f;
-    return;
-  }
-  const float lq2 = (float)kc.lq2, amax = (float)kc.amax, ib0 = (float)kc.inv_beta0;
-  double sum = 0.0;
-  for (int p = 0; p < npt; ++p) {
-    const float2 pg = ptf[p];  // {P, g}
-    const float a = A * pg.y, b = B * pg.y;
-    float s0 = 0.f, s1 = 0.f;
-    for (int n = 0; n < nn; n += 2) {
-      const float4 q0 = nodes[(size_t)p * nn + n];
-      const float4 q1 = nodes[(size_t)p * nn + n + 1];
-      s0 = fmaf(q0.x, fminf(1.f, a * q0.y) * fminf(1.f, b * q0.z), s0);
-      s1 = fmaf(q1.x, fminf(1.f, a * q1.y) * fminf(1.f, b * q1.z), s1);
-    }
-    const float inv = 1.f / (alpha_s_f(a, lq2, amax, ib0) * alpha_s_f(b, lq2, amax, ib0));
-    sum += (double)pg.x * (double)((s0 + s1) * inv);
-  }
-  out[i] = (float)sum;
-}
-
+// node layout: ptd[i] = {P, g} (double), nodes[i * nn + n] = {c, u, v} (double)
 __global__ __launch_bounds__(256) void k_kln_points(const double* ta, const double* tb, double* out, int n,
                                                     const double2* ptd, const double4* nodes, int npt, int nn,
                                                     KlnConst kc) {
@@ -118,14 +85,6 @@ __global__ __launch_bounds__(256) void k_kln_table(const float* ta, const float*
   const double bx = 1.0 / 2.0 * (-3.0 * v00 + 4 * v10 - v20);
   const double by = 1.0 / 2.0 * (-3.0 * v00 + 4 * v01 - v02);
   out[i] = (float)(axx * x * x + axy * x * y + ayy * y * y + bx * x + by * y + v00);
-}
-
-int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell, const void* ptf,
-                      const void* nodes, int npt, int nn, const KlnConst& kc, void* stream) {
-  if (ncell == 0) return 0;
-  hipLaunchKernelGGL(k_kln_direct, dim3((unsigned)((ncell + 255) / 256)), dim3(256), 0, (hipStream_t)stream, ta,
-                     tb, out, ncell, (const float2*)ptf, (const float4*)nodes, npt, nn, kc);
-  return (int)hipGetLastError();
 }
 
 int launch_kln_points(const double* ta, const double* tb, double* out, int n, const void* ptd, const void* nodes,
