@@ -157,3 +157,20 @@ def test_kln_table_overflow_sets_status():
         ctx.density()
         rec = ctx.moments()
     assert ((rec[:, 57].astype(int) & 8) != 0).all()
+
+
+def test_kln_direct_chunked_batch_matches_small_batches():
+    """The per-cell quadrature runs a batch in chunks of at most 2^28 cells
+    (its active-cell list holds 32-bit indices): at 2048^2 that is 64 events,
+    so 130 events take three chunks.  Events on both sides of each chunk
+    edge give bit-identical records when the same events run alone in a
+    batch of four (one chunk)."""
+    kw = dict(kln_direct=1, maxx=20.47, maxy=20.47, dx=0.02, dy=0.02, kln_n_pt=4, kln_n_kt=4, kln_n_phi=8)
+    with _ctx(_kln_params(max_events=130, **kw)) as ctx:
+        assert ctx.nx == 2048 and ctx.ny == 2048
+        big = ctx.run(seed=31, event0=0, nev=130)
+    assert (big[:, 57] == 0).all()
+    with _ctx(_kln_params(max_events=4, **kw)) as ctx:
+        for e0 in (62, 126):
+            small = ctx.run(seed=31, event0=e0, nev=4)
+            assert np.array_equal(small, big[e0:e0 + 4]), e0
