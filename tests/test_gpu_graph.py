@@ -15,10 +15,14 @@ def _ctx(p):
     return Context(p)
 
 
-@pytest.mark.parametrize("model", [5, 1])
+@pytest.mark.parametrize("model", [5, 1, "kln_direct"])
 def test_graph_replays_equal_eager_runs(model):
+    # "kln_direct": MC-KLN by the per-cell quadrature, whose captured
+    # sequence resets the active-cell counter with a kernel
     extra = dict(which_mc_model=1, sub_model=7, ecm=2760.0, lambda_=0.138, kln_n_pt=16, kln_n_kt=16,
-                 kln_n_phi=16, cc_fluctuation_model=0) if model == 1 else {}
+                 kln_n_phi=16, cc_fluctuation_model=0) if model != 5 else {}
+    if model == "kln_direct":
+        extra["kln_direct"] = 1
     p = H.glauber_params("PbPb", max_events=64, **extra)
     with _ctx(p) as g, _ctx(p) as e:
         calls = [(11, 0), (11, 64), (12, 5), (13, 1000), (11, 0)]
