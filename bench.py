@@ -738,8 +738,8 @@ def main():
                     help="run one 256 MiB device copy first (calibration kernel for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(events shard by counter range across GPUs, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py: measures one GPU; --gpus {args.gpus} is not supported "
+              "(across GPUs the events would shard by Philox counter range, with no exchange)", file=sys.stderr)
         sys.exit(2)
 
     from supermc_amd import Context, EV_NPART, EV_NCOLL, EV_STATUS, load
