@@ -34,3 +34,17 @@ def test_run_noalloc_matches_run():
             assert np.array_equal(got[:, :58], want[:, :58]), (seed, event0, nev)
             assert (got[:, 57] == 0).all()
         assert _noalloc(ctx, 1, 0, 2049, dev) == SMC_ERR_CAPACITY
+
+
+def test_run_noalloc_refuses_per_cell_kln():
+    """The per-cell KLN quadrature keeps an active-cell list in context
+    memory, so the allocation-free entry point refuses such a context."""
+    import torch
+    from supermc_amd import Context
+    from supermc_amd._native import SMC_ERR_UNSUPPORTED
+    p = H.glauber_params("PbPb", max_events=8, which_mc_model=1, sub_model=7, ecm=2760.0, lambda_=0.138,
+                         kln_n_pt=8, kln_n_kt=8, kln_n_phi=8, kln_direct=1)
+    with Context(p) as ctx:
+        ctx.run(seed=5, event0=0, nev=8)
+        dev = torch.zeros((8, 64), dtype=torch.float64, device="cuda")
+        assert _noalloc(ctx, 5, 0, 8, dev) == SMC_ERR_UNSUPPORTED
