@@ -176,14 +176,16 @@ def test_order_ranges_match_oracle(orders, dx):
                 assert (rec[:, [n, 10 + n, 20 + n, 30 + n]] == 0).all()
 
 
-@pytest.mark.parametrize("maxx,dx,orders", [(30.0, 0.1, (2, 5)), (13.0, 0.02, (2, 5)), (13.0, 0.02, (1, 9))])
-def test_wide_grids_match_oracle(maxx, dx, orders):
+@pytest.mark.parametrize("maxx,dx,orders,model", [(30.0, 0.1, (2, 5), 5), (13.0, 0.02, (2, 5), 5),
+                                                  (13.0, 0.02, (1, 9), 5), (13.0, 0.02, (1, 9), 7)])
+def test_wide_grids_match_oracle(maxx, dx, orders, model):
     """Grids wider than 576 columns, where the moment kernels take each
     event's rectangle in column segments of 576: 601^2 at 0.1 fm (one
     segment) and 1301^2 at 0.02 fm (rectangles over 576 columns wide), for
-    orders 2..5 and 1..9, against the oracle on fed positions."""
+    orders 2..5 and 1..9, MC-Glauber and sqrt(T_A T_B) (the two-accumulator
+    tile epilogue), against the oracle on fed positions."""
     p = H.glauber_params("PbPb", max_events=3, maxx=maxx, maxy=maxx, dx=dx, dy=dx, ecc_from_order=orders[0],
-                         ecc_to_order=orders[1])
+                         ecc_to_order=orders[1], which_mc_model=model)
     bvals = np.array([0.0, 8.0, 13.0])
     xa, xb, bs = H.cpu_configs(p, len(bvals), seed=91, bvals=bvals)
     with _ctx(p) as ctx:
