@@ -76,8 +76,10 @@ def test_kln_table_entries():
     assert (np.diff(t[1:, 1:], axis=0) > 0).all() and (np.diff(t[1:, 1:], axis=1) > 0).all()
 
 
-def test_kln_table_density_full_chain():
-    p = _kln_params()
+@pytest.mark.parametrize("grid", [{}, dict(maxx=30.0, maxy=30.0)], ids=["261", "601"])
+def test_kln_table_density_full_chain(grid):
+    # 601^2: tiles and moments over grids wider than 576 columns
+    p = _kln_params(**grid)
     xa, xb, bs = H.cpu_configs(p, len(BVALS), seed=321, bvals=BVALS)
     with _ctx(p) as ctx:
         ctx.set_positions(xa, xb, bs)
