@@ -396,10 +396,11 @@ def test_mfma_paths_match_valu_tiles(system, model):
 @pytest.mark.parametrize("model", [5, 7])
 def test_mfma_paths_on_a_wide_grid(model):
     """601^2 (wider than 576 columns): the MFMA contractions (FP32, BF16x3,
-    BF16x6) against the VALU tiles on the same sampled events, grids and
-    records, at the bars of the 261^2 comparison."""
+    BF16x6) and the per-cell exponentials against the VALU tiles on the same
+    sampled events, grids and records, at the bars of the 261^2
+    comparison."""
     grids = {}
-    for path in (0, 1, 3, 4):
+    for path in (0, 1, 2, 3, 4):
         p = H.glauber_params("PbPb", max_events=8, smear_path=path, which_mc_model=model, maxx=30.0, maxy=30.0)
         with _ctx(p) as ctx:
             assert ctx.ny > 576
@@ -407,7 +408,7 @@ def test_mfma_paths_on_a_wide_grid(model):
             s, _, _ = ctx.density()
             grids[path] = (s, ev)
     assert (grids[0][1][:, 57] == 0).all()
-    for path in (1, 3, 4):
+    for path in (1, 2, 3, 4):
         tol = GRID_TOL_BF16X3 if path == 3 else GRID_TOL
         for e in range(8):
             assert H.rel_linf(grids[path][0][e], grids[0][0][e]) <= tol, (path, e)
