@@ -414,3 +414,31 @@ def test_mfma_paths_on_a_wide_grid(model):
             assert H.rel_linf(grids[path][0][e], grids[0][0][e]) <= tol, (path, e)
         etol = 1e-4 if path == 3 else 2e-6
         assert np.abs(grids[path][1][:, :40] - grids[0][1][:, :40]).max() <= etol, path
+
+
+@pytest.mark.parametrize("model", [5, 7])
+@pytest.mark.parametrize("path", PATHS)
+def test_rectangular_grid_and_cells_match_oracle(path, model):
+    """maxx != maxy and dx != dy, which the reference keeps apart
+    (MCnucl.cpp:39-40, 460; the hot-spot bound that divides a y extent by
+    dx, MakeDensity.cpp:2357): 261 x 226 cells of 0.1 x 0.08 fm, Au+Au on
+    fed positions, every smearing path, against the oracle."""
+    p = H.glauber_params("AuAu", max_events=4, smear_path=path, which_mc_model=model, maxx=13.0, maxy=9.0, dx=0.1,
+                         dy=0.08)
+    bvals = np.array([0.0, 4.0, 8.0, 11.0])
+    xa, xb, bs = H.cpu_configs(p, len(bvals), seed=63, bvals=bvals)
+    with _ctx(p) as ctx:
+        assert (ctx.nx, ctx.ny) == (261, 226)
+        ctx.set_positions(xa, xb, bs)
+        ctx.collide(allow=(4,))
+        s, _, _ = ctx.density()
+        rec = ctx.moments()
+        checked = 0
+        for e in range(len(bvals)):
+            o = H.oracle_event(ctx, e)
+            if o["col"]["ncoll"] == 0:
+                continue
+            assert H.rel_linf(s[e], o["s"]) <= _grid_tol(p), (e, H.rel_linf(s[e], o["s"]))
+            _check_event_record(rec[e], o, p)
+            checked += 1
+        assert checked >= 3
