@@ -76,9 +76,11 @@ def test_kln_table_entries():
     assert (np.diff(t[1:, 1:], axis=0) > 0).all() and (np.diff(t[1:, 1:], axis=1) > 0).all()
 
 
-@pytest.mark.parametrize("grid", [{}, dict(maxx=30.0, maxy=30.0)], ids=["261", "601"])
+@pytest.mark.parametrize("grid", [{}, dict(maxx=30.0, maxy=30.0), dict(maxy=9.0, dy=0.08)],
+                         ids=["261", "601", "261x226"])
 def test_kln_table_density_full_chain(grid):
-    # 601^2: tiles and moments over grids wider than 576 columns
+    # 601^2: tiles and moments over grids wider than 576 columns; 261 x 226:
+    # rectangular cells of 0.1 x 0.08 fm
     p = _kln_params(**grid)
     xa, xb, bs = H.cpu_configs(p, len(BVALS), seed=321, bvals=BVALS)
     with _ctx(p) as ctx:
