@@ -43,11 +43,13 @@ def test_fp64_grids_auau_minbias_match_oracle():
         _check_batch(ctx, 24)
 
 
+@pytest.mark.parametrize("grid", [{}, dict(maxy=9.0, dy=0.08)], ids=["261", "261x226"])
 @pytest.mark.parametrize("model", [5, 7])
-def test_fp64_grids_pbpb_fluct_fed_positions(model):
+def test_fp64_grids_pbpb_fluct_fed_positions(model, grid):
     """Pb+Pb with Gamma factors on fed positions at b = 0 .. 14.5 fm, the
-    two-component model and sqrt(T_A T_B)."""
-    p = H.glauber_params("PbPb", max_events=8, which_mc_model=model)
+    two-component model and sqrt(T_A T_B), on 0.1 fm cells and on
+    rectangular 0.1 x 0.08 fm cells."""
+    p = H.glauber_params("PbPb", max_events=8, which_mc_model=model, **grid)
     bvals = np.array([0.0, 2.0, 4.5, 7.0, 9.0, 11.0, 13.0, 14.5])
     xa, xb, bs = H.cpu_configs(p, len(bvals), seed=31 + model, bvals=bvals)
     from supermc_amd import Context

@@ -50,10 +50,12 @@ def _rel(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
+@pytest.mark.parametrize("grid", [{}, dict(maxy=9.0, dy=0.08)], ids=["261", "261x226"])
 @pytest.mark.parametrize("rp", [False, True])
-def test_rotated_average_matches_oracle(rp):
+def test_rotated_average_matches_oracle(rp, grid):
+    # 261 x 226: recentring and turning on rectangular cells (0.1 x 0.08 fm)
     from supermc_amd import Context
-    p = H.glauber_params("PbPb", max_events=6, finalFactor=1.7)
+    p = H.glauber_params("PbPb", max_events=6, finalFactor=1.7, **grid)
     bvals = [0.0, 2.0, 4.5, 7.0, 9.5, 12.0]
     xa, xb, bs = H.cpu_configs(p, len(bvals), 41, bvals)
     with Context(p) as ctx:
