@@ -90,7 +90,8 @@ def test_smear_matches_oracle_on_reference_grid():
 def test_smear_batch_ragged_and_empty_events():
     rng = np.random.default_rng(12)
     srcs = [_random_sources(rng, 9), np.zeros((0, 6)), _random_sources(rng, 70, edge=True), _random_sources(rng, 1)]
-    with _p3d(grid_nx=81, grid_ny=91, grid_neta=31, grid_deta=0.2) as P:
+    # also a grid of unequal sizes and spacings in x and y
+    with _p3d(grid_nx=81, grid_ny=91, grid_neta=31, grid_dy=0.12, grid_deta=0.2) as P:
         rho = P.smear(srcs)
         for e, s in enumerate(srcs):
             ref = _oracle_rho(P, s)
