@@ -33,8 +33,8 @@ __device__ __forceinline__ float kln_direct_cell(float A, float B, const float2*
     const float a = A * pg.y, b = B * pg.y;
     const float ia = 1.f / a, ib = 1.f / b;
     // sixteen nodes per step (the table is padded to a multiple of 16 with
-    // zero nodes), four sums: the scalar loads of a step go out together
-    // node offsets kept wave-uniform (scalar loads through the scalar unit)
+    // zero nodes) into four sums; the node offsets are wave-uniform, so a
+    // step's loads go out together through the scalar unit
     const int n0 = __builtin_amdgcn_readfirstlane(p * nn);
     float s[4] = {0.f, 0.f, 0.f, 0.f};
     for (int n = n0; n < n0 + nn; n = __builtin_amdgcn_readfirstlane(n + 16)) {
@@ -129,12 +129,13 @@ int launch_kln_direct(const float* ta, const float* tb, float* out, size_t ncell
                       void* stream) {
   if (ncell == 0) return 0;
   if (ncell > 0xffffffffull) return (int)hipErrorInvalidValue;  // 32-bit cell indices
+  if (nn <= 0 || nn % 16 != 0) return (int)hipErrorInvalidValue;  // the node loop's 16-node steps
   hipStream_t st = (hipStream_t)stream;
   hipLaunchKernelGGL(k_kln_count_reset, dim3(1), dim3(1), 0, st, count);
   hipLaunchKernelGGL(k_kln_active, dim3((unsigned)((ncell + KLN_ACT_CELLS - 1) / KLN_ACT_CELLS)), dim3(256), 0, st,
                      ta, tb, out, ncell, list, count);
-  // 32 workgroups of 256 lanes per CU (four times the lanes the CUs hold at
-  // once), or fewer for a small batch
+  // 32 workgroups of 256 lanes per CU (several times the waves a CU holds
+  // at once), or fewer for a small batch
   const unsigned blocks = (unsigned)std::min<size_t>((ncell + 255) / 256, 8192);
   hipLaunchKernelGGL(k_kln_direct, dim3(blocks), dim3(256), 0, st, ta, tb, out, list, count, (const float2*)ptf,
                      (const float4*)nodes, npt, nn, kc);
